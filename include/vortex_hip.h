@@ -1,0 +1,197 @@
+/*
+ * libvortex_hip — MI355X (gfx950) commit pipeline for VortexSTARK's
+ * Fibonacci prover: circle LDE -> Blake2s Merkle commit -> FRI folding.
+ *
+ * C ABI only: plain pointers, sizes and an opaque stream pointer.
+ *
+ * Part 1 exports the kernel entry points that VortexSTARK's Rust FFI block
+ * binds (src/cuda/ffi.rs). Names, argument order, argument meaning and the
+ * host-pointer conventions are identical, so that block links against this
+ * library unchanged. Each declaration cites the ffi.rs line it replaces.
+ *
+ * Part 2 replaces the CUDA runtime calls that ffi.rs/device/buffer.rs make
+ * (cudaMalloc, cudaMemcpy, streams, ...). These are vx_* functions, not
+ * cuda* look-alikes: the Rust side binds them under their own names.
+ *
+ * Part 3 is the fused device-resident prover (no per-layer host round trips).
+ * It produces the same KRAK v1 proof bytes as the reference's prove() +
+ * serialize_proof (src/prover.rs:1721-1978, src/bin/stark_cli.rs:907-975).
+ *
+ * Conventions shared by every entry point:
+ *  - device buffers are allocated by the caller; kernels never allocate,
+ *    except the vx_prover_* objects, which own their workspace;
+ *  - "(host)" arguments are read before the call returns;
+ *  - launches go to the calling thread's current stream (vx_set_stream,
+ *    default: the null stream) unless a stream argument is given;
+ *  - functions the reference ends with a device synchronisation (the
+ *    cuda_circle_ntt_* family) are complete when they return; the rest are
+ *    stream ordered;
+ *  - errors never abort: the first failure is kept per thread and read
+ *    with vx_get_last_error() / vx_get_last_error_message().
+ */
+#ifndef VORTEX_HIP_H
+#define VORTEX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================== Part 1: reference kernel ABI ====================== */
+
+/* Elementwise M31 ops (ffi.rs:212-213, cuda/fields.cu:32-43). */
+void cuda_m31_add(const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n);
+void cuda_m31_mul(const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n);
+/* Device-wide synchronisation (ffi.rs:214). */
+void cuda_device_sync(void);
+
+/* Native circle NTT (ffi.rs:243-293, cuda/circle_ntt.cu:181-334).
+ * h_layer_offsets / h_layer_sizes are (host) arrays of n_line_layers words. */
+void cuda_circle_ntt_evaluate(uint32_t* d_data, const uint32_t* d_twiddles, const uint32_t* d_circle_twids,
+                              const uint32_t* h_layer_offsets, const uint32_t* h_layer_sizes,
+                              uint32_t n_line_layers, uint32_t n);                                 /* ffi.rs:243 */
+void cuda_circle_ntt_interpolate(uint32_t* d_data, const uint32_t* d_itwiddles, const uint32_t* d_circle_itwids,
+                                 const uint32_t* h_layer_offsets, const uint32_t* h_layer_sizes,
+                                 uint32_t n_line_layers, uint32_t n);                              /* ffi.rs:253 */
+void cuda_circle_ntt_evaluate_batch(uint32_t** d_columns, const uint32_t* d_twiddles,
+                                    const uint32_t* d_circle_twids, const uint32_t* h_layer_offsets,
+                                    const uint32_t* h_layer_sizes, uint32_t n_line_layers, uint32_t n,
+                                    uint32_t n_cols);                                              /* ffi.rs:263 */
+void cuda_circle_ntt_interpolate_batch(uint32_t** d_columns, const uint32_t* d_itwiddles,
+                                       const uint32_t* d_circle_itwids, const uint32_t* h_layer_offsets,
+                                       const uint32_t* h_layer_sizes, uint32_t n_line_layers, uint32_t n,
+                                       uint32_t n_cols);                                           /* ffi.rs:274 */
+void cuda_circle_ntt_layer(uint32_t* d_data, const uint32_t* d_twiddles, uint32_t layer_idx, uint32_t n,
+                           int32_t forward);                                                       /* ffi.rs:285 */
+void cuda_bit_reverse_m31(uint32_t* data, uint32_t log_n);                                         /* ffi.rs:293 */
+
+/* FRI folds, SoA QM31 columns (ffi.rs:307-323, cuda/fri.cu:82-121).
+ * alpha / alpha_sq are (host) arrays of 4 words. */
+void cuda_fold_line_soa(const uint32_t* in0, const uint32_t* in1, const uint32_t* in2, const uint32_t* in3,
+                        const uint32_t* twiddles, uint32_t* out0, uint32_t* out1, uint32_t* out2,
+                        uint32_t* out3, const uint32_t* alpha, uint32_t half_n);                   /* ffi.rs:307 */
+void cuda_fold_circle_into_line_soa(uint32_t* dst0, uint32_t* dst1, uint32_t* dst2, uint32_t* dst3,
+                                    const uint32_t* src0, const uint32_t* src1, const uint32_t* src2,
+                                    const uint32_t* src3, const uint32_t* twiddles, const uint32_t* alpha,
+                                    const uint32_t* alpha_sq, uint32_t half_n);                    /* ffi.rs:315 */
+
+/* Twiddles (ffi.rs:327-363, cuda/twiddles.cu, cuda/batch_inverse.cu). */
+void cuda_compute_fold_twiddle_sources(uint32_t initial_x, uint32_t initial_y, uint32_t step_x, uint32_t step_y,
+                                       uint32_t* output, uint32_t n, uint32_t log_n,
+                                       int32_t extract_y);                                         /* ffi.rs:327 */
+void cuda_batch_inverse_m31(const uint32_t* input, uint32_t* output, uint32_t n);                  /* ffi.rs:335 */
+void cuda_compute_fold_twiddle_sources_stream(uint32_t initial_x, uint32_t initial_y, uint32_t step_x,
+                                              uint32_t step_y, uint32_t* output, uint32_t n, uint32_t log_n,
+                                              int32_t extract_y, void* stream);                    /* ffi.rs:337 */
+void cuda_batch_inverse_m31_stream(const uint32_t* input, uint32_t* output, uint32_t n, void* stream); /* :346 */
+void cuda_compute_coset_points(uint32_t initial_x, uint32_t initial_y, uint32_t step_x, uint32_t step_y,
+                               uint32_t* output_x, uint32_t* output_y, uint32_t n);                /* ffi.rs:348 */
+void cuda_squash_x(const uint32_t* input, uint32_t* output, uint32_t out_n);                        /* ffi.rs:355 */
+void cuda_extract_and_squash(const uint32_t* input, uint32_t* twiddle_out, uint32_t* squash_out,
+                             uint32_t half_n);                                                     /* ffi.rs:357 */
+
+/* Fibonacci quotient and zero pad (ffi.rs:367-396, cuda/constraint.cu:94-153).
+ * alpha is a (host) array of 4 words. */
+void cuda_fibonacci_quotient(const uint32_t* trace, uint32_t* out0, uint32_t* out1, uint32_t* out2,
+                             uint32_t* out3, const uint32_t* alpha, uint32_t n);                   /* ffi.rs:367 */
+void cuda_zero_pad(const uint32_t* src, uint32_t* dst, uint32_t src_n, uint32_t dst_n);             /* ffi.rs:374 */
+void cuda_fibonacci_quotient_chunk(const uint32_t* trace, uint32_t* out0, uint32_t* out1, uint32_t* out2,
+                                   uint32_t* out3, const uint32_t* alpha, uint32_t offset, uint32_t chunk_n,
+                                   uint32_t global_n);                                             /* ffi.rs:381 */
+void cuda_fibonacci_quotient_chunk_stream(const uint32_t* trace, uint32_t* out0, uint32_t* out1,
+                                          uint32_t* out2, uint32_t* out3, const uint32_t* alpha,
+                                          uint32_t offset, uint32_t chunk_n, uint32_t global_n,
+                                          void* stream);                                           /* ffi.rs:390 */
+
+/* Blake2s Merkle (ffi.rs:643-697, cuda/blake2s.cu:522-632). Hashes are 8 words. */
+void cuda_merkle_hash_leaves(const uint32_t* const* columns, uint32_t* hashes, uint32_t n_cols,
+                             uint32_t n_leaves);                                                   /* ffi.rs:643 */
+void cuda_merkle_hash_nodes(const uint32_t* children, uint32_t* parents, uint32_t n_parents);      /* ffi.rs:650 */
+void cuda_merkle_reduce_to_root(const uint32_t* nodes, uint32_t* root_out, uint32_t n_nodes);      /* ffi.rs:656 */
+void cuda_merkle_commit_small_soa4(const uint32_t* col0, const uint32_t* col1, const uint32_t* col2,
+                                   const uint32_t* col3, uint32_t* root_out, uint32_t n_leaves);   /* ffi.rs:662 */
+void cuda_merkle_hash_leaves_merge_soa4(const uint32_t* col0, const uint32_t* col1, const uint32_t* col2,
+                                        const uint32_t* col3, uint32_t* parents, uint32_t n_leaves); /* :669 */
+void cuda_merkle_tiled_soa4(const uint32_t* col0, const uint32_t* col1, const uint32_t* col2,
+                            const uint32_t* col3, uint32_t* subtree_roots, uint32_t n_leaves);     /* ffi.rs:676 */
+void cuda_merkle_tiled_soa4_stream(const uint32_t* col0, const uint32_t* col1, const uint32_t* col2,
+                                   const uint32_t* col3, uint32_t* subtree_roots, uint32_t n_leaves,
+                                   void* stream);                                                  /* ffi.rs:683 */
+void cuda_merkle_tiled_generic(const uint32_t* const* columns, uint32_t* subtree_roots, uint32_t n_cols,
+                               uint32_t n_leaves);                                                 /* ffi.rs:691 */
+
+/* ===================== Part 2: runtime (replaces ffi.rs:6-76 cuda* calls) ==== */
+
+int32_t vx_get_last_error(void);                 /* 0 = none; reading clears it */
+const char* vx_get_last_error_message(void);     /* message of the last recorded error */
+int32_t vx_device_count(void);
+int32_t vx_set_device(int32_t device);
+int32_t vx_malloc(void** dev_ptr, size_t bytes);
+int32_t vx_free(void* dev_ptr);
+int32_t vx_malloc_host(void** host_ptr, size_t bytes); /* pinned */
+int32_t vx_free_host(void* host_ptr);
+int32_t vx_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int32_t vx_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int32_t vx_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int32_t vx_memset(void* dev_ptr, int32_t value, size_t bytes);
+int32_t vx_device_synchronize(void);
+int32_t vx_mem_get_info(size_t* free_bytes, size_t* total_bytes);
+int32_t vx_stream_create(void** stream);
+int32_t vx_stream_synchronize(void* stream);
+int32_t vx_stream_destroy(void* stream);
+void vx_set_stream(void* stream);                /* per-thread stream for Part 1 launches */
+void* vx_get_stream(void);
+
+/* ===================== Part 3: fused device-resident prover ================= */
+
+typedef struct vx_prover vx_prover;
+
+/* Builds a prover for traces of 2^log_n rows (blowup 4, eval domain 2^(log_n+2)):
+ * twiddle tables (the reference's ProverCache, prover.rs:143-189) and all
+ * workspace are allocated here once. n_queries / log_last_layer default to the
+ * reference protocol (80, 3) when 0 is passed. Returns NULL on failure. */
+vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer);
+void vx_prover_destroy(vx_prover* p);
+/* Stream the prover's launches go to (default: its own stream). */
+void* vx_prover_stream(vx_prover* p);
+/* Size in bytes of the KRAK v1 proof this prover emits. */
+uint64_t vx_prover_proof_size(const vx_prover* p);
+
+/* Enqueues a full proof of the Fibonacci trace at d_trace (2^log_n M31 words in
+ * device memory, t[0] = a, t[1] = b) on the prover's stream: LDE, trace and
+ * quotient commitments, FRI folds and commitments, Fiat-Shamir transcript,
+ * query sampling and decommitment. Nothing waits for the host; the KRAK bytes
+ * are left in device memory (vx_prover_proof_device). Returns 0 when enqueued. */
+int32_t vx_prover_prove_device(vx_prover* p, const uint32_t* d_trace, uint32_t a, uint32_t b);
+/* Device address of the KRAK v1 bytes of the last proof (valid after the stream completes). */
+const void* vx_prover_proof_device(const vx_prover* p);
+/* Waits for the stream and copies the proof to host memory. Returns bytes written or 0. */
+uint64_t vx_prover_proof_copy(vx_prover* p, void* host_out, uint64_t cap);
+
+/* Convenience: generate the Fibonacci trace on device (air.rs:30-45 semantics) and
+ * prove, returning the proof bytes on the host (prove(a, b, log_n) + serialize_proof). */
+uint64_t vx_prove_fibonacci(vx_prover* p, uint32_t a, uint32_t b, void* host_out, uint64_t cap);
+
+/* Fibonacci trace t0 = a, t1 = b, t(i+2) = t(i) + t(i+1) mod p generated on the device
+ * (chunked by 2x2 matrix powers, air.rs:85-189). */
+int32_t vx_fibonacci_trace(uint32_t a, uint32_t b, uint32_t log_n, uint32_t* d_out, void* stream);
+
+/* Launch timeline: when enabled, the prover records a HIP event on its stream
+ * after every launch of vx_prover_prove_device; entry i > 0 is the time between
+ * marks i-1 and i (labels name the phase and kernel). */
+int32_t vx_prover_set_timing(vx_prover* p, int32_t enable);
+uint32_t vx_prover_timeline_count(const vx_prover* p);
+const char* vx_prover_timeline_label(const vx_prover* p, uint32_t i);
+float vx_prover_timeline_ms(vx_prover* p, uint32_t i);
+
+/* Phase hooks for tests: device addresses of the prover's last LDE (2^(log_n+2)
+ * words) and of FRI layer j (4 SoA columns of 2^(log_n+1-j) words each). */
+const uint32_t* vx_prover_eval_device(const vx_prover* p);
+const uint32_t* vx_prover_fri_layer_device(const vx_prover* p, uint32_t j, uint32_t column);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VORTEX_HIP_H */

@@ -1,0 +1,197 @@
+"""ctypes access to the oracle (oracle/liboracle.so), the CPU restatement used
+as the checker. Test infrastructure only."""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+u32, u64, i32, vp = C.c_uint32, C.c_uint64, C.c_int32, C.c_void_p
+P32 = C.POINTER(C.c_uint32)
+
+
+class ChannelStruct(C.Structure):
+    _fields_ = [("state", C.c_uint8 * 32), ("n_draws", C.c_uint32)]
+
+
+_SIGS = {
+    "orc_set_threads": (None, [u32]),
+    "orc_get_threads": (u32, []),
+    "orc_m31_add": (u32, [u32, u32]),
+    "orc_m31_sub": (u32, [u32, u32]),
+    "orc_m31_mul": (u32, [u32, u32]),
+    "orc_m31_inv": (u32, [u32]),
+    "orc_cm31_mul": (None, [P32, P32, P32]),
+    "orc_qm31_mul": (None, [P32, P32, P32]),
+    "orc_qm31_inv": (None, [P32, P32]),
+    "orc_coset": (None, [i32, u32, P32]),
+    "orc_coset_at": (None, [i32, u32, u64, P32]),
+    "orc_circle_mul": (None, [P32, P32, P32]),
+    "orc_circle_repeated_double": (None, [P32, u32, P32]),
+    "orc_ntt_twiddles": (None, [i32, u32, vp, vp, vp, vp, vp, vp]),
+    "orc_batch_inverse": (None, [vp, vp, u64]),
+    "orc_ntt_layer": (None, [vp, vp, u32, u64, i32]),
+    "orc_ntt_evaluate": (None, [vp, vp, vp, u32]),
+    "orc_ntt_interpolate": (None, [vp, vp, vp, u32]),
+    "orc_lde": (None, [vp, u32, u32, vp]),
+    "orc_blake2s": (None, [vp, u64, vp]),
+    "orc_blake2s_single": (None, [vp, u64, vp]),
+    "orc_blake2s_compress": (None, [P32, P32, u32, u32]),
+    "orc_merkle_leaf_hashes": (None, [vp, u32, u64, vp]),
+    "orc_merkle_node_layer": (None, [vp, u64, vp]),
+    "orc_merkle_root": (None, [vp, u32, u64, P32]),
+    "orc_merkle_verify_path": (i32, [P32, P32, u64, P32, u32]),
+    "orc_fibonacci_trace": (None, [u32, u32, u32, vp]),
+    "orc_fibonacci_quotient": (None, [vp, u64, P32, vp, vp, vp, vp]),
+    "orc_fold_twiddles": (None, [i32, u32, i32, vp]),
+    "orc_fold_line": (None, [vp, vp, P32, u64, vp]),
+    "orc_fold_circle_into_line": (None, [vp, vp, vp, P32, u64]),
+    "orc_channel_init": (None, [C.POINTER(ChannelStruct)]),
+    "orc_channel_mix_digest": (None, [C.POINTER(ChannelStruct), P32]),
+    "orc_channel_mix_felts": (None, [C.POINTER(ChannelStruct), P32, u64]),
+    "orc_channel_draw_felt": (None, [C.POINTER(ChannelStruct), P32]),
+    "orc_channel_draw_number": (u64, [C.POINTER(ChannelStruct), u64]),
+    "orc_prove_fibonacci": (u64, [u32, u32, u32, u32, u32, vp, u64, P32]),
+    "orc_time_prove_fibonacci": (C.c_double, [u32, u32, u32, u32, u32, u32]),
+    "orc_verify_krak": (i32, [vp, u64, u32, u32]),
+    "orc_lifted_row": (u32, [u32, u32, u32]),
+    "orc_splitmix64": (u64, [C.POINTER(u64)]),
+}
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = C.CDLL(ORACLE_SO)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def arr(xs) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(xs, dtype=np.uint32))
+
+
+def p32(a: np.ndarray):
+    return a.ctypes.data_as(P32)
+
+
+def ptr_array(cols):
+    """Array of column pointers (keeps the columns alive through the returned tuple)."""
+    cols = [arr(c) for c in cols]
+    ptrs = (vp * len(cols))(*[c.ctypes.data for c in cols])
+    return ptrs, cols
+
+
+def ntt_twiddles(kind: int, log_n: int):
+    n = 1 << log_n
+    line = np.zeros(max(n - 1, 1), np.uint32)
+    circle = np.zeros(max(n // 2, 1), np.uint32)
+    iline = np.zeros_like(line)
+    icircle = np.zeros_like(circle)
+    offs = np.zeros(max(log_n - 1, 1), np.uint32)
+    sizes = np.zeros_like(offs)
+    lib().orc_ntt_twiddles(kind, log_n, line.ctypes.data, circle.ctypes.data, iline.ctypes.data,
+                           icircle.ctypes.data, offs.ctypes.data, sizes.ctypes.data)
+    return line, circle, iline, icircle, offs, sizes
+
+
+def lde(trace, log_n: int, log_blowup: int = 2) -> np.ndarray:
+    t = arr(trace)
+    out = np.zeros(1 << (log_n + log_blowup), np.uint32)
+    lib().orc_lde(t.ctypes.data, log_n, log_blowup, out.ctypes.data)
+    return out
+
+
+def fibonacci_trace(a: int, b: int, log_n: int) -> np.ndarray:
+    out = np.zeros(1 << log_n, np.uint32)
+    lib().orc_fibonacci_trace(a, b, log_n, out.ctypes.data)
+    return out
+
+
+def prove(a: int, b: int, log_n: int, n_queries: int = 80, log_last: int = 3) -> bytes:
+    L = lib()
+    n = L.orc_prove_fibonacci(a, b, log_n, n_queries, log_last, None, 0, None)
+    if n == 0:
+        raise ValueError("oracle rejected parameters")
+    buf = (C.c_uint8 * n)()
+    L.orc_prove_fibonacci(a, b, log_n, n_queries, log_last, buf, n, None)
+    return bytes(buf)
+
+
+def verify(proof: bytes, n_queries: int = 80, log_last: int = 3) -> int:
+    buf = (C.c_uint8 * len(proof)).from_buffer_copy(proof)
+    return lib().orc_verify_krak(buf, len(proof), n_queries, log_last)
+
+
+def merkle_root(cols) -> np.ndarray:
+    ptrs, keep = ptr_array(cols)
+    root = np.zeros(8, np.uint32)
+    lib().orc_merkle_root(ptrs, len(keep), keep[0].size, p32(root))
+    return root
+
+
+def leaf_hashes(cols) -> np.ndarray:
+    ptrs, keep = ptr_array(cols)
+    out = np.zeros(8 * keep[0].size, np.uint32)
+    lib().orc_merkle_leaf_hashes(ptrs, len(keep), keep[0].size, out.ctypes.data)
+    return out
+
+
+def quotient(eval_, alpha):
+    e = arr(eval_)
+    al = arr(alpha)
+    qs = [np.zeros(e.size, np.uint32) for _ in range(4)]
+    lib().orc_fibonacci_quotient(e.ctypes.data, e.size, p32(al), *[q.ctypes.data for q in qs])
+    return np.stack(qs)
+
+
+def fold_twiddles(kind: int, log_size: int, use_y: bool) -> np.ndarray:
+    out = np.zeros(max(1, (1 << log_size) // 2), np.uint32)
+    lib().orc_fold_twiddles(kind, log_size, 1 if use_y else 0, out.ctypes.data)
+    return out
+
+
+def fold_line(cols4, tw, alpha) -> np.ndarray:
+    src = [arr(c) for c in cols4]
+    half = src[0].size // 2
+    outs = [np.zeros(half, np.uint32) for _ in range(4)]
+    ins = (vp * 4)(*[c.ctypes.data for c in src])
+    ous = (vp * 4)(*[o.ctypes.data for o in outs])
+    t = arr(tw)
+    lib().orc_fold_line(ins, t.ctypes.data, p32(arr(alpha)), half, ous)
+    return np.stack(outs)
+
+
+def fold_circle(cols4, tw, alpha) -> np.ndarray:
+    src = [arr(c) for c in cols4]
+    half = src[0].size // 2
+    dst = [np.zeros(half, np.uint32) for _ in range(4)]
+    ins = (vp * 4)(*[c.ctypes.data for c in src])
+    ous = (vp * 4)(*[o.ctypes.data for o in dst])
+    t = arr(tw)
+    lib().orc_fold_circle_into_line(ous, ins, t.ctypes.data, p32(arr(alpha)), half)
+    return np.stack(dst)
+
+
+def splitmix64_stream(seed: int, count: int):
+    s = u64(seed)
+    return [lib().orc_splitmix64(C.byref(s)) for _ in range(count)]

@@ -1,0 +1,177 @@
+"""vortexstark_amd — MI355X-native commit pipeline for VortexSTARK's prover.
+
+Host-side mirror of the reference's prover interface (src/prover.rs:192-231):
+
+    prove(a, b, log_n)              -> KRAK v1 proof bytes  (prove + serialize_proof)
+    Prover(log_n).prove(a, b)       -> same, with the ProverCache reused
+    DeviceBuffer                    -> device memory owned from Python
+
+Everything runs in libvortex_hip.so (HIP kernels for gfx950 behind a C ABI).
+Importing this package fails if that library has not been built.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _ffi
+
+lib = _ffi.load()
+
+P = 0x7FFFFFFF
+BLOWUP_BITS = 2  # prover.rs:86
+N_QUERIES = 80  # prover.rs:90
+LOG_LAST_LAYER = 3  # fri.rs:265
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def check(what: str = "") -> None:
+    code = lib.vx_get_last_error()
+    if code:
+        msg = lib.vx_get_last_error_message().decode(errors="replace")
+        raise HipError(f"{what}: error {code}: {msg}")
+
+
+def gpu_available() -> bool:
+    return lib.vx_device_count() > 0
+
+
+class DeviceBuffer:
+    """Device allocation of n u32 words (src/device/buffer.rs: DeviceBuffer<u32>)."""
+
+    def __init__(self, n_words: int):
+        self.n = int(n_words)
+        self.ptr = C.c_void_p()
+        if lib.vx_malloc(C.byref(self.ptr), max(4, 4 * self.n)):
+            check("vx_malloc")
+
+    @classmethod
+    def from_host(cls, arr) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr, dtype=np.uint32)
+        b = cls(a.size)
+        if a.size and lib.vx_memcpy_h2d(b.ptr, a.ctypes.data, 4 * a.size):
+            check("vx_memcpy_h2d")
+        return b
+
+    def to_host(self, n: Optional[int] = None) -> np.ndarray:
+        n = self.n if n is None else n
+        out = np.empty(n, dtype=np.uint32)
+        if n and lib.vx_memcpy_d2h(out.ctypes.data, self.ptr, 4 * n):
+            check("vx_memcpy_d2h")
+        return out
+
+    @property
+    def addr(self) -> int:
+        return self.ptr.value or 0
+
+    def free(self) -> None:
+        if self.ptr and self.ptr.value:
+            lib.vx_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_copy_words(addr: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint32)
+    if n and lib.vx_memcpy_d2h(out.ctypes.data, C.c_void_p(addr), 4 * n):
+        check("vx_memcpy_d2h")
+    return out
+
+
+def synchronize() -> None:
+    if lib.vx_device_synchronize():
+        check("vx_device_synchronize")
+
+
+class Prover:
+    """A prover for traces of 2^log_n rows; twiddles and workspace live on the device.
+
+    Mirrors ProverCache + prove_cached (prover.rs:143-209); n_queries and
+    log_last_layer default to the reference protocol (80 queries, last layer 2^3).
+    """
+
+    def __init__(self, log_n: int, n_queries: int = N_QUERIES, log_last_layer: int = LOG_LAST_LAYER):
+        self.log_n = log_n
+        self.n_queries = n_queries
+        self.log_last_layer = log_last_layer
+        self.handle = lib.vx_prover_create(log_n, n_queries, log_last_layer)
+        if not self.handle:
+            check("vx_prover_create")
+            raise HipError("vx_prover_create failed")
+        self.proof_size = int(lib.vx_prover_proof_size(self.handle))
+
+    @property
+    def stream(self) -> int:
+        return lib.vx_prover_stream(self.handle) or 0
+
+    def prove(self, a: int, b: int) -> bytes:
+        """prove(a, b, log_n) + serialize_proof: KRAK v1 bytes."""
+        buf = (C.c_uint8 * self.proof_size)()
+        n = lib.vx_prove_fibonacci(self.handle, a % P, b % P, buf, self.proof_size)
+        if n != self.proof_size:
+            check("vx_prove_fibonacci")
+            raise HipError("vx_prove_fibonacci failed")
+        return bytes(buf)
+
+    def prove_device(self, d_trace_addr: int, a: int, b: int) -> None:
+        """Enqueue a proof of the trace already in device memory (no host wait)."""
+        if lib.vx_prover_prove_device(self.handle, C.c_void_p(d_trace_addr), a, b):
+            check("vx_prover_prove_device")
+
+    def proof_bytes(self) -> bytes:
+        buf = (C.c_uint8 * self.proof_size)()
+        n = lib.vx_prover_proof_copy(self.handle, buf, self.proof_size)
+        if n != self.proof_size:
+            check("vx_prover_proof_copy")
+            raise HipError("vx_prover_proof_copy failed")
+        return bytes(buf)
+
+    def eval_words(self) -> np.ndarray:
+        lib.vx_stream_synchronize(C.c_void_p(self.stream))
+        return device_copy_words(lib.vx_prover_eval_device(self.handle), 1 << (self.log_n + BLOWUP_BITS))
+
+    def fri_layer(self, j: int) -> np.ndarray:
+        """FRI layer j as a (4, size) array of SoA columns."""
+        lib.vx_stream_synchronize(C.c_void_p(self.stream))
+        size = 1 << (self.log_n + BLOWUP_BITS - 1 - j)
+        cols = [device_copy_words(lib.vx_prover_fri_layer_device(self.handle, j, c), size) for c in range(4)]
+        return np.stack(cols)
+
+    def close(self) -> None:
+        if self.handle:
+            lib.vx_prover_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def prove(a: int, b: int, log_n: int, n_queries: int = N_QUERIES, log_last_layer: int = LOG_LAST_LAYER) -> bytes:
+    """prover::prove(a, b, log_n) followed by serialize_proof (KRAK v1 bytes)."""
+    p = Prover(log_n, n_queries, log_last_layer)
+    try:
+        return p.prove(a, b)
+    finally:
+        p.close()
+
+
+def fibonacci_trace_device(a: int, b: int, log_n: int) -> DeviceBuffer:
+    buf = DeviceBuffer(1 << log_n)
+    if lib.vx_fibonacci_trace(a % P, b % P, log_n, buf.ptr, None):
+        check("vx_fibonacci_trace")
+    synchronize()
+    return buf

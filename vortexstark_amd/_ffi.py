@@ -1,0 +1,106 @@
+"""ctypes binding of libvortex_hip.so (declarations in include/vortex_hip.h).
+
+The library is the product: this module only loads it and declares argument
+types. There is no fallback; if the shared library is missing the import of
+the package fails with an error that says how to build it.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvortex_hip.so")
+
+u32 = C.c_uint32
+i32 = C.c_int32
+u64 = C.c_uint64
+sz = C.c_size_t
+vp = C.c_void_p
+P32 = C.POINTER(C.c_uint32)
+
+# name -> (restype, argtypes); mirrors include/vortex_hip.h
+SIGNATURES = {
+    # Part 1: reference kernel ABI (src/cuda/ffi.rs)
+    "cuda_m31_add": (None, [vp, vp, vp, u32]),
+    "cuda_m31_mul": (None, [vp, vp, vp, u32]),
+    "cuda_device_sync": (None, []),
+    "cuda_circle_ntt_evaluate": (None, [vp, vp, vp, P32, P32, u32, u32]),
+    "cuda_circle_ntt_interpolate": (None, [vp, vp, vp, P32, P32, u32, u32]),
+    "cuda_circle_ntt_evaluate_batch": (None, [vp, vp, vp, P32, P32, u32, u32, u32]),
+    "cuda_circle_ntt_interpolate_batch": (None, [vp, vp, vp, P32, P32, u32, u32, u32]),
+    "cuda_circle_ntt_layer": (None, [vp, vp, u32, u32, i32]),
+    "cuda_bit_reverse_m31": (None, [vp, u32]),
+    "cuda_fold_line_soa": (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp, P32, u32]),
+    "cuda_fold_circle_into_line_soa": (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp, P32, P32, u32]),
+    "cuda_compute_fold_twiddle_sources": (None, [u32, u32, u32, u32, vp, u32, u32, i32]),
+    "cuda_batch_inverse_m31": (None, [vp, vp, u32]),
+    "cuda_compute_fold_twiddle_sources_stream": (None, [u32, u32, u32, u32, vp, u32, u32, i32, vp]),
+    "cuda_batch_inverse_m31_stream": (None, [vp, vp, u32, vp]),
+    "cuda_compute_coset_points": (None, [u32, u32, u32, u32, vp, vp, u32]),
+    "cuda_squash_x": (None, [vp, vp, u32]),
+    "cuda_extract_and_squash": (None, [vp, vp, vp, u32]),
+    "cuda_fibonacci_quotient": (None, [vp, vp, vp, vp, vp, P32, u32]),
+    "cuda_zero_pad": (None, [vp, vp, u32, u32]),
+    "cuda_fibonacci_quotient_chunk": (None, [vp, vp, vp, vp, vp, P32, u32, u32, u32]),
+    "cuda_fibonacci_quotient_chunk_stream": (None, [vp, vp, vp, vp, vp, P32, u32, u32, u32, vp]),
+    "cuda_merkle_hash_leaves": (None, [vp, vp, u32, u32]),
+    "cuda_merkle_hash_nodes": (None, [vp, vp, u32]),
+    "cuda_merkle_reduce_to_root": (None, [vp, vp, u32]),
+    "cuda_merkle_commit_small_soa4": (None, [vp, vp, vp, vp, vp, u32]),
+    "cuda_merkle_hash_leaves_merge_soa4": (None, [vp, vp, vp, vp, vp, u32]),
+    "cuda_merkle_tiled_soa4": (None, [vp, vp, vp, vp, vp, u32]),
+    "cuda_merkle_tiled_soa4_stream": (None, [vp, vp, vp, vp, vp, u32, vp]),
+    "cuda_merkle_tiled_generic": (None, [vp, vp, u32, u32]),
+    # Part 2: runtime
+    "vx_get_last_error": (i32, []),
+    "vx_get_last_error_message": (C.c_char_p, []),
+    "vx_device_count": (i32, []),
+    "vx_set_device": (i32, [i32]),
+    "vx_malloc": (i32, [C.POINTER(vp), sz]),
+    "vx_free": (i32, [vp]),
+    "vx_malloc_host": (i32, [C.POINTER(vp), sz]),
+    "vx_free_host": (i32, [vp]),
+    "vx_memcpy_h2d": (i32, [vp, vp, sz]),
+    "vx_memcpy_d2h": (i32, [vp, vp, sz]),
+    "vx_memcpy_d2d": (i32, [vp, vp, sz]),
+    "vx_memset": (i32, [vp, i32, sz]),
+    "vx_device_synchronize": (i32, []),
+    "vx_mem_get_info": (i32, [C.POINTER(sz), C.POINTER(sz)]),
+    "vx_stream_create": (i32, [C.POINTER(vp)]),
+    "vx_stream_synchronize": (i32, [vp]),
+    "vx_stream_destroy": (i32, [vp]),
+    "vx_set_stream": (None, [vp]),
+    "vx_get_stream": (vp, []),
+    # Part 3: fused prover
+    "vx_prover_create": (vp, [u32, u32, u32]),
+    "vx_prover_destroy": (None, [vp]),
+    "vx_prover_stream": (vp, [vp]),
+    "vx_prover_proof_size": (u64, [vp]),
+    "vx_prover_prove_device": (i32, [vp, vp, u32, u32]),
+    "vx_prover_proof_device": (vp, [vp]),
+    "vx_prover_proof_copy": (u64, [vp, vp, u64]),
+    "vx_prove_fibonacci": (u64, [vp, u32, u32, vp, u64]),
+    "vx_fibonacci_trace": (i32, [u32, u32, u32, vp, vp]),
+    "vx_prover_eval_device": (vp, [vp]),
+    "vx_prover_fri_layer_device": (vp, [vp, u32, u32]),
+    "vx_prover_set_timing": (i32, [vp, i32]),
+    "vx_prover_timeline_count": (u32, [vp]),
+    "vx_prover_timeline_label": (C.c_char_p, [vp, u32]),
+    "vx_prover_timeline_ms": (C.c_float, [vp, u32]),
+}
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(
+            f"libvortex_hip.so not found at {path}; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C vortexstark_amd/csrc`"
+        )
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib

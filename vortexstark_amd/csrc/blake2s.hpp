@@ -1,0 +1,116 @@
+// Blake2s-256 compression for gfx950 (RFC 7693; the reference's single-block
+// helper is src/channel.rs:222-298 and its GPU copy cuda/include/blake2s.cuh).
+//
+// All state lives in VGPRs. Rotations lower to v_alignbit_b32 and the
+// three-input adds to v_add3_u32, so one G costs 12 VALU ops and a compression
+// about 960 + 16. The round schedule is written out with compile-time message
+// indices so no message word is ever addressed dynamically.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vx {
+
+constexpr uint32_t B2S_IV0 = 0x6A09E667u, B2S_IV1 = 0xBB67AE85u, B2S_IV2 = 0x3C6EF372u, B2S_IV3 = 0xA54FF53Au;
+constexpr uint32_t B2S_IV4 = 0x510E527Fu, B2S_IV5 = 0x9B05688Cu, B2S_IV6 = 0x1F83D9ABu, B2S_IV7 = 0x5BE0CD19u;
+// Parameter block word 0: digest length 32, key length 0, fanout 1, depth 1.
+constexpr uint32_t B2S_H0 = B2S_IV0 ^ 0x01010020u;
+
+struct Digest {
+    uint32_t w[8];
+};
+
+#define VX_ROTR(x, n) __builtin_rotateright32((x), (n))
+#define VX_G(a, b, c, d, x, y)      \
+    do {                            \
+        a = a + b + (x);            \
+        d = VX_ROTR(d ^ a, 16);     \
+        c = c + d;                  \
+        b = VX_ROTR(b ^ c, 12);     \
+        a = a + b + (y);            \
+        d = VX_ROTR(d ^ a, 8);      \
+        c = c + d;                  \
+        b = VX_ROTR(b ^ c, 7);      \
+    } while (0)
+#define VX_ROUND(m, s0, s1, s2, s3, s4, s5, s6, s7, s8, s9, s10, s11, s12, s13, s14, s15) \
+    do {                                                                                     \
+        VX_G(v0, v4, v8, v12, m[s0], m[s1]);                                                 \
+        VX_G(v1, v5, v9, v13, m[s2], m[s3]);                                                 \
+        VX_G(v2, v6, v10, v14, m[s4], m[s5]);                                                \
+        VX_G(v3, v7, v11, v15, m[s6], m[s7]);                                                \
+        VX_G(v0, v5, v10, v15, m[s8], m[s9]);                                                \
+        VX_G(v1, v6, v11, v12, m[s10], m[s11]);                                              \
+        VX_G(v2, v7, v8, v13, m[s12], m[s13]);                                               \
+        VX_G(v3, v4, v9, v14, m[s14], m[s15]);                                               \
+    } while (0)
+
+// h <- compress(h, m, t, f). m must be indexable with constants only.
+__device__ __forceinline__ void b2s_compress(uint32_t h[8], const uint32_t m[16], uint32_t t, uint32_t f) {
+    uint32_t v0 = h[0], v1 = h[1], v2 = h[2], v3 = h[3], v4 = h[4], v5 = h[5], v6 = h[6], v7 = h[7];
+    uint32_t v8 = B2S_IV0, v9 = B2S_IV1, v10 = B2S_IV2, v11 = B2S_IV3;
+    uint32_t v12 = B2S_IV4 ^ t, v13 = B2S_IV5, v14 = B2S_IV6 ^ f, v15 = B2S_IV7;
+    VX_ROUND(m, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15);
+    VX_ROUND(m, 14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3);
+    VX_ROUND(m, 11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4);
+    VX_ROUND(m, 7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8);
+    VX_ROUND(m, 9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13);
+    VX_ROUND(m, 2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9);
+    VX_ROUND(m, 12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11);
+    VX_ROUND(m, 13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10);
+    VX_ROUND(m, 6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5);
+    VX_ROUND(m, 10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0);
+    h[0] ^= v0 ^ v8;
+    h[1] ^= v1 ^ v9;
+    h[2] ^= v2 ^ v10;
+    h[3] ^= v3 ^ v11;
+    h[4] ^= v4 ^ v12;
+    h[5] ^= v5 ^ v13;
+    h[6] ^= v6 ^ v14;
+    h[7] ^= v7 ^ v15;
+}
+
+__device__ __forceinline__ void b2s_init(uint32_t h[8]) {
+    h[0] = B2S_H0;
+    h[1] = B2S_IV1;
+    h[2] = B2S_IV2;
+    h[3] = B2S_IV3;
+    h[4] = B2S_IV4;
+    h[5] = B2S_IV5;
+    h[6] = B2S_IV6;  // no personalisation on any hash of this path (cuda/blake2s.cu:9-10)
+    h[7] = B2S_IV7;
+}
+
+// Single-block hash of up to 16 words (t = 4 * n_words), the Merkle leaf rule
+// (cuda/blake2s.cu:45-60, merkle.rs:538-556). Words at index >= n_words must be 0.
+__device__ __forceinline__ void b2s_leaf(const uint32_t m[16], uint32_t n_words, uint32_t out[8]) {
+    b2s_init(out);
+    b2s_compress(out, m, 4u * n_words, 0xFFFFFFFFu);
+}
+
+// Merkle node: H(left || right), t = 64 (cuda/blake2s.cu:65-89).
+__device__ __forceinline__ void b2s_node(const uint32_t l[8], const uint32_t r[8], uint32_t out[8]) {
+    uint32_t m[16];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        m[i] = l[i];
+        m[8 + i] = r[i];
+    }
+    b2s_init(out);
+    b2s_compress(out, m, 64u, 0xFFFFFFFFu);
+}
+
+// Leaf of one M31 word (single-column trace tree, t = 4).
+__device__ __forceinline__ void b2s_leaf1(uint32_t v, uint32_t out[8]) {
+    uint32_t m[16] = {v, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    b2s_init(out);
+    b2s_compress(out, m, 4u, 0xFFFFFFFFu);
+}
+
+// Leaf of one QM31 (SoA4 trees, t = 16).
+__device__ __forceinline__ void b2s_leaf4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t out[8]) {
+    uint32_t m[16] = {a, b, c, d, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    b2s_init(out);
+    b2s_compress(out, m, 16u, 0xFFFFFFFFu);
+}
+
+}  // namespace vx

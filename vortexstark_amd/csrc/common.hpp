@@ -1,0 +1,41 @@
+// Shared host-side helpers of libvortex_hip: error recording, the stream that
+// reference-ABI launches go to, and launch bookkeeping.
+//
+// Error model (mirrors the reference wrappers, which return void and surface
+// failures through the runtime's last-error query, ffi.rs:6-21): every entry
+// point records the first failure in a thread-local slot read by
+// vx_get_last_error()/vx_get_last_error_message(); nothing aborts the process.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/vortex_hip.h"
+
+namespace vx {
+
+void set_error(int code, const char* what);
+int peek_error();  // current thread's recorded error, not cleared
+hipStream_t current_stream();
+
+// Records launch/runtime failures after a kernel launch or runtime call.
+inline bool check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        set_error((int)e, what);
+        return false;
+    }
+    return true;
+}
+inline bool check_launch(const char* what) { return check(hipGetLastError(), what); }
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+inline uint32_t ilog2(uint64_t n) {
+    uint32_t l = 0;
+    while ((1ull << l) < n) l++;
+    return l;
+}
+inline bool is_pow2(uint64_t n) { return n && !(n & (n - 1)); }
+
+}  // namespace vx

@@ -1,0 +1,53 @@
+// Internal (C++) interface between the kernel files and the prover.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "field.hpp"
+
+namespace vx {
+
+// ---------------------------------------------------------------- NTT
+// Twiddle tables of the native circle NTT for one domain of size 2^log_n
+// (layout of src/circle.rs:357-451: line layers concatenated, layer l at
+// offs[l]; circle twiddles separate).
+struct NttTables {
+    const uint32_t* line = nullptr;
+    const uint32_t* circle = nullptr;
+    uint32_t offs[32] = {0};
+    uint32_t log_n = 0;
+};
+
+// Forward transform (coefficients -> values). Reads src (src_n valid words,
+// the rest taken as zero: the implicit zero pad of cuda/constraint.cu:81-90)
+// and writes dst (2^log_n words). src may equal dst when src_n == 2^log_n.
+void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTables& t, hipStream_t s);
+// Inverse transform including the 2^-log_n scale. src may equal dst.
+void ntt_inverse(uint32_t* dst, const uint32_t* src, const NttTables& t, hipStream_t s);
+// Batched forms over a device array of column pointers (in place).
+void ntt_forward_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables& t, hipStream_t s);
+void ntt_inverse_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables& t, hipStream_t s);
+
+// Builds the twiddles of coset (initial, step) of size 2^log_n directly on the
+// device: line (2^log_n - 1 words), circle (2^(log_n-1) words) and, when the
+// i* pointers are non-null, their inverses. offs receives the layer offsets.
+void build_ntt_tables(CPt initial, CPt step, uint32_t log_n, uint32_t* line, uint32_t* circle, uint32_t* iline,
+                      uint32_t* icircle, uint32_t offs[32], hipStream_t s);
+// FRI fold twiddles of a domain: out[i] = 1 / coord(initial * step^bitrev(2i)), i < 2^(log_n-1).
+void build_fold_twiddles(CPt initial, CPt step, uint32_t log_n, bool use_y, uint32_t* out, uint32_t* scratch,
+                         hipStream_t s);
+void batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t s);
+
+// Host-side circle arithmetic used to derive coset parameters.
+struct HostPt {
+    uint32_t x, y;
+};
+HostPt host_pmul(HostPt a, HostPt b);
+HostPt host_repeated_double(HostPt p, uint32_t n);
+HostPt host_generator();
+// kind: 0 subgroup, 1 half_coset, 2 half_odds (src/circle.rs:98-143)
+void host_coset(int kind, uint32_t log_size, HostPt& initial, HostPt& step);
+
+}  // namespace vx

@@ -1,0 +1,477 @@
+// Native circle NTT and its twiddles for gfx950.
+//
+// Transform semantics are those of cuda/circle_ntt.cu:8-50,181-249: layer b
+// (butterfly stride 2^b) pairs i0 = (h << (b+1)) + l with i1 = i0 + 2^b and
+// uses twiddle table[h]; b >= 1 reads line layer b-1, b = 0 the circle table.
+// Evaluate runs b = k-1 .. 0, interpolate runs b = 0 .. k-1 and scales by
+// 2^((30k) mod 31).
+//
+// Execution is MI355X-shaped instead of one launch per layer: a transform is
+// cut into a few passes, each applying several consecutive layers to tiles
+// held in LDS. The pass holding the low strides works on contiguous tiles of
+// up to 2^14 words (64 KB LDS); passes over higher strides gather 2^m rows of
+// 2^c contiguous words (c = 5: 128-byte segments) so every HBM access stays
+// coalesced. A 2^26-point evaluate is 3 passes instead of 26.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace vx {
+
+// ---------------------------------------------------------------- host circle math
+namespace {
+constexpr uint32_t HP = 0x7FFFFFFFu;
+inline uint32_t hmul(uint32_t a, uint32_t b) {
+    uint64_t p = (uint64_t)a * b;
+    uint32_t r = (uint32_t)(p & HP) + (uint32_t)(p >> 31);
+    return r >= HP ? r - HP : r;
+}
+inline uint32_t hadd(uint32_t a, uint32_t b) {
+    uint32_t r = a + b;
+    return r >= HP ? r - HP : r;
+}
+inline uint32_t hsub(uint32_t a, uint32_t b) { return a >= b ? a - b : a + HP - b; }
+}  // namespace
+
+HostPt host_pmul(HostPt a, HostPt b) {
+    return {hsub(hmul(a.x, b.x), hmul(a.y, b.y)), hadd(hmul(a.x, b.y), hmul(a.y, b.x))};
+}
+HostPt host_repeated_double(HostPt p, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) p = host_pmul(p, p);
+    return p;
+}
+HostPt host_generator() { return {2u, 1268011823u}; }  // src/circle.rs:26-29
+void host_coset(int kind, uint32_t k, HostPt& initial, HostPt& step) {
+    HostPt g = host_generator();
+    step = host_repeated_double(g, 31 - k);
+    if (kind == 0)
+        initial = {1, 0};
+    else if (kind == 1)
+        initial = host_repeated_double(g, 30 - k);
+    else
+        initial = host_repeated_double(g, k >= 29 ? 0 : 29 - k);
+}
+static HostPt host_pow(HostPt p, uint64_t e) {
+    HostPt r = {1, 0};
+    while (e) {
+        if (e & 1) r = host_pmul(r, p);
+        p = host_pmul(p, p);
+        e >>= 1;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------- pass kernel
+struct NttPassArgs {
+    uint32_t* dst;
+    const uint32_t* src;
+    uint32_t* const* cols;  // batched: column blockIdx.y, in place
+    const uint32_t* line;
+    const uint32_t* circle;
+    uint32_t offs[32];
+    uint64_t src_n;  // words of src that are read; the rest is zero
+    uint32_t s, m, c;
+    uint32_t scale;  // multiply on store when non-zero
+};
+
+template <bool FWD>
+__global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
+    extern __shared__ uint32_t tile[];
+    uint32_t* dst = a.cols ? a.cols[blockIdx.y] : a.dst;
+    const uint32_t* src = a.cols ? a.cols[blockIdx.y] : a.src;
+    const uint32_t T = blockDim.x, tid = threadIdx.x;
+    const uint32_t s = a.s, m = a.m, c = a.c;
+    const uint32_t cmask = (1u << c) - 1;
+    const uint64_t blk = blockIdx.x;
+    const uint32_t mid_bits = s - c;
+    const uint64_t mid = blk & ((1ull << mid_bits) - 1);
+    const uint64_t hi = blk >> mid_bits;
+    const uint64_t base = (hi << (s + m)) + (mid << c);
+    const uint32_t tile_n = 1u << (m + c);
+
+    for (uint32_t e = tid; e < tile_n; e += T) {
+        uint64_t g = base + ((uint64_t)(e >> c) << s) + (e & cmask);
+        tile[e] = g < a.src_n ? src[g] : 0u;
+    }
+    __syncthreads();
+
+    const uint32_t half = tile_n >> 1;
+    for (uint32_t step = 0; step < m; step++) {
+        const uint32_t j = FWD ? (m - 1 - step) : step;  // row-bit of this layer
+        const uint32_t b = s + j;                         // global stride bit
+        const uint32_t* tw = b ? a.line + a.offs[b - 1] : a.circle;
+        for (uint32_t u = tid; u < half; u += T) {
+            uint32_t q = u & cmask, v = u >> c;
+            uint32_t r0 = ((v >> j) << (j + 1)) | (v & ((1u << j) - 1));
+            uint32_t e0 = (r0 << c) | q, e1 = e0 + (1u << (j + c));
+            uint64_t g0 = base + ((uint64_t)r0 << s) + q;
+            uint32_t w = tw[g0 >> (b + 1)];
+            uint32_t v0 = tile[e0], v1 = tile[e1];
+            if (FWD) {
+                uint32_t t = mmul(v1, w);
+                tile[e0] = madd(v0, t);
+                tile[e1] = msub(v0, t);
+            } else {
+                tile[e0] = madd(v0, v1);
+                tile[e1] = mmul(msub(v0, v1), w);
+            }
+        }
+        __syncthreads();
+    }
+
+    for (uint32_t e = tid; e < tile_n; e += T) {
+        uint64_t g = base + ((uint64_t)(e >> c) << s) + (e & cmask);
+        uint32_t v = tile[e];
+        dst[g] = a.scale ? mmul(v, a.scale) : v;
+    }
+}
+
+struct Pass {
+    uint32_t s, m, c;
+};
+static std::vector<Pass> plan_passes(uint32_t k) {
+    std::vector<Pass> ps;
+    if (k == 0) return ps;
+    const uint32_t LOW = 14, HIGH = 6;
+    uint32_t m0 = k < LOW ? k : LOW;
+    ps.push_back({0, m0, 0});
+    uint32_t R = k - m0;
+    if (R) {
+        uint32_t np = (R + HIGH - 1) / HIGH, s = m0;
+        for (uint32_t i = 0; i < np; i++) {
+            uint32_t mi = R / np + (i < R % np ? 1 : 0);
+            ps.push_back({s, mi, s < 5 ? s : 5});
+            s += mi;
+        }
+    }
+    return ps;
+}
+
+static void launch_pass(bool fwd, NttPassArgs a, uint32_t k, uint32_t n_cols, hipStream_t st) {
+    uint32_t tile_n = 1u << (a.m + a.c);
+    uint32_t T = tile_n / 2 < 256 ? (tile_n / 2 < 64 ? 64 : tile_n / 2) : 256;
+    dim3 grid((uint32_t)(1ull << (k - a.m - a.c)), n_cols);
+    size_t lds = (size_t)tile_n * 4;
+    if (fwd)
+        ntt_pass_kernel<true><<<grid, T, lds, st>>>(a);
+    else
+        ntt_pass_kernel<false><<<grid, T, lds, st>>>(a);
+    check_launch("ntt_pass_kernel");
+}
+
+static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n, uint32_t* const* cols,
+                    uint32_t n_cols, const NttTables& t, hipStream_t st) {
+    uint32_t k = t.log_n;
+    auto ps = plan_passes(k);
+    uint32_t ex = (30u * k) % 31u;
+    uint32_t inv_n = ex == 0 ? 1u : (1u << ex);
+    for (size_t i = 0; i < ps.size(); i++) {
+        size_t pi = fwd ? ps.size() - 1 - i : i;
+        NttPassArgs a = {};
+        a.cols = cols;
+        a.dst = dst;
+        a.src = i == 0 ? src : dst;
+        a.src_n = i == 0 ? src_n : (1ull << k);
+        a.line = t.line;
+        a.circle = t.circle;
+        for (int l = 0; l < 32; l++) a.offs[l] = t.offs[l];
+        a.s = ps[pi].s;
+        a.m = ps[pi].m;
+        a.c = ps[pi].c;
+        a.scale = (!fwd && i + 1 == ps.size() && inv_n != 1u) ? inv_n : 0u;
+        launch_pass(fwd, a, k, n_cols, st);
+    }
+    if (!fwd && ps.empty() && src != dst && src_n) {
+        check(hipMemcpyAsync(dst, src, 4, hipMemcpyDeviceToDevice, st), "ntt copy");
+    }
+}
+
+void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTables& t, hipStream_t s) {
+    if (t.log_n == 0) {
+        if (src != dst) check(hipMemcpyAsync(dst, src, 4, hipMemcpyDeviceToDevice, s), "ntt copy");
+        return;
+    }
+    run_ntt(true, dst, src, src_n, nullptr, 1, t, s);
+}
+void ntt_inverse(uint32_t* dst, const uint32_t* src, const NttTables& t, hipStream_t s) {
+    run_ntt(false, dst, src, 1ull << t.log_n, nullptr, 1, t, s);
+}
+void ntt_forward_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& t, hipStream_t s) {
+    if (n_cols) run_ntt(true, nullptr, nullptr, 1ull << t.log_n, cols, n_cols, t, s);
+}
+void ntt_inverse_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& t, hipStream_t s) {
+    if (n_cols) run_ntt(false, nullptr, nullptr, 1ull << t.log_n, cols, n_cols, t, s);
+}
+
+// ---------------------------------------------------------------- twiddle kernels
+// Reference-ABI kernels (cuda/twiddles.cu): coordinates of initial * step^i.
+__global__ void coset_points_kernel(CPt initial, CPt step, uint32_t* ox, uint32_t* oy, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    CPt p = cp_pow_mul(initial, step, i);
+    ox[i] = p.x;
+    oy[i] = p.y;
+}
+__global__ void fold_sources_kernel(CPt initial, CPt step, uint32_t* out, uint32_t half_n, uint32_t log_n,
+                                    int use_y) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= half_n) return;
+    CPt p = cp_pow_mul(initial, step, brev_bits(i << 1, log_n));
+    out[i] = use_y ? p.y : p.x;
+}
+__global__ void squash_kernel(const uint32_t* in, uint32_t* tw_out, uint32_t* sq_out, uint32_t half_n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= half_n) return;
+    uint32_t x = in[2 * i];
+    if (tw_out) tw_out[i] = x;
+    uint32_t x2 = mmul(x, x);
+    sq_out[i] = msub(madd(x2, x2), 1u);
+}
+
+// Closed form of the line twiddles: layer l, entry i equals
+// x(initial^(2^l) * step^(i * 2^(2l+1))), i.e. the x-coordinate that the
+// extract-and-squash chain of circle.rs:394-425 produces for that slot
+// (each squash doubles the point and keeps every second one).
+__global__ void line_twiddles_kernel(const CPt* layer_base, const CPt* layer_step, const uint32_t* offs,
+                                     uint32_t n_layers, uint32_t* line, uint64_t total) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    // find layer: offs ascending
+    uint32_t l = 0;
+    while (l + 1 < n_layers && offs[l + 1] <= g) l++;
+    uint32_t i = (uint32_t)(g - offs[l]);
+    CPt p = cp_pow_mul(layer_base[l], layer_step[l], i);
+    line[g] = p.x;
+}
+__global__ void circle_twiddles_kernel(CPt initial, CPt step, uint32_t* circle, uint32_t half_n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= half_n) return;
+    circle[i] = cp_pow_mul(initial, step, i).y;
+}
+
+// Montgomery batch inverse over 64-word chunks, one chunk per thread, with the
+// zero conventions of cuda/batch_inverse.cu:8-41 (zeros map to zero).
+__global__ void batch_inverse_kernel(const uint32_t* in, uint32_t* out, uint64_t n) {
+    const uint32_t CH = 64;
+    uint64_t start = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * CH;
+    if (start >= n) return;
+    uint32_t count = (uint32_t)(n - start < CH ? n - start : CH);
+    uint32_t prefix[64];
+    prefix[0] = in[start];
+    for (uint32_t i = 1; i < count; i++) {
+        uint32_t v = in[start + i];
+        prefix[i] = v == 0 ? prefix[i - 1] : mmul(prefix[i - 1], v);
+    }
+    uint32_t iv = minv(prefix[count - 1]);
+    for (uint32_t i = count - 1; i > 0; i--) {
+        uint32_t v = in[start + i];
+        if (v == 0) {
+            out[start + i] = 0;
+        } else {
+            out[start + i] = mmul(iv, prefix[i - 1]);
+            iv = mmul(iv, v);
+        }
+    }
+    out[start] = in[start] == 0 ? 0 : iv;
+}
+
+void batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t s) {
+    if (!n) return;
+    uint64_t chunks = (n + 63) / 64;
+    batch_inverse_kernel<<<ceil_div(chunks, 128), 128, 0, s>>>(in, out, n);
+    check_launch("batch_inverse_kernel");
+}
+
+void build_ntt_tables(CPt initial, CPt step, uint32_t log_n, uint32_t* line, uint32_t* circle, uint32_t* iline,
+                      uint32_t* icircle, uint32_t offs[32], hipStream_t s) {
+    uint64_t n = 1ull << log_n;
+    uint32_t n_line = log_n > 0 ? log_n - 1 : 0;
+    for (int i = 0; i < 32; i++) offs[i] = 0;
+    uint64_t off = 0, cur = n;
+    std::vector<CPt> bases(n_line ? n_line : 1), steps(n_line ? n_line : 1);
+    std::vector<uint32_t> hoffs(n_line ? n_line : 1, 0);
+    HostPt hi = {initial.x, initial.y}, hs = {step.x, step.y};
+    for (uint32_t l = 0; l < n_line; l++) {
+        offs[l] = (uint32_t)off;
+        hoffs[l] = (uint32_t)off;
+        HostPt b = host_repeated_double(hi, l);                 // initial^(2^l)
+        HostPt st = host_pow(hs, 1ull << (2 * l + 1));          // step^(2^(2l+1))
+        bases[l] = {b.x, b.y};
+        steps[l] = {st.x, st.y};
+        off += cur / 2;
+        cur /= 2;
+    }
+    if (n_line) {
+        // n - 1 line words in total; the last entry of each layer is unused by
+        // the transform but is part of the reference layout.
+        CPt* d_meta = nullptr;
+        uint32_t* d_offs = nullptr;
+        check(hipMalloc((void**)&d_meta, sizeof(CPt) * 2 * n_line), "hipMalloc");
+        check(hipMalloc((void**)&d_offs, 4 * n_line), "hipMalloc");
+        std::vector<CPt> meta(bases);
+        meta.insert(meta.end(), steps.begin(), steps.end());
+        check(hipMemcpyAsync(d_meta, meta.data(), sizeof(CPt) * 2 * n_line, hipMemcpyHostToDevice, s), "H2D");
+        check(hipMemcpyAsync(d_offs, hoffs.data(), 4 * n_line, hipMemcpyHostToDevice, s), "H2D");
+        line_twiddles_kernel<<<ceil_div(off, 256), 256, 0, s>>>(d_meta, d_meta + n_line, d_offs, n_line, line, off);
+        check_launch("line_twiddles_kernel");
+        // Word n-2 of the n-1 word layout belongs to no layer; keep it defined (zero)
+        // so the batch inverse below never reads an undefined value.
+        check(hipMemsetAsync(line + off, 0, 4 * (n - 1 - off), s), "hipMemsetAsync");
+        check(hipStreamSynchronize(s), "sync");
+        check(hipFree(d_meta), "hipFree");
+        check(hipFree(d_offs), "hipFree");
+    }
+    if (n >= 2) {
+        circle_twiddles_kernel<<<ceil_div(n / 2, 256), 256, 0, s>>>(initial, step, circle, (uint32_t)(n / 2));
+        check_launch("circle_twiddles_kernel");
+    }
+    if (iline && n_line) batch_inverse(line, iline, n - 1, s);
+    if (icircle && n >= 2) batch_inverse(circle, icircle, n / 2, s);
+}
+
+void build_fold_twiddles(CPt initial, CPt step, uint32_t log_n, bool use_y, uint32_t* out, uint32_t* scratch,
+                         hipStream_t s) {
+    uint32_t half = (uint32_t)((1ull << log_n) / 2);
+    if (!half) return;
+    fold_sources_kernel<<<ceil_div(half, 256), 256, 0, s>>>(initial, step, scratch, half, log_n, use_y ? 1 : 0);
+    check_launch("fold_sources_kernel");
+    batch_inverse(scratch, out, half, s);
+}
+
+// ---------------------------------------------------------------- single layer / bit reverse
+__global__ void ntt_layer_kernel(uint32_t* d, const uint32_t* tw, uint32_t li, uint32_t half_n, int fwd) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= half_n) return;
+    uint32_t stride = 1u << li, h = t >> li, l = t & (stride - 1);
+    uint32_t i0 = (h << (li + 1)) + l, i1 = i0 + stride;
+    uint32_t v0 = d[i0], v1 = d[i1], w = tw[h];
+    if (fwd) {
+        uint32_t x = mmul(v1, w);
+        d[i0] = madd(v0, x);
+        d[i1] = msub(v0, x);
+    } else {
+        d[i0] = madd(v0, v1);
+        d[i1] = mmul(msub(v0, v1), w);
+    }
+}
+__global__ void bit_reverse_kernel(uint32_t* d, uint32_t log_n, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t j = brev_bits(i, log_n);
+    if (i < j) {
+        uint32_t t = d[i];
+        d[i] = d[j];
+        d[j] = t;
+    }
+}
+
+static bool tables_from_abi(const uint32_t* tw, const uint32_t* circ, const uint32_t* h_offs, uint32_t n_line,
+                            uint32_t n, NttTables& t) {
+    if (!is_pow2(n)) {
+        set_error(-2, "circle NTT: n must be a power of two");
+        return false;
+    }
+    t.log_n = ilog2(n);
+    uint32_t expect = t.log_n ? t.log_n - 1 : 0;
+    if (n_line != expect) {
+        set_error(-2, "circle NTT: n_line_layers must be log2(n) - 1");
+        return false;
+    }
+    t.line = tw;
+    t.circle = circ;
+    for (uint32_t l = 0; l < 32; l++) t.offs[l] = l < n_line ? h_offs[l] : 0;
+    return true;
+}
+
+}  // namespace vx
+
+using namespace vx;
+
+extern "C" {
+
+void cuda_circle_ntt_evaluate(uint32_t* d_data, const uint32_t* d_twiddles, const uint32_t* d_circle_twids,
+                              const uint32_t* h_layer_offsets, const uint32_t* /*h_layer_sizes*/,
+                              uint32_t n_line_layers, uint32_t n) {
+    NttTables t;
+    if (!tables_from_abi(d_twiddles, d_circle_twids, h_layer_offsets, n_line_layers, n, t)) return;
+    hipStream_t s = current_stream();
+    ntt_forward(d_data, d_data, n, t, s);
+    check(hipStreamSynchronize(s), "cuda_circle_ntt_evaluate");
+}
+void cuda_circle_ntt_interpolate(uint32_t* d_data, const uint32_t* d_itwiddles, const uint32_t* d_circle_itwids,
+                                 const uint32_t* h_layer_offsets, const uint32_t* /*h_layer_sizes*/,
+                                 uint32_t n_line_layers, uint32_t n) {
+    NttTables t;
+    if (!tables_from_abi(d_itwiddles, d_circle_itwids, h_layer_offsets, n_line_layers, n, t)) return;
+    hipStream_t s = current_stream();
+    ntt_inverse(d_data, d_data, t, s);
+    check(hipStreamSynchronize(s), "cuda_circle_ntt_interpolate");
+}
+void cuda_circle_ntt_evaluate_batch(uint32_t** d_columns, const uint32_t* d_twiddles, const uint32_t* d_circle_twids,
+                                    const uint32_t* h_layer_offsets, const uint32_t* /*h_layer_sizes*/,
+                                    uint32_t n_line_layers, uint32_t n, uint32_t n_cols) {
+    NttTables t;
+    if (!tables_from_abi(d_twiddles, d_circle_twids, h_layer_offsets, n_line_layers, n, t)) return;
+    hipStream_t s = current_stream();
+    if (t.log_n) ntt_forward_batch(d_columns, n_cols, t, s);
+    check(hipStreamSynchronize(s), "cuda_circle_ntt_evaluate_batch");
+}
+void cuda_circle_ntt_interpolate_batch(uint32_t** d_columns, const uint32_t* d_itwiddles,
+                                       const uint32_t* d_circle_itwids, const uint32_t* h_layer_offsets,
+                                       const uint32_t* /*h_layer_sizes*/, uint32_t n_line_layers, uint32_t n,
+                                       uint32_t n_cols) {
+    NttTables t;
+    if (!tables_from_abi(d_itwiddles, d_circle_itwids, h_layer_offsets, n_line_layers, n, t)) return;
+    hipStream_t s = current_stream();
+    if (t.log_n) ntt_inverse_batch(d_columns, n_cols, t, s);
+    check(hipStreamSynchronize(s), "cuda_circle_ntt_interpolate_batch");
+}
+void cuda_circle_ntt_layer(uint32_t* d_data, const uint32_t* d_twiddles, uint32_t layer_idx, uint32_t n,
+                           int32_t forward) {
+    uint32_t half = n / 2;
+    if (!half) return;
+    ntt_layer_kernel<<<ceil_div(half, 256), 256, 0, current_stream()>>>(d_data, d_twiddles, layer_idx, half,
+                                                                         forward);
+    check_launch("cuda_circle_ntt_layer");
+}
+void cuda_bit_reverse_m31(uint32_t* data, uint32_t log_n) {
+    uint32_t n = 1u << log_n;
+    bit_reverse_kernel<<<ceil_div(n, 256), 256, 0, current_stream()>>>(data, log_n, n);
+    check_launch("cuda_bit_reverse_m31");
+}
+
+void cuda_compute_fold_twiddle_sources(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uint32_t* output,
+                                       uint32_t n, uint32_t log_n, int32_t extract_y) {
+    cuda_compute_fold_twiddle_sources_stream(ix, iy, sx, sy, output, n, log_n, extract_y, current_stream());
+}
+void cuda_compute_fold_twiddle_sources_stream(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uint32_t* output,
+                                              uint32_t n, uint32_t log_n, int32_t extract_y, void* stream) {
+    uint32_t half = n / 2;
+    if (!half) return;
+    fold_sources_kernel<<<ceil_div(half, 256), 256, 0, (hipStream_t)stream>>>(CPt{ix, iy}, CPt{sx, sy}, output,
+                                                                              half, log_n, extract_y);
+    check_launch("cuda_compute_fold_twiddle_sources");
+}
+void cuda_batch_inverse_m31(const uint32_t* input, uint32_t* output, uint32_t n) {
+    batch_inverse(input, output, n, current_stream());
+}
+void cuda_batch_inverse_m31_stream(const uint32_t* input, uint32_t* output, uint32_t n, void* stream) {
+    batch_inverse(input, output, n, (hipStream_t)stream);
+}
+void cuda_compute_coset_points(uint32_t ix, uint32_t iy, uint32_t sx, uint32_t sy, uint32_t* ox, uint32_t* oy,
+                               uint32_t n) {
+    if (!n) return;
+    coset_points_kernel<<<ceil_div(n, 256), 256, 0, current_stream()>>>(CPt{ix, iy}, CPt{sx, sy}, ox, oy, n);
+    check_launch("cuda_compute_coset_points");
+}
+void cuda_squash_x(const uint32_t* input, uint32_t* output, uint32_t out_n) {
+    if (!out_n) return;
+    squash_kernel<<<ceil_div(out_n, 256), 256, 0, current_stream()>>>(input, nullptr, output, out_n);
+    check_launch("cuda_squash_x");
+}
+void cuda_extract_and_squash(const uint32_t* input, uint32_t* tw_out, uint32_t* sq_out, uint32_t half_n) {
+    if (!half_n) return;
+    squash_kernel<<<ceil_div(half_n, 256), 256, 0, current_stream()>>>(input, tw_out, sq_out, half_n);
+    check_launch("cuda_extract_and_squash");
+}
+
+}  // extern "C"

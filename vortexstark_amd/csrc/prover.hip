@@ -1,0 +1,623 @@
+// Fused, device-resident Fibonacci prover (Part 3 of the C ABI).
+//
+// Same protocol and bytes as the reference's prove() + serialize_proof
+// (src/prover.rs:1721-1978, src/bin/stark_cli.rs:907-975), organised for one
+// MI355X instead of around host round trips:
+//
+//   INTT (trace domain) -> NTT (eval domain, zero pad folded into the first
+//   pass) -> trace tree -> quotient tree (quotient recomputed on the fly) ->
+//   circle fold fused with the first FRI tree -> line fold fused with each
+//   next FRI tree -> queries -> decommitment
+//
+// Every Merkle root is mixed into the Fiat-Shamir transcript by the kernel
+// that computes it, and that kernel also draws the next challenge into device
+// memory, so the whole proof is one stream of launches with no host wait.
+// The KRAK v1 byte image is assembled in device memory: a template with the
+// static words (magic, counts, path depths) is written once when the prover is
+// created and every proof overwrites all the data words.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "merkle_engine.hpp"
+
+namespace vx {
+
+// ---------------------------------------------------------------- trace generation
+// air.rs:85-131: (F(k), F(k+1)) from (a, b) by powers of [[1,1],[1,0]].
+__device__ void fib_at(uint32_t a, uint32_t b, uint64_t k, uint32_t& fk, uint32_t& fk1) {
+    uint32_t m00 = 1, m01 = 0, m10 = 0, m11 = 1;
+    uint32_t b00 = 1, b01 = 1, b10 = 1, b11 = 0;
+    while (k) {
+        if (k & 1) {
+            uint32_t t00 = madd(mmul(m00, b00), mmul(m01, b10)), t01 = madd(mmul(m00, b01), mmul(m01, b11));
+            uint32_t t10 = madd(mmul(m10, b00), mmul(m11, b10)), t11 = madd(mmul(m10, b01), mmul(m11, b11));
+            m00 = t00;
+            m01 = t01;
+            m10 = t10;
+            m11 = t11;
+        }
+        uint32_t t00 = madd(mmul(b00, b00), mmul(b01, b10)), t01 = madd(mmul(b00, b01), mmul(b01, b11));
+        uint32_t t10 = madd(mmul(b10, b00), mmul(b11, b10)), t11 = madd(mmul(b10, b01), mmul(b11, b11));
+        b00 = t00;
+        b01 = t01;
+        b10 = t10;
+        b11 = t11;
+        k >>= 1;
+    }
+    fk = madd(mmul(m10, b), mmul(m11, a));
+    fk1 = madd(mmul(m00, b), mmul(m01, a));
+}
+__global__ void fib_trace_kernel(uint32_t a, uint32_t b, uint64_t n, uint32_t chunk, uint32_t* out) {
+    uint64_t start = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * chunk;
+    if (start >= n) return;
+    uint32_t f0, f1;
+    fib_at(a, b, start, f0, f1);
+    uint64_t end = start + chunk < n ? start + chunk : n;
+    for (uint64_t i = start; i < end; i++) {
+        out[i] = f0;
+        uint32_t f2 = madd(f0, f1);
+        f0 = f1;
+        f1 = f2;
+    }
+}
+
+// ---------------------------------------------------------------- queries
+// Last layer into the proof, mix_felts(last layer), then n_q draw_number(E)
+// (prover.rs:1922-1927). After mix_felts the draws only differ in n_draws, so
+// every query is squeezed by its own lane.
+constexpr uint32_t MAX_LAST_WORDS = 4096;
+__global__ __launch_bounds__(256) void queries_kernel(DevChannel* ch, const uint32_t* l0, const uint32_t* l1,
+                                                     const uint32_t* l2, const uint32_t* l3, uint32_t n_last,
+                                                     uint32_t* proof, uint64_t last_off, uint64_t q_off,
+                                                     uint32_t n_q, uint64_t E, uint32_t* queries, uint32_t a,
+                                                     uint32_t b) {
+    __shared__ uint32_t words[MAX_LAST_WORDS];
+    __shared__ uint32_t st[8];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < n_last; i += blockDim.x) {
+        uint32_t v[4] = {l0[i], l1[i], l2[i], l3[i]};
+        for (int c = 0; c < 4; c++) {
+            words[4 * i + c] = v[c];
+            proof[last_off + 4 * i + c] = v[c];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        ch_mix_felts(ch, words, 4 * n_last);
+        for (int c = 0; c < 8; c++) st[c] = ch->state[c];
+        ch->n_draws = n_q;
+        proof[3] = a;
+        proof[4] = b;
+    }
+    __syncthreads();
+    uint32_t s[8];
+    for (int c = 0; c < 8; c++) s[c] = st[c];
+    for (uint32_t i = tid; i < n_q; i += blockDim.x) {
+        uint32_t w[8];
+        ch_squeeze(s, i, w);
+        uint64_t raw = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+        uint32_t qi = (uint32_t)(raw % E);
+        queries[i] = qi;
+        proof[q_off + i] = qi;
+    }
+}
+
+// ---------------------------------------------------------------- decommitment
+// prover.rs:1184-1247 and 2171-2234: value at the query index and at its
+// sibling (index ^ 1), with both authentication paths (siblings from the leaf
+// level up, root excluded). One wave per (tree, query).
+struct TreeDesc {
+    uint32_t kind;   // 0: one M31 column, 1: Fibonacci quotient of the LDE, 2: SoA4
+    uint32_t depth;  // log2(leaves)
+    uint32_t shift;  // leaf index = query >> shift
+    uint32_t from;   // level store covers levels >= from
+    const uint32_t* c[4];
+    uint64_t mask;
+    const uint32_t* alpha;
+    uint32_t* store;
+    uint64_t val_off, sib_off, path_off, sib_path_off;
+};
+
+__device__ void tree_values(const TreeDesc& t, uint64_t i, uint32_t v[4]) {
+    if (t.kind == 0) {
+        SrcCol1{t.c[0]}.values(i, v);
+    } else if (t.kind == 1) {
+        SrcQuotient{t.c[0], t.mask, t.alpha}.values(i, v);
+    } else {
+        SrcSoa4{t.c[0], t.c[1], t.c[2], t.c[3]}.values(i, v);
+    }
+}
+__device__ void tree_leaf(const TreeDesc& t, uint64_t i, uint32_t h[8]) {
+    if (t.kind == 0) {
+        SrcCol1{t.c[0]}.leaf(i, h);
+    } else if (t.kind == 1) {
+        SrcQuotient{t.c[0], t.mask, t.alpha}.leaf(i, h);
+    } else {
+        SrcSoa4{t.c[0], t.c[1], t.c[2], t.c[3]}.leaf(i, h);
+    }
+}
+
+constexpr uint32_t DECOMMIT_SUB = 5;  // recomputed subtree: 32 leaves
+__global__ __launch_bounds__(64) void decommit_kernel(const TreeDesc* trees, const uint32_t* queries, uint32_t* proof) {
+    __shared__ uint32_t lv[8 * 32];    // current level of the subtree, word-major
+    __shared__ uint32_t path[8 * 32];  // sibling of the query's ancestor per level
+    __shared__ uint32_t leaf_self[8];  // leaf hash of the query index itself
+    const TreeDesc t = trees[blockIdx.y];
+    const uint32_t q = blockIdx.x, lane = threadIdx.x;
+    const uint64_t li = (uint64_t)queries[q] >> t.shift;
+    const uint32_t D = t.depth;
+    const uint32_t lam = t.from < D ? t.from : D;
+    const uint32_t width = t.kind == 0 ? 1 : 4;
+    const uint64_t sub_base = li & ~((1ull << lam) - 1);
+
+    if (lane == 0) {
+        uint32_t v[4], s[4];
+        tree_values(t, li, v);
+        tree_values(t, li ^ 1, s);
+        for (uint32_t k = 0; k < width; k++) {
+            proof[t.val_off + (uint64_t)q * width + k] = v[k];
+            proof[t.sib_off + (uint64_t)q * width + k] = s[k];
+        }
+    }
+    uint32_t cnt = 1u << lam;
+    if (lane < cnt) {
+        uint32_t h[8];
+        tree_leaf(t, sub_base + lane, h);
+        for (int w = 0; w < 8; w++) lv[w * 32 + lane] = h[w];
+    }
+    __syncthreads();
+    uint32_t pos = (uint32_t)(li - sub_base);
+    if (lane < 8) leaf_self[lane] = lv[lane * 32 + pos];
+    for (uint32_t L = 0; L < lam; L++) {
+        if (lane < 8) path[L * 8 + lane] = lv[lane * 32 + (pos ^ 1)];
+        uint32_t half = cnt / 2, l[8], r[8];
+        if (lane < half)
+            for (int w = 0; w < 8; w++) {
+                l[w] = lv[w * 32 + 2 * lane];
+                r[w] = lv[w * 32 + 2 * lane + 1];
+            }
+        __syncthreads();
+        if (lane < half) {
+            uint32_t h[8];
+            b2s_node(l, r, h);
+            for (int w = 0; w < 8; w++) lv[w * 32 + lane] = h[w];
+        }
+        __syncthreads();
+        cnt = half;
+        pos >>= 1;
+    }
+    LevelStore st = {t.store, D, lam};
+    for (uint32_t L = lam; L < D; L++) {
+        if (lane < 8) path[L * 8 + lane] = st.at(L, ((li >> L) ^ 1))[lane];
+    }
+    __syncthreads();
+    const uint64_t rec = 1 + 8ull * D;
+    uint32_t* p0 = proof + t.path_off + (uint64_t)q * rec;
+    uint32_t* p1 = proof + t.sib_path_off + (uint64_t)q * rec;
+    if (lane == 0) {
+        p0[0] = D;
+        p1[0] = D;
+    }
+    for (uint32_t k = lane; k < 8 * D; k += 64) {
+        p0[1 + k] = path[k];
+        p1[1 + k] = k < 8 ? leaf_self[k] : path[k];
+    }
+}
+
+// ---------------------------------------------------------------- Merkle driver
+static uint32_t pick_logc(uint64_t n) {
+    const uint64_t MIN_WAVES = 1024;
+    for (int c = (int)MERKLE_LOGC_MAX; c > 0; c--)
+        if ((n >> (7 + c)) >= MIN_WAVES) return (uint32_t)c;
+    return 0;
+}
+
+struct Scratch {
+    uint32_t* a;
+    uint32_t* b;
+};
+
+// Commits n_leaves (power of two >= 2) items of a leaf source.
+template <class Src, class Mark>
+static void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, Scratch sc, hipStream_t s,
+                        Mark&& mark) {
+    if (n_leaves <= MERKLE_TAIL_MAX) {
+        merkle_tail_kernel<Src><<<1, TAIL_THREADS, 0, s>>>(src, 0, (uint32_t)n_leaves, st, out);
+        check_launch("merkle_tail_kernel");
+        mark("tail");
+        return;
+    }
+    uint32_t logc = pick_logc(n_leaves);
+    uint64_t waves = n_leaves >> (7 + logc);
+    merkle_stage_kernel<Src><<<ceil_div(waves, MERKLE_STAGE_WAVES), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
+        src, 0, n_leaves, logc, sc.a, st);
+    check_launch("merkle_stage_kernel");
+    mark("stage_leaf");
+    uint64_t n = n_leaves >> (logc + 1);
+    uint32_t lvl = logc + 1;
+    uint32_t* cur = sc.a;
+    uint32_t* nxt = sc.b;
+    while (n > MERKLE_TAIL_MAX) {
+        logc = pick_logc(n);
+        waves = n >> (7 + logc);
+        merkle_stage_kernel<SrcNodes><<<ceil_div(waves, MERKLE_STAGE_WAVES), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
+            SrcNodes{cur}, lvl, n, logc, nxt, st);
+        check_launch("merkle_stage_kernel<nodes>");
+        mark("stage_node");
+        n >>= logc + 1;
+        lvl += logc + 1;
+        uint32_t* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    merkle_tail_kernel<SrcNodes><<<1, TAIL_THREADS, 0, s>>>(SrcNodes{cur}, lvl, (uint32_t)n, st, out);
+    check_launch("merkle_tail_kernel<nodes>");
+    mark("tail");
+}
+
+static uint64_t max_stage_nodes(uint64_t n_leaves) {
+    if (n_leaves <= MERKLE_TAIL_MAX) return 0;
+    uint64_t mx = 0, n = n_leaves;
+    while (n > MERKLE_TAIL_MAX) {
+        uint32_t c = pick_logc(n);
+        n >>= c + 1;
+        if (n > mx) mx = n;
+    }
+    return mx;
+}
+
+}  // namespace vx
+
+// ---------------------------------------------------------------- prover object
+using namespace vx;
+
+struct vx_prover {
+    uint32_t log_n = 0, log_e = 0, n_q = 0, log_last = 0, J = 0;
+    uint64_t N = 0, E = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    // twiddles
+    NttTables trace_tab, eval_tab;
+    std::vector<uint32_t*> fri_tw;
+    // work
+    uint32_t* trace_buf = nullptr;
+    uint32_t* coeffs = nullptr;
+    uint32_t* eval = nullptr;
+    std::vector<uint32_t*> layer;  // J layers, 4 columns each contiguous
+    std::vector<LevelStore> stores;
+    Scratch scratch = {nullptr, nullptr};
+    DevChannel* ch = nullptr;
+    uint32_t* alphas = nullptr;
+    uint32_t* queries = nullptr;
+    uint32_t* proof = nullptr;
+    uint64_t proof_words = 0;
+    TreeDesc* d_trees = nullptr;
+    // layout
+    uint64_t off_fri_roots = 0, off_last = 0, off_queries = 0;
+    // optional launch timeline (events recorded on the prover's stream)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::string> labels;
+    size_t n_marks = 0;
+    void mark(const char* label) {
+        if (!timing) return;
+        if (n_marks == ev_pool.size()) {
+            hipEvent_t e;
+            if (!check(hipEventCreate(&e), "hipEventCreate")) return;
+            ev_pool.push_back(e);
+            labels.emplace_back();
+        }
+        labels[n_marks] = label;
+        check(hipEventRecord(ev_pool[n_marks], stream), "hipEventRecord");
+        n_marks++;
+    }
+
+    uint32_t* col(uint32_t j, uint32_t c) const { return layer[j] + (uint64_t)c * (E >> (j + 1)); }
+    uint32_t* alpha_q() const { return alphas; }
+    uint32_t* alpha_f(uint32_t j) const { return alphas + 4 * (1 + j); }
+
+    bool alloc(void** p, uint64_t bytes) {
+        if (!check(hipMalloc(p, bytes ? bytes : 4), "vx_prover: hipMalloc")) return false;
+        allocs.push_back(*p);
+        return true;
+    }
+    ~vx_prover() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+struct Layout {
+    std::vector<uint32_t> words;  // static template
+    uint64_t fri_roots, last, queries;
+    std::vector<TreeDesc> trees;  // offsets only
+};
+
+// KRAK v1 word layout (stark_cli.rs:948-975).
+void build_layout(vx_prover* p, Layout& L) {
+    std::vector<uint32_t>& w = L.words;
+    const uint32_t nq = p->n_q;
+    w = {0x4B52414Bu, 1u, p->log_n, 0, 0};
+    w.resize(5 + 16, 0);  // trace root, quotient root
+    w.push_back(p->J);
+    L.fri_roots = w.size();
+    w.resize(w.size() + 8ull * p->J, 0);
+    w.push_back(1u << p->log_last);
+    L.last = w.size();
+    w.resize(w.size() + (4ull << p->log_last), 0);
+    w.push_back(nq);
+    L.queries = w.size();
+    w.resize(w.size() + nq, 0);
+    auto decom = [&](uint32_t width, uint32_t depth, TreeDesc& t) {
+        w.push_back(nq);
+        t.val_off = w.size();
+        w.resize(w.size() + (uint64_t)nq * width, 0);
+        t.sib_off = w.size();
+        w.resize(w.size() + (uint64_t)nq * width, 0);
+        t.path_off = w.size();
+        for (uint32_t q = 0; q < nq; q++) {
+            w.push_back(depth);
+            w.resize(w.size() + 8ull * depth, 0);
+        }
+        t.sib_path_off = w.size();
+        for (uint32_t q = 0; q < nq; q++) {
+            w.push_back(depth);
+            w.resize(w.size() + 8ull * depth, 0);
+        }
+    };
+    L.trees.assign(2 + p->J, TreeDesc{});
+    decom(1, p->log_e, L.trees[0]);
+    decom(4, p->log_e, L.trees[1]);
+    w.push_back(p->J);
+    for (uint32_t j = 0; j < p->J; j++) decom(4, p->log_e - 1 - j, L.trees[2 + j]);
+}
+
+CPt to_cpt(HostPt h) { return {h.x, h.y}; }
+
+}  // namespace
+
+extern "C" {
+
+vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer) {
+    if (n_queries == 0) n_queries = 80;       // prover.rs:90
+    if (log_last_layer == 0) log_last_layer = 3;  // fri.rs:265
+    if (log_n < 2 || log_n + 2 > 30 || log_last_layer > log_n + 1 || log_last_layer > 10 || n_queries > 4096) {
+        set_error(-2, "vx_prover_create: unsupported parameters");
+        return nullptr;
+    }
+    vx_prover* p = new vx_prover();
+    p->log_n = log_n;
+    p->log_e = log_n + 2;  // BLOWUP_BITS = 2 (prover.rs:86)
+    p->n_q = n_queries;
+    p->log_last = log_last_layer;
+    p->N = 1ull << log_n;
+    p->E = 1ull << p->log_e;
+    p->J = p->log_e - log_last_layer;
+    const uint64_t N = p->N, E = p->E;
+    bool ok = check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hipStream_t s = p->stream;
+
+    // twiddle tables (ProverCache, prover.rs:159-176)
+    uint32_t *t_line, *t_circle, *t_iline, *t_icircle, *e_line, *e_circle;
+    ok = ok && p->alloc((void**)&t_line, 4 * (N - 1)) && p->alloc((void**)&t_circle, 4 * (N / 2)) &&
+         p->alloc((void**)&t_iline, 4 * (N - 1)) && p->alloc((void**)&t_icircle, 4 * (N / 2)) &&
+         p->alloc((void**)&e_line, 4 * (E - 1)) && p->alloc((void**)&e_circle, 4 * (E / 2));
+    if (!ok) {
+        delete p;
+        return nullptr;
+    }
+    HostPt ti, ts, ei, es;
+    host_coset(1, log_n, ti, ts);
+    host_coset(1, p->log_e, ei, es);
+    build_ntt_tables(to_cpt(ti), to_cpt(ts), log_n, t_line, t_circle, t_iline, t_icircle, p->trace_tab.offs, s);
+    build_ntt_tables(to_cpt(ei), to_cpt(es), p->log_e, e_line, e_circle, nullptr, nullptr, p->eval_tab.offs, s);
+    p->trace_tab.line = t_iline;
+    p->trace_tab.circle = t_icircle;
+    p->trace_tab.log_n = log_n;
+    p->eval_tab.line = e_line;
+    p->eval_tab.circle = e_circle;
+    p->eval_tab.log_n = p->log_e;
+
+    // FRI fold twiddles (fri.rs:143-160): [0] circle on half_coset(log_e), y;
+    // [j] line on half_odds(log_e - j), x.
+    uint32_t* tw_scratch;
+    ok = p->alloc((void**)&tw_scratch, 4 * (E / 2));
+    p->fri_tw.assign(p->J, nullptr);
+    for (uint32_t j = 0; ok && j < p->J; j++) {
+        uint32_t k = p->log_e - j;
+        ok = p->alloc((void**)&p->fri_tw[j], 4 * ((1ull << k) / 2));
+        if (!ok) break;
+        HostPt di, dstp;
+        host_coset(j == 0 ? 1 : 2, k, di, dstp);
+        build_fold_twiddles(to_cpt(di), to_cpt(dstp), k, j == 0, p->fri_tw[j], tw_scratch, s);
+    }
+
+    // work buffers
+    ok = ok && p->alloc((void**)&p->trace_buf, 4 * N) && p->alloc((void**)&p->coeffs, 4 * N) &&
+         p->alloc((void**)&p->eval, 4 * E);
+    p->layer.assign(p->J, nullptr);
+    for (uint32_t j = 0; ok && j < p->J; j++) ok = p->alloc((void**)&p->layer[j], 16 * (E >> (j + 1)));
+    // level stores: trace, quotient, FRI layers
+    p->stores.clear();
+    auto add_store = [&](uint32_t depth) {
+        uint32_t from = DECOMMIT_SUB < depth ? DECOMMIT_SUB : depth;
+        uint32_t* b = nullptr;
+        ok = ok && p->alloc((void**)&b, 4 * level_store_words(depth, from));
+        p->stores.push_back(LevelStore{b, depth, from});
+    };
+    add_store(p->log_e);
+    add_store(p->log_e);
+    for (uint32_t j = 0; j < p->J; j++) add_store(p->log_e - 1 - j);
+    uint64_t mx = max_stage_nodes(E);
+    ok = ok && p->alloc((void**)&p->scratch.a, 32 * (mx ? mx : 1)) && p->alloc((void**)&p->scratch.b, 32 * (mx ? mx : 1));
+    ok = ok && p->alloc((void**)&p->ch, sizeof(DevChannel)) && p->alloc((void**)&p->alphas, 16 * (1 + p->J)) &&
+         p->alloc((void**)&p->queries, 4 * p->n_q);
+
+    Layout L;
+    build_layout(p, L);
+    p->proof_words = L.words.size();
+    p->off_fri_roots = L.fri_roots;
+    p->off_last = L.last;
+    p->off_queries = L.queries;
+    ok = ok && p->alloc((void**)&p->proof, 4 * p->proof_words);
+    ok = ok && p->alloc((void**)&p->d_trees, sizeof(TreeDesc) * L.trees.size());
+    if (ok) {
+        ok = check(hipMemcpyAsync(p->proof, L.words.data(), 4 * p->proof_words, hipMemcpyHostToDevice, s), "H2D");
+        std::vector<TreeDesc>& T = L.trees;
+        T[0].kind = 0;
+        T[0].depth = p->log_e;
+        T[0].shift = 0;
+        T[0].c[0] = p->eval;
+        T[1].kind = 1;
+        T[1].depth = p->log_e;
+        T[1].shift = 0;
+        T[1].c[0] = p->eval;
+        T[1].mask = E - 1;
+        T[1].alpha = p->alpha_q();
+        for (uint32_t j = 0; j < p->J; j++) {
+            TreeDesc& t = T[2 + j];
+            t.kind = 2;
+            t.depth = p->log_e - 1 - j;
+            t.shift = j + 1;
+            for (int c = 0; c < 4; c++) t.c[c] = p->col(j, c);
+        }
+        for (size_t i = 0; i < T.size(); i++) {
+            T[i].from = p->stores[i].from;
+            T[i].store = p->stores[i].base;
+        }
+        ok = ok && check(hipMemcpyAsync(p->d_trees, T.data(), sizeof(TreeDesc) * T.size(), hipMemcpyHostToDevice, s),
+                         "H2D");
+        ok = ok && check(hipStreamSynchronize(s), "vx_prover_create");
+    }
+    if (!ok) {
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+
+void vx_prover_destroy(vx_prover* p) { delete p; }
+void* vx_prover_stream(vx_prover* p) { return p ? (void*)p->stream : nullptr; }
+uint64_t vx_prover_proof_size(const vx_prover* p) { return p ? 4 * p->proof_words : 0; }
+const void* vx_prover_proof_device(const vx_prover* p) { return p ? p->proof : nullptr; }
+const uint32_t* vx_prover_eval_device(const vx_prover* p) { return p ? p->eval : nullptr; }
+const uint32_t* vx_prover_fri_layer_device(const vx_prover* p, uint32_t j, uint32_t column) {
+    if (!p || j >= p->J || column >= 4) return nullptr;
+    return p->col(j, column);
+}
+
+int32_t vx_prover_prove_device(vx_prover* p, const uint32_t* d_trace, uint32_t a, uint32_t b) {
+    if (!p || !d_trace) {
+        set_error(-2, "vx_prover_prove_device: null argument");
+        return -1;
+    }
+    hipStream_t s = p->stream;
+    const uint64_t N = p->N, E = p->E;
+    p->n_marks = 0;
+    p->mark("start");
+    std::string phase;
+    auto mk = [&](const char* what) {
+        std::string l = phase + ":" + what;
+        p->mark(l.c_str());
+    };
+    // LDE: interpolate on half_coset(log_n), evaluate on half_coset(log_n + 2)
+    ntt_inverse(p->coeffs, d_trace, p->trace_tab, s);
+    p->mark("lde:interpolate");
+    ntt_forward(p->eval, p->coeffs, N, p->eval_tab, s);
+    p->mark("lde:evaluate");
+    // trace commitment -> alpha (quotient)
+    phase = "trace_commit";
+    commit_tree(SrcCol1{p->eval}, E, p->stores[0], TailOut{p->proof + 5, p->ch, p->alpha_q(), 1}, p->scratch, s, mk);
+    // quotient commitment -> alpha_0 (circle fold)
+    phase = "quotient_commit";
+    commit_tree(SrcQuotient{p->eval, E - 1, p->alpha_q()}, E, p->stores[1],
+                TailOut{p->proof + 13, p->ch, p->alpha_f(0), 0}, p->scratch, s, mk);
+    // FRI layer 0: circle fold fused with its commitment
+    {
+        phase = "fri0_circle_fold_commit";
+        SrcCircleFold src = {p->eval, E - 1, p->alpha_q(), p->alpha_f(0), p->fri_tw[0],
+                             p->col(0, 0), p->col(0, 1), p->col(0, 2), p->col(0, 3)};
+        uint32_t* nxt = p->J > 1 ? p->alpha_f(1) : nullptr;
+        commit_tree(src, E / 2, p->stores[2], TailOut{p->proof + p->off_fri_roots, p->ch, nxt, 0}, p->scratch, s, mk);
+    }
+    // FRI layers 1..J-1: line folds fused with their commitments
+    for (uint32_t j = 1; j < p->J; j++) {
+        phase = "fri" + std::to_string(j) + "_line_fold_commit";
+        SrcLineFold src = {p->col(j - 1, 0), p->col(j - 1, 1), p->col(j - 1, 2), p->col(j - 1, 3), p->alpha_f(j),
+                           p->fri_tw[j],     p->col(j, 0),     p->col(j, 1),     p->col(j, 2),     p->col(j, 3)};
+        uint32_t* nxt = j + 1 < p->J ? p->alpha_f(j + 1) : nullptr;
+        commit_tree(src, E >> (j + 1), p->stores[2 + j],
+                    TailOut{p->proof + p->off_fri_roots + 8ull * j, p->ch, nxt, 0}, p->scratch, s, mk);
+    }
+    // queries and decommitment
+    uint32_t jl = p->J - 1;
+    queries_kernel<<<1, 256, 0, s>>>(p->ch, p->col(jl, 0), p->col(jl, 1), p->col(jl, 2), p->col(jl, 3),
+                                     1u << p->log_last, p->proof, p->off_last, p->off_queries, p->n_q, E, p->queries,
+                                     a, b);
+    check_launch("queries_kernel");
+    p->mark("queries");
+    decommit_kernel<<<dim3(p->n_q, 2 + p->J), 64, 0, s>>>(p->d_trees, p->queries, p->proof);
+    check_launch("decommit_kernel");
+    p->mark("decommit");
+    return peek_error() == 0 ? 0 : -1;
+}
+
+uint64_t vx_prover_proof_copy(vx_prover* p, void* host_out, uint64_t cap) {
+    if (!p) return 0;
+    uint64_t bytes = 4 * p->proof_words;
+    if (!host_out || cap < bytes) {
+        set_error(-2, "vx_prover_proof_copy: buffer too small");
+        return 0;
+    }
+    if (!check(hipMemcpyAsync(host_out, p->proof, bytes, hipMemcpyDeviceToHost, p->stream), "D2H")) return 0;
+    if (!check(hipStreamSynchronize(p->stream), "vx_prover_proof_copy")) return 0;
+    DevChannel ch;
+    if (!check(hipMemcpy(&ch, p->ch, sizeof(ch), hipMemcpyDeviceToHost), "D2H")) return 0;
+    if (ch.failed) {
+        set_error(-3, "transcript: draw_felt exhausted its retry budget");
+        return 0;
+    }
+    return bytes;
+}
+
+int32_t vx_prover_set_timing(vx_prover* p, int32_t enable) {
+    if (!p) return -1;
+    p->timing = enable != 0;
+    p->n_marks = 0;
+    return 0;
+}
+uint32_t vx_prover_timeline_count(const vx_prover* p) { return p ? (uint32_t)p->n_marks : 0; }
+const char* vx_prover_timeline_label(const vx_prover* p, uint32_t i) {
+    return (p && i < p->n_marks) ? p->labels[i].c_str() : "";
+}
+float vx_prover_timeline_ms(vx_prover* p, uint32_t i) {
+    if (!p || i == 0 || i >= p->n_marks) return 0.f;
+    float ms = 0.f;
+    if (!check(hipEventSynchronize(p->ev_pool[i]), "hipEventSynchronize")) return -1.f;
+    if (!check(hipEventElapsedTime(&ms, p->ev_pool[i - 1], p->ev_pool[i]), "hipEventElapsedTime")) return -1.f;
+    return ms;
+}
+
+int32_t vx_fibonacci_trace(uint32_t a, uint32_t b, uint32_t log_n, uint32_t* d_out, void* stream) {
+    uint64_t n = 1ull << log_n;
+    const uint32_t chunk = 256;
+    uint64_t threads = (n + chunk - 1) / chunk;
+    fib_trace_kernel<<<ceil_div(threads, 256), 256, 0, (hipStream_t)stream>>>(a, b, n, chunk, d_out);
+    return check_launch("vx_fibonacci_trace") ? 0 : -1;
+}
+
+uint64_t vx_prove_fibonacci(vx_prover* p, uint32_t a, uint32_t b, void* host_out, uint64_t cap) {
+    if (!p) return 0;
+    if (vx_fibonacci_trace(a, b, p->log_n, p->trace_buf, p->stream)) return 0;
+    if (vx_prover_prove_device(p, p->trace_buf, a, b)) return 0;
+    return vx_prover_proof_copy(p, host_out, cap);
+}
+
+}  // extern "C"
