@@ -59,8 +59,10 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=100)
     ap.add_argument("--log-last", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-log-n", type=int, default=20)
+    ap.add_argument("--cpu-log-n", type=int, default=24)
     ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -89,6 +91,14 @@ def main() -> int:
     vx.lib.vx_stream_synchronize(stream)
     vx.check("warmup")
     log(f"warmup done ({args.warmup} proofs)")
+    if args.calibrate_copy_mb:
+        nb = args.calibrate_copy_mb << 20
+        src, dst = vx.DeviceBuffer(nb // 4), vx.DeviceBuffer(nb // 4)
+        vx.lib.vx_memcpy_d2d(dst.ptr, src.ptr, nb)
+        vx.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mb} MiB device to device")
+        src.free()
+        dst.free()
 
     vx.synchronize()
     t_start = time.perf_counter()

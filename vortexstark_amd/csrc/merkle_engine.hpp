@@ -12,11 +12,13 @@
 //    the 64 lanes compute 64 parents of 128 items (one chunk); chunks merge
 //    pairwise like a binary counter, the left chunk waiting in the wave's LDS
 //    stack, so every compression runs with all 64 lanes busy and no
-//    workgroup barrier is ever needed. The wave emits 64 nodes, 2^(logc+1)
-//    levels up.
-//  * Tail kernel: one workgroup reduces the last <= 4096 items to the root,
+//    workgroup barrier is ever needed. The last 64 nodes are reduced to one
+//    with quad-cooperative compressions (4 lanes each), so the wave emits one
+//    node, 7 + logc levels above its items.
+//  * Tail kernel: one workgroup reduces the last <= 2048 items to the root,
 //    writes it where the caller asks and advances the Fiat-Shamir transcript
-//    (mix the root, draw the next challenge) in the same launch.
+//    (mix the root, draw the next challenge) in the same launch; its narrow
+//    levels and the transcript hashes also run quad-cooperatively.
 //  * Every node at level >= store_from is also written to a per-tree level
 //    store; decommitment recomputes the 2^store_from-leaf subtree of a query
 //    and reads the rest, so full trees are never written to HBM.
@@ -27,7 +29,15 @@
 
 namespace vx {
 
-constexpr uint32_t MERKLE_TAIL_MAX = 4096;  // items reduced by the tail kernel
+// Orders this wave's LDS writes before its later LDS reads (the LDS executes a
+// wave's requests in order; this keeps the compiler from reordering them).
+#define VX_WAVE_SYNC()                                        \
+    do {                                                      \
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); \
+        __builtin_amdgcn_wave_barrier();                      \
+    } while (0)
+
+constexpr uint32_t MERKLE_TAIL_MAX = 2048;  // most items one tail-kernel block reduces
 constexpr uint32_t MERKLE_LOGC_MAX = 5;     // chunks per wave in a stage: up to 32
 constexpr uint32_t MERKLE_STAGE_WAVES = 4;  // waves per stage workgroup
 
@@ -223,17 +233,88 @@ struct SrcLineFold {
     }
 };
 
+// ---------------------------------------------------------------- narrow levels
+// Reduces n (a power of two <= 64) nodes stored node-major ([node][8 words]) in
+// the LDS buffer `a` to one node with the quads of the calling wave, ping-ponging
+// with `b`. The nodes are at tree level `lvl`, node 0 having index idx0 there.
+// Every lane of the wave must call this; returns the buffer holding the root.
+__device__ __forceinline__ uint32_t* narrow_reduce(uint32_t* a, uint32_t* b, uint32_t n, uint32_t lvl, uint64_t idx0,
+                                                   const LevelStore& st) {
+    const uint32_t lane = threadIdx.x & 63, quad = lane >> 2, l = lane & 3;
+    while (n > 1) {
+        const uint32_t half = n / 2;
+        for (uint32_t q0 = 0; q0 < half; q0 += 16) {
+            const uint32_t q = q0 + quad;
+            const uint32_t qq = q < half ? q : half - 1;  // whole quads stay active for the DPP permutes
+            uint32_t hl, hh;
+            b2s_quad_init(hl, hh);
+            b2s_compress_quad(a + 16 * qq, 64u, 0xFFFFFFFFu, hl, hh);
+            if (q < half) {
+                b[8 * q + l] = hl;
+                b[8 * q + 4 + l] = hh;
+                if (st.want(lvl + 1)) {
+                    uint32_t* dst = st.at(lvl + 1, (idx0 >> 1) + q);
+                    dst[l] = hl;
+                    dst[4 + l] = hh;
+                }
+            }
+        }
+        VX_WAVE_SYNC();
+        uint32_t* t = a;
+        a = b;
+        b = t;
+        n = half;
+        lvl++;
+        idx0 >>= 1;
+    }
+    return a;
+}
+
+// Fiat-Shamir step after a commitment, by one whole wave (quad 0 is the one whose
+// results are kept): optional reset, mix_digest(root) and optional draw_felt
+// (channel.rs:30-39, 67-86). root and msg are LDS (8 and 16 words).
+__device__ __forceinline__ void quad_transcript(DevChannel* ch, const uint32_t* root, uint32_t* msg, bool init,
+                                                uint32_t* alpha_dst) {
+    const uint32_t lane = threadIdx.x & 63, l = lane & 3;
+    if (init && lane == 0) ch->failed = 0;
+    if (lane < 8) msg[lane] = init ? 0u : ch->state[lane];
+    if (lane >= 8 && lane < 16) msg[lane] = root[lane - 8];
+    VX_WAVE_SYNC();
+    uint32_t hl, hh;
+    b2s_quad_init(hl, hh);
+    b2s_compress_quad(msg, 64u, 0xFFFFFFFFu, hl, hh);
+    VX_WAVE_SYNC();
+    // new state words l and 4 + l live in every quad; publish quad 0's
+    if (lane < 4) {
+        msg[l] = hl;
+        msg[4 + l] = hh;
+    }
+    VX_WAVE_SYNC();
+    if (lane < 8) ch->state[lane] = msg[lane];
+    uint32_t n_draws = 0;
+    if (alpha_dst) {
+        for (int tries = 0; tries < 64; tries++) {
+            if (lane < 8) msg[8 + lane] = lane == 0 ? n_draws : 0u;
+            VX_WAVE_SYNC();
+            uint32_t wl, wh;
+            b2s_quad_init(wl, wh);
+            b2s_compress_quad(msg, 37u, 0xFFFFFFFFu, wl, wh);
+            n_draws++;
+            const uint64_t ok = __ballot(wl < 2u * P && wh < 2u * P);
+            VX_WAVE_SYNC();
+            if ((ok & 0xFull) == 0xFull) {
+                if (lane < 4) alpha_dst[l] = canon(wl);
+                break;
+            }
+            if (tries == 63 && lane == 0) ch->failed = 1;
+        }
+    }
+    if (lane == 0) ch->n_draws = n_draws;
+}
+
 // ---------------------------------------------------------------- stage kernel
 // LDS per wave: MERKLE_LOGC_MAX pending chunks + 1 exchange chunk, each 64
 // nodes stored word-major ([word][lane]) so chunk writes are conflict-free.
-// Orders this wave's LDS writes before its later LDS reads (the LDS executes a
-// wave's requests in order; this keeps the compiler from reordering them).
-#define VX_WAVE_SYNC()                                        \
-    do {                                                      \
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); \
-        __builtin_amdgcn_wave_barrier();                      \
-    } while (0)
-
 constexpr uint32_t STAGE_SLOTS = MERKLE_LOGC_MAX + 1;
 constexpr uint32_t STAGE_WAVE_LDS = STAGE_SLOTS * 8 * 64;
 
@@ -284,7 +365,12 @@ __global__ __launch_bounds__(64 * MERKLE_STAGE_WAVES) void merkle_stage_kernel(S
             VX_WAVE_SYNC();
         }
     }
-    store8(out_nodes + 8 * (wave_id * 64 + lane), h);
+    // the wave's 64 top nodes (level item_level + 1 + logc) -> one node, 6 levels up
+#pragma unroll
+    for (int q = 0; q < 8; q++) wl[lane * 8 + q] = h[q];
+    VX_WAVE_SYNC();
+    uint32_t* root = narrow_reduce(wl, wl + 512, 64, item_level + 1 + logc, wave_id * 64, st);
+    if (lane < 8) out_nodes[8 * wave_id + lane] = root[lane];
 }
 
 // ---------------------------------------------------------------- tail kernel
@@ -303,8 +389,10 @@ constexpr uint32_t TAIL_THREADS = 1024;
 template <class Src>
 __global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src, uint32_t item_level, uint32_t n_items,
                                                                   LevelStore st, TailOut out) {
-    // nodes at one level, word-major: word q of node j at q * 2048 + j
-    __shared__ uint32_t nodes[8 * (MERKLE_TAIL_MAX / 2)];
+    constexpr uint32_t W = MERKLE_TAIL_MAX / 2;
+    __shared__ uint32_t nodes[8 * W];  // wide levels, word-major: word q of node j at q * W + j
+    __shared__ uint32_t na[8 * 64], nb[8 * 64];
+    __shared__ uint32_t msg[16];
     const uint32_t tid = threadIdx.x;
     const uint64_t item_base = (uint64_t)blockIdx.x * n_items;
     uint32_t n = n_items / 2;
@@ -314,17 +402,17 @@ __global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src, uint
         src.pair(item_base + 2ull * j, h);
         if (st.want(lvl)) store8(st.at(lvl, (item_base >> 1) + j), h);
 #pragma unroll
-        for (int q = 0; q < 8; q++) nodes[q * 2048 + j] = h[q];
+        for (int q = 0; q < 8; q++) nodes[q * W + j] = h[q];
     }
     __syncthreads();
-    while (n > 1) {
+    while (n > 64) {
         const uint32_t half = n / 2;
         uint32_t l[8], r[8];
         if (tid < half) {
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                l[q] = nodes[q * 2048 + 2 * tid];
-                r[q] = nodes[q * 2048 + 2 * tid + 1];
+                l[q] = nodes[q * W + 2 * tid];
+                r[q] = nodes[q * W + 2 * tid + 1];
             }
         }
         __syncthreads();
@@ -333,28 +421,19 @@ __global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src, uint
             uint32_t h[8];
             b2s_node(l, r, h);
 #pragma unroll
-            for (int q = 0; q < 8; q++) nodes[q * 2048 + tid] = h[q];
+            for (int q = 0; q < 8; q++) nodes[q * W + tid] = h[q];
             if (st.want(lvl)) store8(st.at(lvl, (item_base >> (lvl - item_level)) + tid), h);
         }
         __syncthreads();
         n = half;
     }
-    if (tid == 0) {
-        uint32_t root[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) root[q] = nodes[q * 2048];
-        if (out.root_dst)
-            for (int q = 0; q < 8; q++) out.root_dst[8ull * blockIdx.x + q] = root[q];
-        if (out.channel) {
-            if (out.init_channel) {
-                for (int q = 0; q < 8; q++) out.channel->state[q] = 0;
-                out.channel->n_draws = 0;
-                out.channel->failed = 0;
-            }
-            ch_mix_digest(out.channel, root);
-            if (out.alpha_dst) ch_draw_felt(out.channel, out.alpha_dst);
-        }
-    }
+    if (tid >= 64) return;  // one wave finishes: narrow levels and transcript
+    if (tid < n)
+        for (int q = 0; q < 8; q++) na[tid * 8 + q] = nodes[q * W + tid];
+    VX_WAVE_SYNC();
+    uint32_t* root = narrow_reduce(na, nb, n, lvl, item_base >> (lvl - item_level), st);
+    if (out.root_dst && tid < 8) out.root_dst[8ull * blockIdx.x + tid] = root[tid];
+    if (out.channel) quad_transcript(out.channel, root, msg, out.init_channel != 0, out.alpha_dst);
 }
 
 }  // namespace vx

@@ -69,61 +69,116 @@ struct NttPassArgs {
     const uint32_t* circle;
     uint32_t offs[32];
     uint64_t src_n;  // words of src that are read; the rest is zero
-    uint32_t s, m, c;
+    uint32_t s;      // lowest global stride bit of this pass
+    uint32_t k;      // log2 of the transform size
     uint32_t scale;  // multiply on store when non-zero
 };
 
-template <bool FWD>
-__global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
-    extern __shared__ uint32_t tile[];
-    uint32_t* dst = a.cols ? a.cols[blockIdx.y] : a.dst;
-    const uint32_t* src = a.cols ? a.cols[blockIdx.y] : a.src;
-    const uint32_t T = blockDim.x, tid = threadIdx.x;
-    const uint32_t s = a.s, m = a.m, c = a.c;
-    const uint32_t cmask = (1u << c) - 1;
-    const uint64_t blk = blockIdx.x;
-    const uint32_t mid_bits = s - c;
-    const uint64_t mid = blk & ((1ull << mid_bits) - 1);
-    const uint64_t hi = blk >> mid_bits;
-    const uint64_t base = (hi << (s + m)) + (mid << c);
-    const uint32_t tile_n = 1u << (m + c);
+// LDS word index with one pad word per 8, which keeps the strided radix-8
+// accesses below free of bank conflicts.
+__device__ __forceinline__ uint32_t lpad(uint32_t e) { return e + (e >> 3); }
 
-    for (uint32_t e = tid; e < tile_n; e += T) {
-        uint64_t g = base + ((uint64_t)(e >> c) << s) + (e & cmask);
-        tile[e] = g < a.src_n ? src[g] : 0u;
-    }
-    __syncthreads();
-
-    const uint32_t half = tile_n >> 1;
-    for (uint32_t step = 0; step < m; step++) {
-        const uint32_t j = FWD ? (m - 1 - step) : step;  // row-bit of this layer
-        const uint32_t b = s + j;                         // global stride bit
-        const uint32_t* tw = b ? a.line + a.offs[b - 1] : a.circle;
-        for (uint32_t u = tid; u < half; u += T) {
-            uint32_t q = u & cmask, v = u >> c;
-            uint32_t r0 = ((v >> j) << (j + 1)) | (v & ((1u << j) - 1));
-            uint32_t e0 = (r0 << c) | q, e1 = e0 + (1u << (j + c));
-            uint64_t g0 = base + ((uint64_t)r0 << s) + q;
-            uint32_t w = tw[g0 >> (b + 1)];
-            uint32_t v0 = tile[e0], v1 = tile[e1];
-            if (FWD) {
-                uint32_t t = mmul(v1, w);
-                tile[e0] = madd(v0, t);
-                tile[e1] = msub(v0, t);
-            } else {
-                tile[e0] = madd(v0, v1);
-                tile[e1] = mmul(msub(v0, v1), w);
+// One radix-2^R round over row bits [j, j + R) of an LDS tile of 2^M rows x
+// 2^C columns. Each group of 2^R values (fixed column and other row bits) is
+// loaded into registers, transformed through R layers, and written back.
+template <bool FWD, int M, int C, int R>
+__device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi, uint32_t j) {
+    constexpr uint32_t GROUPS = 1u << (M + C - R);
+    for (uint32_t g = threadIdx.x; g < GROUPS; g += blockDim.x) {
+        const uint32_t q = g & ((1u << C) - 1);
+        const uint32_t rest = g >> C;
+        const uint32_t row_lo = rest & ((1u << j) - 1);
+        const uint32_t row_base = ((rest >> j) << (j + R)) | row_lo;
+        uint32_t v[1 << R];
+#pragma unroll
+        for (int d = 0; d < (1 << R); d++) v[d] = tile[lpad(((row_base + ((uint32_t)d << j)) << C) + q)];
+#pragma unroll
+        for (int tt = 0; tt < R; tt++) {
+            const int t = FWD ? (R - 1 - tt) : tt;  // layer inside the group
+            const uint32_t rb = j + t;                 // row bit of the layer
+            const uint32_t b = a.s + rb;               // global stride bit
+            const uint32_t* tw = b ? a.line + a.offs[b - 1] : a.circle;
+#pragma unroll
+            for (int d = 0; d < (1 << R); d++) {
+                if (d & (1 << t)) continue;
+                const uint32_t r0 = row_base + ((uint32_t)d << j);
+                const uint32_t h = (uint32_t)((hi << (M - rb - 1)) + (r0 >> (rb + 1)));
+                const uint32_t w = tw[h];
+                uint32_t& x0 = v[d];
+                uint32_t& x1 = v[d | (1 << t)];
+                if (FWD) {
+                    uint32_t m = mmul(x1, w);
+                    x1 = msub(x0, m);
+                    x0 = madd(x0, m);
+                } else {
+                    uint32_t s0 = madd(x0, x1);
+                    x1 = mmul(msub(x0, x1), w);
+                    x0 = s0;
+                }
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < (1 << R); d++) tile[lpad(((row_base + ((uint32_t)d << j)) << C) + q)] = v[d];
     }
+}
 
-    for (uint32_t e = tid; e < tile_n; e += T) {
-        uint64_t g = base + ((uint64_t)(e >> c) << s) + (e & cmask);
-        uint32_t v = tile[e];
+// Applies the M layers with stride bits [s, s + M) of a size-2^k transform to
+// the tile of 2^M rows (stride 2^s) by 2^C contiguous columns owned by this block.
+template <bool FWD, int M, int C>
+__global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
+    __shared__ uint32_t tile[(1u << (M + C)) + (1u << (M + C)) / 8];
+    uint32_t* dst = a.cols ? a.cols[blockIdx.y] : a.dst;
+    const uint32_t* src = a.cols ? a.cols[blockIdx.y] : a.src;
+    const uint32_t s = a.s;
+    constexpr uint32_t TILE = 1u << (M + C);
+    const uint32_t mid_bits = s - C;
+    const uint64_t blk = blockIdx.x;
+    const uint64_t mid = blk & ((1ull << mid_bits) - 1);
+    const uint64_t hi = blk >> mid_bits;
+    const uint64_t base = (hi << (s + M)) + (mid << C);
+
+    for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
+        uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+        tile[lpad(e)] = g < a.src_n ? src[g] : 0u;
+    }
+    __syncthreads();
+    if (FWD) {
+        int top = M;
+        while (top > 0) {
+            int r = top >= 3 ? 3 : top;
+            top -= r;
+            if (r == 3)
+                ntt_round<FWD, M, C, (M >= 3 ? 3 : 1)>(tile, a, hi, (uint32_t)top);
+            else if (r == 2)
+                ntt_round<FWD, M, C, (M >= 2 ? 2 : 1)>(tile, a, hi, (uint32_t)top);
+            else
+                ntt_round<FWD, M, C, 1>(tile, a, hi, (uint32_t)top);
+            __syncthreads();
+        }
+    } else {
+        int lo = 0;
+        while (lo < M) {
+            int r = M - lo >= 3 ? 3 : M - lo;
+            if (r == 3)
+                ntt_round<FWD, M, C, (M >= 3 ? 3 : 1)>(tile, a, hi, (uint32_t)lo);
+            else if (r == 2)
+                ntt_round<FWD, M, C, (M >= 2 ? 2 : 1)>(tile, a, hi, (uint32_t)lo);
+            else
+                ntt_round<FWD, M, C, 1>(tile, a, hi, (uint32_t)lo);
+            lo += r;
+            __syncthreads();
+        }
+    }
+    for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
+        uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+        uint32_t v = tile[lpad(e)];
         dst[g] = a.scale ? mmul(v, a.scale) : v;
     }
 }
+
+constexpr uint32_t NTT_LOW = 12;   // layers of the contiguous (lowest-stride) pass
+constexpr uint32_t NTT_HIGH = 7;   // max layers of a strided pass
+constexpr uint32_t NTT_COLS = 5;   // strided passes move 2^5 contiguous words per row
 
 struct Pass {
     uint32_t s, m, c;
@@ -131,30 +186,54 @@ struct Pass {
 static std::vector<Pass> plan_passes(uint32_t k) {
     std::vector<Pass> ps;
     if (k == 0) return ps;
-    const uint32_t LOW = 14, HIGH = 6;
-    uint32_t m0 = k < LOW ? k : LOW;
+    uint32_t m0 = k < NTT_LOW ? k : NTT_LOW;
     ps.push_back({0, m0, 0});
     uint32_t R = k - m0;
     if (R) {
-        uint32_t np = (R + HIGH - 1) / HIGH, s = m0;
+        uint32_t np = (R + NTT_HIGH - 1) / NTT_HIGH, s = m0;
         for (uint32_t i = 0; i < np; i++) {
             uint32_t mi = R / np + (i < R % np ? 1 : 0);
-            ps.push_back({s, mi, s < 5 ? s : 5});
+            ps.push_back({s, mi, NTT_COLS});
             s += mi;
         }
     }
     return ps;
 }
 
-static void launch_pass(bool fwd, NttPassArgs a, uint32_t k, uint32_t n_cols, hipStream_t st) {
-    uint32_t tile_n = 1u << (a.m + a.c);
-    uint32_t T = tile_n / 2 < 256 ? (tile_n / 2 < 64 ? 64 : tile_n / 2) : 256;
-    dim3 grid((uint32_t)(1ull << (k - a.m - a.c)), n_cols);
-    size_t lds = (size_t)tile_n * 4;
-    if (fwd)
-        ntt_pass_kernel<true><<<grid, T, lds, st>>>(a);
-    else
-        ntt_pass_kernel<false><<<grid, T, lds, st>>>(a);
+template <bool FWD, int M, int C>
+static void launch_tile(const NttPassArgs& a, uint32_t k, uint32_t n_cols, hipStream_t st) {
+    dim3 grid((uint32_t)(1ull << (k - M - C)), n_cols);
+    constexpr uint32_t groups = 1u << (M + C > 3 ? M + C - 3 : 0);
+    uint32_t T = groups >= 256 ? 256 : (groups < 64 ? 64 : groups);
+    ntt_pass_kernel<FWD, M, C><<<grid, T, 0, st>>>(a);
+}
+
+template <bool FWD, int C, int M>
+static void dispatch_m(uint32_t m, const NttPassArgs& a, uint32_t k, uint32_t n_cols, hipStream_t st) {
+    if constexpr (M >= 1) {
+        if (m == (uint32_t)M) {
+            launch_tile<FWD, M, C>(a, k, n_cols, st);
+            return;
+        }
+        dispatch_m<FWD, C, M - 1>(m, a, k, n_cols, st);
+    } else {
+        set_error(-2, "ntt: unsupported pass shape");
+    }
+}
+
+static void launch_pass(bool fwd, const NttPassArgs& a, uint32_t m, uint32_t c, uint32_t k, uint32_t n_cols,
+                        hipStream_t st) {
+    if (c == 0) {
+        if (fwd)
+            dispatch_m<true, 0, NTT_LOW>(m, a, k, n_cols, st);
+        else
+            dispatch_m<false, 0, NTT_LOW>(m, a, k, n_cols, st);
+    } else {
+        if (fwd)
+            dispatch_m<true, NTT_COLS, NTT_HIGH>(m, a, k, n_cols, st);
+        else
+            dispatch_m<false, NTT_COLS, NTT_HIGH>(m, a, k, n_cols, st);
+    }
     check_launch("ntt_pass_kernel");
 }
 
@@ -175,10 +254,9 @@ static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n
         a.circle = t.circle;
         for (int l = 0; l < 32; l++) a.offs[l] = t.offs[l];
         a.s = ps[pi].s;
-        a.m = ps[pi].m;
-        a.c = ps[pi].c;
+        a.k = k;
         a.scale = (!fwd && i + 1 == ps.size() && inv_n != 1u) ? inv_n : 0u;
-        launch_pass(fwd, a, k, n_cols, st);
+        launch_pass(fwd, a, ps[pi].m, ps[pi].c, k, n_cols, st);
     }
     if (!fwd && ps.empty() && src != dst && src_n) {
         check(hipMemcpyAsync(dst, src, 4, hipMemcpyDeviceToDevice, st), "ntt copy");

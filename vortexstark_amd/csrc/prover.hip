@@ -215,6 +215,10 @@ static uint32_t pick_logc(uint64_t n) {
     return 0;
 }
 
+// Trees of at most this many items go straight to the tail kernel; larger
+// ones are reduced by stage kernels until this many nodes remain.
+constexpr uint32_t PROVER_TAIL_ITEMS = 1024;
+
 struct Scratch {
     uint32_t* a;
     uint32_t* b;
@@ -224,7 +228,7 @@ struct Scratch {
 template <class Src, class Mark>
 static void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, Scratch sc, hipStream_t s,
                         Mark&& mark) {
-    if (n_leaves <= MERKLE_TAIL_MAX) {
+    if (n_leaves <= PROVER_TAIL_ITEMS) {
         merkle_tail_kernel<Src><<<1, TAIL_THREADS, 0, s>>>(src, 0, (uint32_t)n_leaves, st, out);
         check_launch("merkle_tail_kernel");
         mark("tail");
@@ -236,19 +240,19 @@ static void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOu
         src, 0, n_leaves, logc, sc.a, st);
     check_launch("merkle_stage_kernel");
     mark("stage_leaf");
-    uint64_t n = n_leaves >> (logc + 1);
-    uint32_t lvl = logc + 1;
+    uint64_t n = waves;  // one node per wave
+    uint32_t lvl = 7 + logc;
     uint32_t* cur = sc.a;
     uint32_t* nxt = sc.b;
-    while (n > MERKLE_TAIL_MAX) {
+    while (n > PROVER_TAIL_ITEMS) {
         logc = pick_logc(n);
         waves = n >> (7 + logc);
         merkle_stage_kernel<SrcNodes><<<ceil_div(waves, MERKLE_STAGE_WAVES), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
             SrcNodes{cur}, lvl, n, logc, nxt, st);
         check_launch("merkle_stage_kernel<nodes>");
         mark("stage_node");
-        n >>= logc + 1;
-        lvl += logc + 1;
+        n = waves;
+        lvl += 7 + logc;
         uint32_t* tmp = cur;
         cur = nxt;
         nxt = tmp;
@@ -259,11 +263,10 @@ static void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOu
 }
 
 static uint64_t max_stage_nodes(uint64_t n_leaves) {
-    if (n_leaves <= MERKLE_TAIL_MAX) return 0;
+    if (n_leaves <= PROVER_TAIL_ITEMS) return 0;
     uint64_t mx = 0, n = n_leaves;
-    while (n > MERKLE_TAIL_MAX) {
-        uint32_t c = pick_logc(n);
-        n >>= c + 1;
+    while (n > PROVER_TAIL_ITEMS) {
+        n >>= 7 + pick_logc(n);
         if (n > mx) mx = n;
     }
     return mx;
