@@ -972,6 +972,216 @@ int verify(const Proof& pf, uint32_t nq, uint32_t log_last) {
     return 0;
 }
 
+
+// ---------------------------------------------------------------- stwo-basis NTT
+// Twiddles of src/ntt.rs:262-324 (StwoNttCache::new, restating stwo's
+// slow_precompute_twiddles): for a domain of size 2^log_n, start from
+// half_odds(log_n - 1); per level store the x-coordinates of the first half of
+// the coset in bit-reversed order, then double the coset; finally append 1.
+// Total 2^(log_n-1) words.
+void stwo_twiddles(uint32_t log_n, std::vector<uint32_t>& fwd, std::vector<uint32_t>& inv_out) {
+    uint64_t tc = 1ull << (log_n - 1);
+    Coset cur = half_odds(log_n - 1);
+    fwd.clear();
+    for (uint32_t lv = 0, L = cur.log_size; lv < L; lv++) {
+        uint64_t half = (1ull << cur.log_size) / 2;
+        uint32_t layer_log = 0;
+        while ((1ull << layer_log) < half) layer_log++;
+        std::vector<uint32_t> xs(half), brt(half);
+        Pt pt = cur.initial;
+        for (uint64_t j = 0; j < half; j++) {
+            xs[j] = pt.x;
+            pt = pmul(pt, cur.step);
+        }
+        for (uint64_t i = 0; i < half; i++) brt[layer_log == 0 ? 0 : bitrev((uint32_t)i, layer_log)] = xs[i];
+        fwd.insert(fwd.end(), brt.begin(), brt.end());
+        cur = {pmul(cur.initial, cur.initial), pmul(cur.step, cur.step), cur.log_size - 1};
+    }
+    fwd.push_back(1);
+    while (fwd.size() < tc) fwd.push_back(0);
+    fwd.resize(tc);
+    // ntt.rs:307-320: Montgomery batch inverse (no zero entries occur)
+    inv_out.assign(tc, 0);
+    std::vector<uint32_t> prefix(tc);
+    prefix[0] = fwd[0];
+    for (uint64_t i = 1; i < tc; i++) prefix[i] = mul(prefix[i - 1], fwd[i]);
+    uint32_t ip = inv(prefix[tc - 1]);
+    for (uint64_t i = tc - 1; i > 0; i--) {
+        inv_out[i] = mul(ip, prefix[i - 1]);
+        ip = mul(ip, fwd[i]);
+    }
+    inv_out[0] = ip;
+}
+// cuda/circle_ntt_stwo.cu: line layer k (stride 2^(k+1)) reads the flat buffer at
+// half_n - 2^(n_line - k) with index h = i0 >> (k+2); the circle layer pairs
+// (2t, 2t+1) with [y, -y, -x, x][t % 4], x = T[2(t/4)], y = T[2(t/4)+1].
+void stwo_ntt(uint32_t* d, const uint32_t* T, uint32_t log_n, bool fwd) {
+    uint64_t n = 1ull << log_n, half = n / 2;
+    uint32_t n_line = log_n - 1;
+    auto circle = [&]() {
+        parallel_for(half, [&](uint64_t b, uint64_t e) {
+            for (uint64_t t = b; t < e; t++) {
+                uint32_t x = T[2 * (t / 4)], y = T[2 * (t / 4) + 1];
+                uint32_t w = (t % 4 == 0) ? y : (t % 4 == 1) ? neg(y) : (t % 4 == 2) ? neg(x) : x;
+                uint32_t v0 = d[2 * t], v1 = d[2 * t + 1];
+                if (fwd) {
+                    uint32_t tmp = mul(v1, w);
+                    d[2 * t + 1] = sub(v0, tmp);
+                    d[2 * t] = add(v0, tmp);
+                } else {
+                    d[2 * t] = add(v0, v1);
+                    d[2 * t + 1] = mul(sub(v0, v1), w);
+                }
+            }
+        });
+    };
+    auto layer = [&](uint32_t k) {
+        const uint32_t* tw = T + (half - (1ull << (n_line - k)));
+        ntt_layer(d, tw, k + 1, n, fwd);
+    };
+    if (fwd) {
+        for (int k = (int)n_line - 1; k >= 0; k--) layer((uint32_t)k);
+        if (log_n >= 1) circle();
+    } else {
+        if (log_n >= 1) circle();
+        for (uint32_t k = 0; k < n_line; k++) layer(k);
+        uint32_t ex = (30u * log_n) % 31u;
+        uint32_t inv_n = ex == 0 ? 1u : (1u << ex);
+        for (uint64_t i = 0; i < n; i++) d[i] = mul(d[i], inv_n);
+    }
+}
+
+// ---------------------------------------------------------------- lifted leaves
+// cuda/merkle_leaves_lifted.cu:26-98 and crates/cuda-backend/src/merkle_ops.rs:24-148:
+// leaf L hashes, with standard multi-block Blake2s, the LE32 words of every
+// column (in the given order, ascending log size) at its lifted row.
+inline uint32_t lifted_row(uint32_t leaf, uint32_t S, uint32_t L) {
+    if (L == S) return leaf;
+    uint32_t sh = S - L + 1;
+    return ((leaf >> sh) << 1) | (leaf & 1);
+}
+void lifted_leaves(const uint32_t* const* cols, const uint32_t* logs, uint32_t n_cols, uint32_t S, uint32_t* out) {
+    uint64_t n = 1ull << S;
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        std::vector<uint8_t> msg(4 * n_cols);
+        for (uint64_t i = b; i < e; i++) {
+            for (uint32_t c = 0; c < n_cols; c++) put_le32(msg.data() + 4 * c, cols[c][lifted_row((uint32_t)i, S, logs[c])]);
+            uint8_t h[32];
+            blake2s_full(msg.data(), msg.size(), h);
+            for (int k = 0; k < 8; k++) out[8 * i + k] = le32(h + 4 * k);
+        }
+    });
+}
+
+// ---------------------------------------------------------------- wide commit
+// The batched wide-commitment workload: stwo-basis LDE of every column (blowup
+// 2^blowup), one lifted Blake2s tree over all of them, then FRI on
+// f = sum_c beta^c * lifted_c (beta drawn after the root), the circle fold on the
+// canonic domain of size 2^S (fri_ops.rs:40-78 twiddles: 1/y of
+// domain.at(bitrev(2i))) and line folds on half_odds (fri.rs:143-160), each
+// layer committed as an SoA4 tree, down to 2^log_last.
+struct WideOut {
+    uint32_t root[8];
+    std::vector<std::array<uint32_t, 8>> fri_roots;
+    std::vector<uint32_t> last;  // 4 words per element
+    uint32_t beta[4];
+};
+void wide_commit(const std::vector<std::vector<uint32_t>>& trace, const std::vector<uint32_t>& logs, uint32_t blowup,
+                 uint32_t log_last, WideOut& out, std::vector<std::vector<uint32_t>>* lde_out) {
+    uint32_t C = (uint32_t)trace.size();
+    std::vector<std::vector<uint32_t>> ev(C);
+    uint32_t S = 0;
+    for (uint32_t c = 0; c < C; c++) {
+        uint32_t l = logs[c], le = l + blowup;
+        std::vector<uint32_t> tw, itw, etw, eitw;
+        stwo_twiddles(l, tw, itw);
+        stwo_twiddles(le, etw, eitw);
+        std::vector<uint32_t> coef(trace[c]);
+        stwo_ntt(coef.data(), itw.data(), l, false);
+        ev[c].assign(1ull << le, 0);
+        memcpy(ev[c].data(), coef.data(), coef.size() * 4);  // extend (poly_ops.rs:496-514)
+        stwo_ntt(ev[c].data(), etw.data(), le, true);
+        if (le > S) S = le;
+    }
+    std::vector<uint32_t> elogs(C);
+    std::vector<const uint32_t*> cp(C);
+    for (uint32_t c = 0; c < C; c++) {
+        elogs[c] = logs[c] + blowup;
+        cp[c] = ev[c].data();
+    }
+    uint64_t n = 1ull << S;
+    std::vector<uint32_t> lay(8 * n), nx;
+    lifted_leaves(cp.data(), elogs.data(), C, S, lay.data());
+    while (lay.size() > 8) {
+        nx.assign(lay.size() / 2, 0);
+        node_layer(lay.data(), lay.size() / 16, nx.data());
+        lay.swap(nx);
+    }
+    memcpy(out.root, lay.data(), 32);
+    Channel ch;
+    ch.mix_digest(out.root);
+    QM31 beta = ch.draw_felt();
+    qto(beta, out.beta);
+    // f = sum_c beta^c * lifted_c
+    Soa4 f;
+    f.resize(n);
+    std::vector<QM31> pw(C);
+    QM31 acc = {{1, 0}, {0, 0}};
+    for (uint32_t c = 0; c < C; c++) {
+        pw[c] = acc;
+        acc = qmul(acc, beta);
+    }
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; i++) {
+            QM31 s = {{0, 0}, {0, 0}};
+            for (uint32_t c = 0; c < C; c++) s = qadd(s, qmul_m31(pw[c], cp[c][lifted_row((uint32_t)i, S, elogs[c])]));
+            f.set(i, s);
+        }
+    });
+    // circle fold on the canonic domain: twiddle i = 1 / y(half_odds(S-1).at(bitrev_{S-1}(i)))
+    std::vector<uint32_t> src(n / 2), tw(n / 2);
+    Coset ho = half_odds(S - 1);
+    for (uint64_t i = 0; i < n / 2; i++) src[i] = ho.at(bitrev((uint32_t)i, S - 1)).y;
+    batch_inverse(src.data(), tw.data(), n / 2);
+    QM31 a0 = ch.draw_felt();
+    std::vector<Soa4> layers(1);
+    layers[0].resize(n / 2);
+    fold_circle(layers[0], f, tw, a0);
+    out.fri_roots.clear();
+    auto commit = [&](const Soa4& L) {
+        const uint32_t* lc[4] = {L.c[0].data(), L.c[1].data(), L.c[2].data(), L.c[3].data()};
+        uint32_t r[8];
+        std::vector<uint32_t> cur(8 * L.size()), nn;
+        leaf_layer(lc, 4, L.size(), cur.data());
+        while (cur.size() > 8) {
+            nn.assign(cur.size() / 2, 0);
+            node_layer(cur.data(), cur.size() / 16, nn.data());
+            cur.swap(nn);
+        }
+        memcpy(r, cur.data(), 32);
+        std::array<uint32_t, 8> a;
+        memcpy(a.data(), r, 32);
+        out.fri_roots.push_back(a);
+        ch.mix_digest(r);
+    };
+    commit(layers[0]);
+    uint32_t cur = S - 1;
+    while (cur > log_last) {
+        QM31 al = ch.draw_felt();
+        std::vector<uint32_t> ltw;
+        fold_twiddles(half_odds(cur), false, ltw);
+        Soa4 nxt;
+        fold_line(layers.back(), ltw, al, nxt);
+        layers.push_back(std::move(nxt));
+        cur--;
+        commit(layers.back());
+    }
+    out.last.clear();
+    const Soa4& L = layers.back();
+    for (uint64_t i = 0; i < L.size(); i++)
+        for (int c = 0; c < 4; c++) out.last.push_back(L.c[c][i]);
+    if (lde_out) *lde_out = ev;
+}
 }  // namespace
 
 // ================================================================ C ABI
@@ -1193,6 +1403,60 @@ int orc_verify_krak(const uint8_t* proof, uint64_t len, uint32_t n_queries, uint
     Proof pf;
     if (!deserialize(proof, len, pf)) return 1;
     return verify(pf, n_queries, log_last_layer);
+}
+
+
+void orc_stwo_twiddles(uint32_t log_n, uint32_t* fwd, uint32_t* inv_out) {
+    std::vector<uint32_t> f, i;
+    stwo_twiddles(log_n, f, i);
+    if (fwd) memcpy(fwd, f.data(), f.size() * 4);
+    if (inv_out) memcpy(inv_out, i.data(), i.size() * 4);
+}
+void orc_stwo_ntt_evaluate(uint32_t* data, const uint32_t* twiddles, uint32_t log_n) {
+    stwo_ntt(data, twiddles, log_n, true);
+}
+void orc_stwo_ntt_interpolate(uint32_t* data, const uint32_t* itwiddles, uint32_t log_n) {
+    stwo_ntt(data, itwiddles, log_n, false);
+}
+void orc_lifted_leaves(const uint32_t* const* cols, const uint32_t* log_sizes, uint32_t n_cols, uint32_t lifting_log,
+                       uint32_t* out) {
+    lifted_leaves(cols, log_sizes, n_cols, lifting_log, out);
+}
+void orc_pack_leaves(const uint32_t* const* in4, uint32_t* const* out64, uint64_t n) {
+    // cuda/pack_leaves.cu:22-38: out[c*16 + p][i] = in[c][16 i + p]
+    for (uint32_t c = 0; c < 4; c++)
+        for (uint64_t j = 0; j < n; j++) out64[c * 16 + (j % 16)][j / 16] = in4[c][j];
+}
+void orc_bit_reverse_qm31(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t log_n) {
+    // cuda/bit_reverse_wide.cu:21-40
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t j = bitrev((uint32_t)i, log_n);
+        for (int k = 0; k < 4; k++) out[4 * j + k] = in[4 * i + k];
+    }
+}
+uint32_t orc_wide_commit(const uint32_t* const* trace, const uint32_t* log_sizes, uint32_t n_cols, uint32_t blowup,
+                         uint32_t log_last, uint32_t* root8, uint32_t* fri_roots, uint32_t* last, uint32_t* beta4) {
+    std::vector<std::vector<uint32_t>> tr(n_cols);
+    std::vector<uint32_t> logs(log_sizes, log_sizes + n_cols);
+    for (uint32_t c = 0; c < n_cols; c++) tr[c].assign(trace[c], trace[c] + (1ull << logs[c]));
+    WideOut o;
+    wide_commit(tr, logs, blowup, log_last, o, nullptr);
+    if (root8) memcpy(root8, o.root, 32);
+    if (fri_roots)
+        for (size_t j = 0; j < o.fri_roots.size(); j++) memcpy(fri_roots + 8 * j, o.fri_roots[j].data(), 32);
+    if (last) memcpy(last, o.last.data(), o.last.size() * 4);
+    if (beta4) memcpy(beta4, o.beta, 16);
+    return (uint32_t)o.fri_roots.size();
+}
+double orc_time_wide_commit(const uint32_t* const* trace, const uint32_t* log_sizes, uint32_t n_cols, uint32_t blowup,
+                            uint32_t log_last) {
+    std::vector<std::vector<uint32_t>> tr(n_cols);
+    std::vector<uint32_t> logs(log_sizes, log_sizes + n_cols);
+    for (uint32_t c = 0; c < n_cols; c++) tr[c].assign(trace[c], trace[c] + (1ull << logs[c]));
+    WideOut o;
+    auto t0 = std::chrono::steady_clock::now();
+    wide_commit(tr, logs, blowup, log_last, o, nullptr);
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 uint32_t orc_lifted_row(uint32_t leaf, uint32_t lifting_log, uint32_t col_log) {

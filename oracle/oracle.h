@@ -112,6 +112,21 @@ int orc_verify_krak(const uint8_t* proof, uint64_t len, uint32_t n_queries, uint
 
 /* ---- stwo-basis pieces for the wide/lifted commit ---- */
 uint32_t orc_lifted_row(uint32_t leaf, uint32_t lifting_log, uint32_t col_log);
+/* twiddles of a stwo-basis NTT of size 2^log_n (src/ntt.rs:262-324): 2^(log_n-1) words each */
+void orc_stwo_twiddles(uint32_t log_n, uint32_t* fwd, uint32_t* inv_out);
+void orc_stwo_ntt_evaluate(uint32_t* data, const uint32_t* twiddles, uint32_t log_n);
+void orc_stwo_ntt_interpolate(uint32_t* data, const uint32_t* itwiddles, uint32_t log_n);
+/* lifted leaf hashes of columns in the given order (merkle_leaves_lifted.cu) */
+void orc_lifted_leaves(const uint32_t* const* cols, const uint32_t* log_sizes, uint32_t n_cols, uint32_t lifting_log,
+                       uint32_t* out);
+void orc_pack_leaves(const uint32_t* const* in4, uint32_t* const* out64, uint64_t n);
+void orc_bit_reverse_qm31(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t log_n);
+/* batched wide commitment (stwo LDE + lifted tree + FRI on a beta-combination); returns
+ * the number of FRI roots written to fri_roots (8 words each); last gets 4 << log_last words. */
+uint32_t orc_wide_commit(const uint32_t* const* trace, const uint32_t* log_sizes, uint32_t n_cols, uint32_t blowup,
+                         uint32_t log_last, uint32_t* root8, uint32_t* fri_roots, uint32_t* last, uint32_t* beta4);
+double orc_time_wide_commit(const uint32_t* const* trace, const uint32_t* log_sizes, uint32_t n_cols, uint32_t blowup,
+                            uint32_t log_last);
 
 /* ---- splitmix64 input generator ---- */
 uint64_t orc_splitmix64(uint64_t* state);

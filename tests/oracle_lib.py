@@ -62,6 +62,14 @@ _SIGS = {
     "orc_time_prove_fibonacci": (C.c_double, [u32, u32, u32, u32, u32, u32]),
     "orc_verify_krak": (i32, [vp, u64, u32, u32]),
     "orc_lifted_row": (u32, [u32, u32, u32]),
+    "orc_stwo_twiddles": (None, [u32, vp, vp]),
+    "orc_stwo_ntt_evaluate": (None, [vp, vp, u32]),
+    "orc_stwo_ntt_interpolate": (None, [vp, vp, u32]),
+    "orc_lifted_leaves": (None, [vp, vp, u32, u32, vp]),
+    "orc_pack_leaves": (None, [vp, vp, u64]),
+    "orc_bit_reverse_qm31": (None, [vp, vp, u64, u32]),
+    "orc_wide_commit": (u32, [vp, vp, u32, u32, u32, P32, vp, vp, vp]),
+    "orc_time_wide_commit": (C.c_double, [vp, vp, u32, u32, u32]),
     "orc_splitmix64": (u64, [C.POINTER(u64)]),
 }
 

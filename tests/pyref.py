@@ -312,3 +312,122 @@ def prove(a: int, b: int, log_n: int, n_queries: int = 80, log_last: int = 3) ->
         out += decom(t, lay, fi, 4)
         fi = [x // 2 for x in fi]
     return w2b(out)
+
+
+# ------------------------------------------------------------------ stwo basis (tiny sizes)
+def stwo_twiddles(log_n: int):
+    """Flat stwo twiddle buffer of a size-2^log_n domain, from exponents: layer j holds
+    x(half_odds(log_n-1-j).at(bitrev(i))) for i < 2^(log_n-2-j), then a final 1."""
+    out = []
+    for j in range(log_n - 1):
+        k = log_n - 1 - j
+        m = k - 1
+        out += [gpow(half_odds_exp(k, bitrev(i, m)))[0] for i in range(1 << m)]
+    out.append(1)
+    return out
+
+
+def stwo_point(log_n: int, k: int):
+    """Canonic circle-domain point k (natural order) of size 2^log_n."""
+    half = 1 << (log_n - 1)
+    if k < half:
+        return gpow(half_odds_exp(log_n - 1, k))
+    x, y = gpow(half_odds_exp(log_n - 1, k - half))
+    return (x, (P - y) % P)
+
+
+def eval_circle_poly(coeffs, point):
+    """stwo circle-polynomial evaluation by recursive folding (src/oods.rs:133-215):
+    factors [pi^(k-2)(x), ..., pi(x), x, y] for the halves from outermost to innermost."""
+    k = len(coeffs).bit_length() - 1
+    x, y = point
+    raw = [y]
+    for _ in range(1, k):
+        raw.append(x)
+        x = (2 * x * x - 1) % P
+    factors = raw[::-1]
+
+    def fold(vals, fs):
+        if len(vals) == 1:
+            return vals[0] % P
+        h = len(vals) // 2
+        return (fold(vals[:h], fs[1:]) + fs[0] * fold(vals[h:], fs[1:])) % P
+
+    return fold(list(coeffs), factors)
+
+
+def stwo_evaluate(coeffs, log_n: int):
+    """Evaluations at the canonic domain in bit-reversed order (what the stwo NTT outputs)."""
+    return [eval_circle_poly(coeffs, stwo_point(log_n, bitrev(j, log_n))) for j in range(1 << log_n)]
+
+
+def stwo_interpolate(values, log_n: int):
+    n = 1 << log_n
+    rows = []
+    for j in range(n):
+        pt = stwo_point(log_n, bitrev(j, log_n))
+        basis = [eval_circle_poly([1 if t == i else 0 for t in range(n)], pt) for i in range(n)]
+        rows.append(basis + [values[j] % P])
+    for col in range(n):
+        piv = next(r for r in range(col, n) if rows[r][col])
+        rows[col], rows[piv] = rows[piv], rows[col]
+        iv = inv(rows[col][col])
+        rows[col] = [v * iv % P for v in rows[col]]
+        for r in range(n):
+            if r != col and rows[r][col]:
+                f = rows[r][col]
+                rows[r] = [(a - f * b) % P for a, b in zip(rows[r], rows[col])]
+    return [rows[r][n] for r in range(n)]
+
+
+def lifted_row(leaf: int, S: int, L: int) -> int:
+    if L == S:
+        return leaf
+    sh = S - L + 1
+    return ((leaf >> sh) << 1) | (leaf & 1)
+
+
+def lifted_leaf(cols, logs, S, leaf):
+    return h2w(hashlib.blake2s(w2b([cols[c][lifted_row(leaf, S, logs[c])] for c in range(len(cols))])).digest())
+
+
+def wide_commit(trace_cols, logs, blowup=2, log_last=2):
+    """Batched wide commitment (tiny sizes): stwo LDE of each column, lifted tree, then FRI
+    on f = sum_c beta^c lifted_c with the canonic-domain circle fold and half_odds line folds."""
+    ev = []
+    for col, l in zip(trace_cols, logs):
+        coef = stwo_interpolate(col, l)
+        ev.append(stwo_evaluate(coef + [0] * ((1 << (l + blowup)) - (1 << l)), l + blowup))
+    elogs = [l + blowup for l in logs]
+    S = max(elogs)
+    n = 1 << S
+    leaves = [lifted_leaf(ev, elogs, S, i) for i in range(n)]
+    root = merkle_layers(leaves)[-1][0]
+    ch = Channel()
+    ch.mix_digest(root)
+    beta = ch.draw_felt()
+    pw = [(1, 0, 0, 0)]
+    for _ in range(len(ev) - 1):
+        pw.append(qmul(pw[-1], beta))
+    f = []
+    for i in range(n):
+        acc = (0, 0, 0, 0)
+        for c in range(len(ev)):
+            acc = qadd(acc, qscale(pw[c], ev[c][lifted_row(i, S, elogs[c])]))
+        f.append(acc)
+    tw = [inv(gpow(half_odds_exp(S - 1, bitrev(i, S - 1)))[1]) for i in range(n // 2)]
+    a0 = ch.draw_felt()
+    layers = [fold(f, a0, tw)]
+    roots = []
+    t = merkle_layers([leaf_hash(list(v)) for v in layers[0]])
+    roots.append(t[-1][0])
+    ch.mix_digest(roots[-1])
+    cur = S - 1
+    while cur > log_last:
+        al = ch.draw_felt()
+        layers.append(fold(layers[-1], al, fold_twiddles_line(cur)))
+        cur -= 1
+        t = merkle_layers([leaf_hash(list(v)) for v in layers[-1]])
+        roots.append(t[-1][0])
+        ch.mix_digest(roots[-1])
+    return root, roots, [w for v in layers[-1] for w in v], beta
