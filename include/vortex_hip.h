@@ -17,9 +17,13 @@
  * It produces the same KRAK v1 proof bytes as the reference's prove() +
  * serialize_proof (src/prover.rs:1721-1978, src/bin/stark_cli.rs:907-975).
  *
+ * Part 4 is the batched wide commitment: many M31 columns of mixed sizes
+ * through the stwo-basis LDE, one lifted Blake2s tree and fused FRI
+ * fold+commit layers.
+ *
  * Conventions shared by every entry point:
  *  - device buffers are allocated by the caller; kernels never allocate,
- *    except the vx_prover_* objects, which own their workspace;
+ *    except the vx_prover_* and vx_wide_* objects, which own their workspace;
  *  - "(host)" arguments are read before the call returns;
  *  - launches go to the calling thread's current stream (vx_set_stream,
  *    default: the null stream) unless a stream argument is given;
@@ -122,6 +126,29 @@ void cuda_merkle_tiled_soa4_stream(const uint32_t* col0, const uint32_t* col1, c
 void cuda_merkle_tiled_generic(const uint32_t* const* columns, uint32_t* subtree_roots, uint32_t n_cols,
                                uint32_t n_leaves);                                                 /* ffi.rs:691 */
 
+/* stwo-basis circle NTT over the flat twiddle buffer of StwoNttCache::new
+ * (ffi.rs:234-239, cuda/circle_ntt_stwo.cu:268-402; twiddles ntt.rs:262-324):
+ * evaluate maps coefficients to values on the canonic domain in bit-reversed
+ * order, interpolate inverts it including the 1/n scale. Complete on return.
+ * n must be a power of two >= 4 (n = 2 would read past the 1-word buffer). */
+void cuda_stwo_ntt_evaluate(uint32_t* d_data, const uint32_t* d_twiddles, uint32_t n);           /* ffi.rs:236 */
+void cuda_stwo_ntt_interpolate(uint32_t* d_data, const uint32_t* d_itwiddles, uint32_t n);       /* ffi.rs:238 */
+
+/* Lifted multi-column Blake2s leaves (ffi.rs:769-776, cuda/merkle_leaves_lifted.cu):
+ * col_ptrs is a device array of column pointers, schedule a device array of
+ * n_chunks LeafHashChunk {u32 col_indices[16]; u32 n_cols; u32 log_size} (72 bytes
+ * each, at most 64 chunks). Leaf i hashes the columns in schedule order, each at
+ * its lifted row. output_hashes receives 8 words per leaf. */
+void cuda_build_leaves_lifted(const uint32_t* const* col_ptrs, const void* schedule, uint32_t n_chunks,
+                              uint32_t lifting_log_size, uint32_t* output_hashes, uint32_t n_leaves); /* :769 */
+
+/* QM31 bit reversal out of place (4 words per element) and the 4 -> 64 column
+ * leaf packing of stwo's PackLeavesOps (ffi.rs:956-965, cuda/bit_reverse_wide.cu,
+ * cuda/pack_leaves.cu): output_cols[c*16 + p][i] = input_cols[c][16 i + p]. */
+void cuda_bit_reverse_qm31(const uint32_t* inp, uint32_t* out, uint32_t n, uint32_t log_n);       /* ffi.rs:956 */
+void cuda_pack_leaves(const uint32_t* const* input_col_ptrs, uint32_t* const* output_col_ptrs,
+                      uint32_t n);                                                                 /* ffi.rs:960 */
+
 /* ===================== Part 2: runtime (replaces ffi.rs:6-76 cuda* calls) ==== */
 
 int32_t vx_get_last_error(void);                 /* 0 = none; reading clears it */
@@ -190,6 +217,45 @@ float vx_prover_timeline_ms(vx_prover* p, uint32_t i);
  * words) and of FRI layer j (4 SoA columns of 2^(log_n+1-j) words each). */
 const uint32_t* vx_prover_eval_device(const vx_prover* p);
 const uint32_t* vx_prover_fri_layer_device(const vx_prover* p, uint32_t j, uint32_t column);
+
+/* ===================== Part 4: batched wide commitment ===================== */
+
+/* Builds the stwo flat twiddle buffer of a domain of size 2^log_n on the device
+ * (2^(log_n-1) words; ntt.rs:262-324) and, when d_inv is not NULL, its inverse. */
+int32_t vx_stwo_twiddles(uint32_t log_n, uint32_t* d_fwd, uint32_t* d_inv, void* stream);
+
+typedef struct vx_wide vx_wide;
+
+/* A wide commitment context for n_cols M31 trace columns of 2^log_sizes[c] rows
+ * (ascending, as stwo orders tree columns, merkle_ops.rs:49), LDE blowup
+ * 2^log_blowup, FRI down to 2^log_last_layer. Per commitment, on the
+ * context's stream:
+ *   1. every column: stwo interpolate on its canonic domain, evaluate on the
+ *      2^log_blowup larger one (poly_ops.rs:242-260, 496-535);
+ *   2. one lifted Blake2s tree over all LDE columns, S = max log size +
+ *      log_blowup leaves (merkle_leaves_lifted.cu:26-98); fresh transcript:
+ *      mix_digest(root), beta = draw_felt, alpha_0 = draw_felt;
+ *   3. f = sum_c beta^c * lifted_c on the 2^S canonic domain, circle-folded
+ *      with alpha_0 and 1/y of its points (fri_ops.rs:40-78), committed as SoA4;
+ *   4. line folds on half_odds domains (fri.rs:143-224), each committed, each
+ *      root mixed and the next alpha drawn, down to 2^log_last_layer values. */
+vx_wide* vx_wide_create(uint32_t n_cols, const uint32_t* log_sizes, uint32_t log_blowup, uint32_t log_last_layer);
+void vx_wide_destroy(vx_wide* w);
+void* vx_wide_stream(vx_wide* w);
+uint32_t vx_wide_n_fri_layers(const vx_wide* w);      /* S - log_last_layer */
+uint32_t vx_wide_lifting_log_size(const vx_wide* w);  /* S */
+/* Enqueues one commitment; h_cols is a (host) array of n_cols device pointers. */
+int32_t vx_wide_commit_device(vx_wide* w, const uint32_t* const* h_cols);
+/* Waits and copies results: lifted root (8 words), FRI roots (8 per layer), the
+ * last layer (4 words per value, interleaved) and beta (4 words); any may be NULL. */
+int32_t vx_wide_results(vx_wide* w, uint32_t* root8, uint32_t* fri_roots, uint32_t* last, uint32_t* beta4);
+/* Device addresses of LDE column c and of FRI layer j (4 SoA columns). */
+const uint32_t* vx_wide_lde_device(const vx_wide* w, uint32_t c);
+const uint32_t* vx_wide_fri_layer_device(const vx_wide* w, uint32_t j, uint32_t column);
+int32_t vx_wide_set_timing(vx_wide* w, int32_t enable);
+uint32_t vx_wide_timeline_count(const vx_wide* w);
+const char* vx_wide_timeline_label(const vx_wide* w, uint32_t i);
+float vx_wide_timeline_ms(vx_wide* w, uint32_t i);
 
 #ifdef __cplusplus
 }

@@ -1436,6 +1436,9 @@ void orc_bit_reverse_qm31(const uint32_t* in, uint32_t* out, uint64_t n, uint32_
 }
 uint32_t orc_wide_commit(const uint32_t* const* trace, const uint32_t* log_sizes, uint32_t n_cols, uint32_t blowup,
                          uint32_t log_last, uint32_t* root8, uint32_t* fri_roots, uint32_t* last, uint32_t* beta4) {
+    // columns of one lifted tree come in ascending log size (merkle_ops.rs:49)
+    for (uint32_t c = 1; c < n_cols; c++)
+        if (log_sizes[c] < log_sizes[c - 1]) return 0;
     std::vector<std::vector<uint32_t>> tr(n_cols);
     std::vector<uint32_t> logs(log_sizes, log_sizes + n_cols);
     for (uint32_t c = 0; c < n_cols; c++) tr[c].assign(trace[c], trace[c] + (1ull << logs[c]));

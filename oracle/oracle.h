@@ -121,8 +121,9 @@ void orc_lifted_leaves(const uint32_t* const* cols, const uint32_t* log_sizes, u
                        uint32_t* out);
 void orc_pack_leaves(const uint32_t* const* in4, uint32_t* const* out64, uint64_t n);
 void orc_bit_reverse_qm31(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t log_n);
-/* batched wide commitment (stwo LDE + lifted tree + FRI on a beta-combination); returns
- * the number of FRI roots written to fri_roots (8 words each); last gets 4 << log_last words. */
+/* batched wide commitment (stwo LDE + lifted tree + FRI on a beta-combination); log_sizes
+ * ascending; returns the number of FRI roots written to fri_roots (8 words each; 0 when the
+ * sizes are not ascending); last gets 4 << log_last words. */
 uint32_t orc_wide_commit(const uint32_t* const* trace, const uint32_t* log_sizes, uint32_t n_cols, uint32_t blowup,
                          uint32_t log_last, uint32_t* root8, uint32_t* fri_roots, uint32_t* last, uint32_t* beta4);
 double orc_time_wide_commit(const uint32_t* const* trace, const uint32_t* log_sizes, uint32_t n_cols, uint32_t blowup,

@@ -11,7 +11,12 @@ reference-protocol and configured runs of the plain CPU workload (one
 Fibonacci column, 2^16 rows, a/b from splitmix64 seed 1), and the plain GPU
 workload (2^24 rows, a/b from splitmix64 seed 2, 100 queries, last layer 2^4).
 
-    python tests/golden/make_golden.py [--skip-large]
+tests/golden/wide.json holds the batched wide commitment of 34 uniform M31
+columns (4 x 2^14, 10 x 2^18, 20 x 2^20 from splitmix64 seed 3, blowup 4,
+last layer 2^4): lifted root, FRI roots, last layer, beta and the SHA-256 of
+every LDE column.
+
+    python tests/golden/make_golden.py [--skip-large] [--wide-only]
 """
 
 from __future__ import annotations
@@ -26,6 +31,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
+import inputs  # noqa: E402
 import krak  # noqa: E402
 import oracle_lib  # noqa: E402
 
@@ -54,10 +60,33 @@ def case(a, b, log_n, nq, last, label):
     }
 
 
+def wide_case(label, seed, log_sizes, log_blowup, log_last):
+    t = time.time()
+    cols = inputs.m31_columns(seed, log_sizes)
+    root, fri, last, beta = oracle_lib.wide_commit(cols, log_sizes, log_blowup, log_last)
+    ldes = [hashlib.sha256(oracle_lib.stwo_lde(c, l, log_blowup).tobytes()).hexdigest()
+            for c, l in zip(cols, log_sizes)]
+    print(f"{label}: {len(cols)} columns, {time.time() - t:.1f}s", flush=True)
+    return {"label": label, "seed": seed, "log_sizes": list(log_sizes), "log_blowup": log_blowup,
+            "log_last": log_last, "root": [int(x) for x in root], "fri_roots": fri.tolist(),
+            "last_layer": last.tolist(), "beta": [int(x) for x in beta], "lde_sha256": ldes}
+
+
+def write_wide():
+    cases = [wide_case("wide_config", inputs.WIDE_SEED, inputs.WIDE_LOG_SIZES, 2, 4),
+             wide_case("wide_small", 12, [5, 5, 7, 8, 8, 8], 2, 2)]
+    with open(os.path.join(HERE, "wide.json"), "w") as f:
+        json.dump(cases, f, separators=(",", ":"))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-large", action="store_true")
+    ap.add_argument("--wide-only", action="store_true")
     args = ap.parse_args()
+    write_wide()
+    if args.wide_only:
+        return
     cases = []
     for log_n in (4, 5, 6, 8):
         for a, b in ((1, 1), (2, 3), (42, 99), (1000, 2000)):

@@ -203,3 +203,37 @@ def fold_circle(cols4, tw, alpha) -> np.ndarray:
 def splitmix64_stream(seed: int, count: int):
     s = u64(seed)
     return [lib().orc_splitmix64(C.byref(s)) for _ in range(count)]
+
+
+def stwo_twiddles(log_n: int):
+    half = max(1, (1 << log_n) // 2)
+    f, i = np.zeros(half, np.uint32), np.zeros(half, np.uint32)
+    lib().orc_stwo_twiddles(log_n, f.ctypes.data, i.ctypes.data)
+    return f, i
+
+
+def stwo_lde(col, log_n: int, log_blowup: int) -> np.ndarray:
+    """stwo interpolate on the size-2^log_n canonic domain, zero-extend, evaluate on 2^(log_n+blowup)."""
+    _, itw = stwo_twiddles(log_n)
+    etw, _ = stwo_twiddles(log_n + log_blowup)
+    d = np.zeros(1 << (log_n + log_blowup), np.uint32)
+    d[: 1 << log_n] = arr(col)
+    lib().orc_stwo_ntt_interpolate(d.ctypes.data, itw.ctypes.data, log_n)
+    lib().orc_stwo_ntt_evaluate(d.ctypes.data, etw.ctypes.data, log_n + log_blowup)
+    return d
+
+
+def wide_commit(cols, log_sizes, log_blowup: int, log_last: int):
+    """(root[8], fri_roots[n][8], last[2^log_last][4], beta[4]) of the batched wide commitment."""
+    cols = [arr(c) for c in cols]
+    ptrs = (vp * len(cols))(*[c.ctypes.data for c in cols])
+    lg = arr(log_sizes)
+    root = np.zeros(8, np.uint32)
+    S = max(log_sizes) + log_blowup
+    fri = np.zeros(8 * S, np.uint32)
+    last = np.zeros(4 << log_last, np.uint32)
+    beta = np.zeros(4, np.uint32)
+    nf = lib().orc_wide_commit(ptrs, lg.ctypes.data, len(cols), log_blowup, log_last, p32(root), fri.ctypes.data,
+                               last.ctypes.data, beta.ctypes.data)
+    assert nf == S - log_last, "oracle rejected the wide-commit shape"
+    return root, fri[: 8 * nf].reshape(-1, 8), last.reshape(-1, 4), beta

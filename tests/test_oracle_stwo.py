@@ -133,3 +133,23 @@ def test_wide_commit_against_python_model(oracle):
     assert [list(r) for r in fri] == roots2
     assert list(last) == last2
     assert tuple(beta) == beta2
+
+
+def test_splitmix_inputs_match_oracle_generator(oracle):
+    import inputs
+
+    ref = oracle.splitmix64_stream(7, 40)
+    assert [int(x) for x in inputs.splitmix64(7, 40)] == ref
+    assert [int(x) for x in inputs.splitmix64(7, 10, 30)] == ref[30:]
+    cols = inputs.m31_columns(9, [2, 3])
+    s = oracle.splitmix64_stream(9, 12)
+    assert [int(x) for x in cols[0]] == [v % P for v in s[:4]]
+    assert [int(x) for x in cols[1]] == [v % P for v in s[4:]]
+
+
+def test_wide_commit_rejects_unsorted_sizes(oracle):
+    cols = [np.zeros(16, np.uint32), np.zeros(8, np.uint32)]
+    ptrs = (C.c_void_p * 2)(*[c.ctypes.data for c in cols])
+    lg = np.array([4, 3], np.uint32)
+    root = np.zeros(8, np.uint32)
+    assert oracle.lib().orc_wide_commit(ptrs, lg.ctypes.data, 2, 2, 2, oracle.p32(root), None, None, None) == 0

@@ -4,6 +4,7 @@ Host-side mirror of the reference's prover interface (src/prover.rs:192-231):
 
     prove(a, b, log_n)              -> KRAK v1 proof bytes  (prove + serialize_proof)
     Prover(log_n).prove(a, b)       -> same, with the ProverCache reused
+    WideCommit(log_sizes)           -> batched stwo-basis LDE + lifted tree + FRI commitments
     DeviceBuffer                    -> device memory owned from Python
 
 Everything runs in libvortex_hip.so (HIP kernels for gfx950 behind a C ABI).
@@ -175,3 +176,90 @@ def fibonacci_trace_device(a: int, b: int, log_n: int) -> DeviceBuffer:
         check("vx_fibonacci_trace")
     synchronize()
     return buf
+
+
+def _timeline(count, label, ms, handle) -> dict:
+    out = {}
+    for i in range(1, count(handle)):
+        key = label(handle, i).decode()
+        out[key] = out.get(key, 0.0) + ms(handle, i)
+    return out
+
+
+class WideCommit:
+    """Batched wide commitment over M31 columns of mixed sizes (Part 4 of the C ABI).
+
+    Columns are committed in ascending log-size order, the order stwo gives the
+    columns of one lifted tree (crates/cuda-backend/src/merkle_ops.rs:49).
+    """
+
+    def __init__(self, log_sizes, log_blowup: int = BLOWUP_BITS, log_last_layer: int = 4):
+        self.log_sizes = [int(x) for x in log_sizes]
+        arr = (C.c_uint32 * len(self.log_sizes))(*self.log_sizes)
+        self.handle = lib.vx_wide_create(len(self.log_sizes), arr, log_blowup, log_last_layer)
+        if not self.handle:
+            check("vx_wide_create")
+            raise HipError("vx_wide_create failed")
+        self.log_blowup = log_blowup
+        self.log_last_layer = log_last_layer
+        self.n_fri = int(lib.vx_wide_n_fri_layers(self.handle))
+        self.lifting_log_size = int(lib.vx_wide_lifting_log_size(self.handle))
+
+    @property
+    def stream(self) -> int:
+        return lib.vx_wide_stream(self.handle) or 0
+
+    def commit_device(self, col_addrs) -> None:
+        ptrs = (C.c_void_p * len(col_addrs))(*col_addrs)
+        if lib.vx_wide_commit_device(self.handle, ptrs):
+            check("vx_wide_commit_device")
+
+    def results(self):
+        """(root[8], fri_roots[n_fri][8], last[2^log_last][4], beta[4]) as numpy arrays."""
+        root = np.zeros(8, np.uint32)
+        fri = np.zeros(8 * self.n_fri, np.uint32)
+        last = np.zeros(4 << self.log_last_layer, np.uint32)
+        beta = np.zeros(4, np.uint32)
+        if lib.vx_wide_results(self.handle, root.ctypes.data, fri.ctypes.data, last.ctypes.data, beta.ctypes.data):
+            check("vx_wide_results")
+            raise HipError("vx_wide_results failed")
+        return root, fri.reshape(-1, 8), last.reshape(-1, 4), beta
+
+    def lde(self, c: int) -> np.ndarray:
+        synchronize_stream(self.stream)
+        n = 1 << (self.log_sizes[c] + self.log_blowup)
+        return device_copy_words(lib.vx_wide_lde_device(self.handle, c), n)
+
+    def set_timing(self, on: bool) -> None:
+        lib.vx_wide_set_timing(self.handle, 1 if on else 0)
+
+    def timeline(self) -> dict:
+        return _timeline(lib.vx_wide_timeline_count, lib.vx_wide_timeline_label, lib.vx_wide_timeline_ms,
+                         self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            lib.vx_wide_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize_stream(stream: int) -> None:
+    if lib.vx_stream_synchronize(C.c_void_p(stream)):
+        check("vx_stream_synchronize")
+
+
+def stwo_twiddles_device(log_n: int):
+    """stwo flat twiddle buffers (forward, inverse; 2^(log_n-1) words each) of a
+    size-2^log_n domain, built on the device."""
+    half = max(1, (1 << log_n) // 2)
+    fwd, inv = DeviceBuffer(half), DeviceBuffer(half)
+    if lib.vx_stwo_twiddles(log_n, fwd.ptr, inv.ptr, None):
+        check("vx_stwo_twiddles")
+    synchronize()
+    return fwd, inv

@@ -53,6 +53,11 @@ SIGNATURES = {
     "cuda_merkle_tiled_soa4": (None, [vp, vp, vp, vp, vp, u32]),
     "cuda_merkle_tiled_soa4_stream": (None, [vp, vp, vp, vp, vp, u32, vp]),
     "cuda_merkle_tiled_generic": (None, [vp, vp, u32, u32]),
+    "cuda_stwo_ntt_evaluate": (None, [vp, vp, u32]),
+    "cuda_stwo_ntt_interpolate": (None, [vp, vp, u32]),
+    "cuda_build_leaves_lifted": (None, [vp, vp, u32, u32, vp, u32]),
+    "cuda_bit_reverse_qm31": (None, [vp, vp, u32, u32]),
+    "cuda_pack_leaves": (None, [vp, vp, u32]),
     # Part 2: runtime
     "vx_get_last_error": (i32, []),
     "vx_get_last_error_message": (C.c_char_p, []),
@@ -89,6 +94,21 @@ SIGNATURES = {
     "vx_prover_timeline_count": (u32, [vp]),
     "vx_prover_timeline_label": (C.c_char_p, [vp, u32]),
     "vx_prover_timeline_ms": (C.c_float, [vp, u32]),
+    # Part 4: batched wide commitment
+    "vx_stwo_twiddles": (i32, [u32, vp, vp, vp]),
+    "vx_wide_create": (vp, [u32, P32, u32, u32]),
+    "vx_wide_destroy": (None, [vp]),
+    "vx_wide_stream": (vp, [vp]),
+    "vx_wide_n_fri_layers": (u32, [vp]),
+    "vx_wide_lifting_log_size": (u32, [vp]),
+    "vx_wide_commit_device": (i32, [vp, vp]),
+    "vx_wide_results": (i32, [vp, vp, vp, vp, vp]),
+    "vx_wide_lde_device": (vp, [vp, u32]),
+    "vx_wide_fri_layer_device": (vp, [vp, u32, u32]),
+    "vx_wide_set_timing": (i32, [vp, i32]),
+    "vx_wide_timeline_count": (u32, [vp]),
+    "vx_wide_timeline_label": (C.c_char_p, [vp, u32]),
+    "vx_wide_timeline_ms": (C.c_float, [vp, u32]),
 }
 
 

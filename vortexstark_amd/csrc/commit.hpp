@@ -1,0 +1,74 @@
+// Host-side driver of the Merkle engine: commits a leaf source with one stage
+// kernel over the leaves, stage kernels over node layers while more than
+// PROVER_TAIL_ITEMS nodes remain, and the tail kernel (root + transcript).
+#pragma once
+#include "common.hpp"
+#include "merkle_engine.hpp"
+
+namespace vx {
+
+inline uint32_t pick_logc(uint64_t n) {
+    const uint64_t MIN_WAVES = 1024;
+    for (int c = (int)MERKLE_LOGC_MAX; c > 0; c--)
+        if ((n >> (7 + c)) >= MIN_WAVES) return (uint32_t)c;
+    return 0;
+}
+
+// Trees of at most this many items go straight to the tail kernel; larger
+// ones are reduced by stage kernels until this many nodes remain.
+constexpr uint32_t PROVER_TAIL_ITEMS = 1024;
+
+struct Scratch {
+    uint32_t* a;
+    uint32_t* b;
+};
+
+// Commits n_leaves (power of two >= 2) items of a leaf source.
+template <class Src, class Mark>
+void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, Scratch sc, hipStream_t s,
+                        Mark&& mark) {
+    if (n_leaves <= PROVER_TAIL_ITEMS) {
+        merkle_tail_kernel<Src><<<1, TAIL_THREADS, 0, s>>>(src, 0, (uint32_t)n_leaves, st, out);
+        check_launch("merkle_tail_kernel");
+        mark("tail");
+        return;
+    }
+    uint32_t logc = pick_logc(n_leaves);
+    uint64_t waves = n_leaves >> (7 + logc);
+    merkle_stage_kernel<Src><<<ceil_div(waves, MERKLE_STAGE_WAVES), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
+        src, 0, n_leaves, logc, sc.a, st);
+    check_launch("merkle_stage_kernel");
+    mark("stage_leaf");
+    uint64_t n = waves;  // one node per wave
+    uint32_t lvl = 7 + logc;
+    uint32_t* cur = sc.a;
+    uint32_t* nxt = sc.b;
+    while (n > PROVER_TAIL_ITEMS) {
+        logc = pick_logc(n);
+        waves = n >> (7 + logc);
+        merkle_stage_kernel<SrcNodes><<<ceil_div(waves, MERKLE_STAGE_WAVES), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
+            SrcNodes{cur}, lvl, n, logc, nxt, st);
+        check_launch("merkle_stage_kernel<nodes>");
+        mark("stage_node");
+        n = waves;
+        lvl += 7 + logc;
+        uint32_t* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    merkle_tail_kernel<SrcNodes><<<1, TAIL_THREADS, 0, s>>>(SrcNodes{cur}, lvl, (uint32_t)n, st, out);
+    check_launch("merkle_tail_kernel<nodes>");
+    mark("tail");
+}
+
+inline uint64_t max_stage_nodes(uint64_t n_leaves) {
+    if (n_leaves <= PROVER_TAIL_ITEMS) return 0;
+    uint64_t mx = 0, n = n_leaves;
+    while (n > PROVER_TAIL_ITEMS) {
+        n >>= 7 + pick_logc(n);
+        if (n > mx) mx = n;
+    }
+    return mx;
+}
+
+}  // namespace vx

@@ -11,6 +11,7 @@
 
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/vortex_hip.h"
 
@@ -37,5 +38,41 @@ inline uint32_t ilog2(uint64_t n) {
     return l;
 }
 inline bool is_pow2(uint64_t n) { return n && !(n & (n - 1)); }
+
+// Optional launch timeline of a context object: HIP events recorded on its
+// stream after labelled launches; entry i > 0 spans marks i-1 .. i.
+struct Timeline {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<std::string> labels;
+    size_t n = 0;
+    void reset(bool enable) {
+        on = enable;
+        n = 0;
+    }
+    void mark(const std::string& label, hipStream_t s) {
+        if (!on) return;
+        if (n == ev.size()) {
+            hipEvent_t e;
+            if (!check(hipEventCreate(&e), "hipEventCreate")) return;
+            ev.push_back(e);
+            labels.emplace_back();
+        }
+        labels[n] = label;
+        check(hipEventRecord(ev[n], s), "hipEventRecord");
+        n++;
+    }
+    const char* label(uint32_t i) const { return i < n ? labels[i].c_str() : ""; }
+    float ms(uint32_t i) {
+        if (i == 0 || i >= n) return 0.f;
+        float t = 0.f;
+        if (!check(hipEventSynchronize(ev[i]), "hipEventSynchronize")) return -1.f;
+        if (!check(hipEventElapsedTime(&t, ev[i - 1], ev[i]), "hipEventElapsedTime")) return -1.f;
+        return t;
+    }
+    ~Timeline() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
 
 }  // namespace vx

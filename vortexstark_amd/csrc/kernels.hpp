@@ -18,7 +18,11 @@ struct NttTables {
     const uint32_t* circle = nullptr;
     uint32_t offs[32] = {0};
     uint32_t log_n = 0;
+    bool stwo = false;  // line is a flat stwo twiddle buffer; circle twiddles derived from it
 };
+// Tables of the stwo-basis transform of size 2^log_n over a flat twiddle buffer
+// of 2^(log_n-1) words (src/ntt.rs:262-324): layer b at n/2 - n/2^b.
+NttTables stwo_tables(const uint32_t* flat, uint32_t log_n);
 
 // Forward transform (coefficients -> values). Reads src (src_n valid words,
 // the rest taken as zero: the implicit zero pad of cuda/constraint.cu:81-90)
@@ -26,9 +30,13 @@ struct NttTables {
 void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTables& t, hipStream_t s);
 // Inverse transform including the 2^-log_n scale. src may equal dst.
 void ntt_inverse(uint32_t* dst, const uint32_t* src, const NttTables& t, hipStream_t s);
-// Batched forms over a device array of column pointers (in place).
-void ntt_forward_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables& t, hipStream_t s);
-void ntt_inverse_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables& t, hipStream_t s);
+// Batched forms over a device array of column pointers (one launch per pass for
+// all columns). Forward works in place and reads src_n words of each column (the
+// rest taken as zero; 0 = all). Inverse reads d_srcs when given, else in place.
+void ntt_forward_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables& t, hipStream_t s,
+                       uint64_t src_n = 0);
+void ntt_inverse_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables& t, hipStream_t s,
+                       const uint32_t* const* d_srcs = nullptr);
 
 // Builds the twiddles of coset (initial, step) of size 2^log_n directly on the
 // device: line (2^log_n - 1 words), circle (2^(log_n-1) words) and, when the

@@ -233,6 +233,132 @@ struct SrcLineFold {
     }
 };
 
+// Lifted multi-column leaves (cuda/merkle_leaves_lifted.cu:26-98): leaf i hashes,
+// with standard streaming Blake2s (full blocks non-final, the last block final,
+// t = bytes so far), the LE32 words of every column in the given order, column
+// c read at row i when its log size equals the tree's, else at
+// ((i >> sh) << 1) | (i & 1) with sh = S - log_size + 1.
+struct LiftCol {
+    const uint32_t* p;
+    uint32_t sh;  // 0: full-size column
+    uint32_t pad;
+};
+// The column table is read-only for the whole launch and every lane reads the
+// same entry: reading it through the constant address space lets the compiler
+// use scalar loads, and the column data through the global one plain vector
+// loads (a generic pointer would make them flat loads).
+typedef const __attribute__((address_space(4))) LiftCol ConstLiftCol;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) u32x2 GlobalV2;
+typedef const __attribute__((address_space(1))) u32x4 GlobalV4;
+__device__ __forceinline__ ConstLiftCol* const_table(const LiftCol* t) { return (ConstLiftCol*)t; }
+
+struct SrcLifted {
+    const LiftCol* cols;  // device, hashing order
+    uint32_t n_cols;
+    // leaves i (even) and i + 1 read rows r and r + 1 of every column
+    __device__ __forceinline__ void load_block(ConstLiftCol* tab, uint32_t c0, uint64_t i, uint2 v[16]) const {
+        const uint32_t nw = n_cols - c0 < 16 ? n_cols - c0 : 16;
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) {
+            v[k] = make_uint2(0, 0);
+            if (k < nw) {
+                const uint32_t* p = tab[c0 + k].p;
+                const uint32_t sh = tab[c0 + k].sh;
+                const uint64_t r = sh ? ((i >> sh) << 1) : i;
+                const u32x2 x = *(GlobalV2*)(p + r);
+                v[k] = make_uint2(x.x, x.y);
+            }
+        }
+    }
+    __device__ __forceinline__ void pair(uint64_t i, uint32_t h[8]) const {
+        ConstLiftCol* tab = const_table(cols);
+        uint32_t ha[8], hb[8];
+        b2s_init(ha);
+        b2s_init(hb);
+        uint2 cur[16];
+        load_block(tab, 0, i, cur);
+        for (uint32_t c0 = 0;; c0 += 16) {
+            // the next block's loads are in flight while this block is hashed
+            const bool last = c0 + 16 >= n_cols;
+            uint2 nxt[16];
+            if (!last) load_block(tab, c0 + 16, i, nxt);
+            uint32_t ma[16], mb[16];
+#pragma unroll
+            for (uint32_t k = 0; k < 16; k++) {
+                ma[k] = cur[k].x;
+                mb[k] = cur[k].y;
+            }
+            const uint32_t nw = n_cols - c0 < 16 ? n_cols - c0 : 16;
+            const uint32_t t = 4 * (c0 + nw), f = last ? 0xFFFFFFFFu : 0u;
+            b2s_compress(ha, ma, t, f);
+            b2s_compress(hb, mb, t, f);
+            if (last) break;
+#pragma unroll
+            for (uint32_t k = 0; k < 16; k++) cur[k] = nxt[k];
+        }
+        b2s_node(ha, hb, h);
+    }
+};
+
+// First FRI layer of the wide commitment: f = sum_c beta^c * lifted_c is formed
+// on the fly from the LDE columns, circle-folded pairwise (2j, 2j + 1) with
+// alpha and tw[j], stored to out and hashed as an SoA4 leaf.
+struct SrcWideFold {
+    const LiftCol* cols;
+    uint32_t n_cols;
+    const uint32_t* beta_pows;  // device, 4 words per column
+    const uint32_t* alpha;
+    const uint32_t* tw;
+    uint32_t* o0;
+    uint32_t* o1;
+    uint32_t* o2;
+    uint32_t* o3;
+    __device__ __forceinline__ void pair(uint64_t i, uint32_t h[8]) const {
+        const uint64_t j = 2 * i;  // f index of the first of four inputs (a multiple of 4)
+        QM31 f[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) f[k] = {{0, 0, 0, 0}};
+        ConstLiftCol* tab = const_table(cols);
+        typedef const __attribute__((address_space(4))) uint32_t ConstU32;
+        ConstU32* bp = (ConstU32*)beta_pows;
+#pragma unroll 4
+        for (uint32_t c = 0; c < n_cols; c++) {
+            const uint32_t* p = tab[c].p;
+            const uint32_t sh = tab[c].sh;
+            const QM31 w = {{bp[4 * c], bp[4 * c + 1], bp[4 * c + 2], bp[4 * c + 3]}};
+            if (sh == 0) {
+                const u32x4 v = *(GlobalV4*)(p + j);
+                f[0] = qadd(f[0], qmul_m31(w, v.x));
+                f[1] = qadd(f[1], qmul_m31(w, v.y));
+                f[2] = qadd(f[2], qmul_m31(w, v.z));
+                f[3] = qadd(f[3], qmul_m31(w, v.w));
+            } else {
+                // rows of j .. j + 3 are r, r + 1, r, r + 1 (sh >= 2)
+                const u32x2 v = *(GlobalV2*)(p + ((j >> sh) << 1));
+                const QM31 a = qmul_m31(w, v.x), b = qmul_m31(w, v.y);
+                f[0] = qadd(f[0], a);
+                f[1] = qadd(f[1], b);
+                f[2] = qadd(f[2], a);
+                f[3] = qadd(f[3], b);
+            }
+        }
+        const QM31 al = load_alpha(alpha);
+        const uint2 w = *(const uint2*)(tw + i);
+        const QM31 g0 = fold_pair(f[0], f[1], al, w.x);
+        const QM31 g1 = fold_pair(f[2], f[3], al, w.y);
+        *(uint2*)(o0 + i) = make_uint2(g0.v[0], g1.v[0]);
+        *(uint2*)(o1 + i) = make_uint2(g0.v[1], g1.v[1]);
+        *(uint2*)(o2 + i) = make_uint2(g0.v[2], g1.v[2]);
+        *(uint2*)(o3 + i) = make_uint2(g0.v[3], g1.v[3]);
+        uint32_t l[8], r[8];
+        b2s_leaf4(g0.v[0], g0.v[1], g0.v[2], g0.v[3], l);
+        b2s_leaf4(g1.v[0], g1.v[1], g1.v[2], g1.v[3], r);
+        b2s_node(l, r, h);
+    }
+};
+
 // ---------------------------------------------------------------- narrow levels
 // Reduces n (a power of two <= 64) nodes stored node-major ([node][8 words]) in
 // the LDS buffer `a` to one node with the quads of the calling wave, ping-ponging
@@ -271,10 +397,11 @@ __device__ __forceinline__ uint32_t* narrow_reduce(uint32_t* a, uint32_t* b, uin
 }
 
 // Fiat-Shamir step after a commitment, by one whole wave (quad 0 is the one whose
-// results are kept): optional reset, mix_digest(root) and optional draw_felt
-// (channel.rs:30-39, 67-86). root and msg are LDS (8 and 16 words).
+// results are kept): optional reset, mix_digest(root) and n_felts draw_felt
+// calls into alpha_dst[4k..4k+4) (channel.rs:30-39, 67-86). root and msg are
+// LDS (8 and 16 words).
 __device__ __forceinline__ void quad_transcript(DevChannel* ch, const uint32_t* root, uint32_t* msg, bool init,
-                                                uint32_t* alpha_dst) {
+                                                uint32_t* alpha_dst, uint32_t n_felts = 1) {
     const uint32_t lane = threadIdx.x & 63, l = lane & 3;
     if (init && lane == 0) ch->failed = 0;
     if (lane < 8) msg[lane] = init ? 0u : ch->state[lane];
@@ -292,7 +419,8 @@ __device__ __forceinline__ void quad_transcript(DevChannel* ch, const uint32_t* 
     VX_WAVE_SYNC();
     if (lane < 8) ch->state[lane] = msg[lane];
     uint32_t n_draws = 0;
-    if (alpha_dst) {
+    for (uint32_t k = 0; alpha_dst && k < n_felts; k++) {
+        bool got = false;
         for (int tries = 0; tries < 64; tries++) {
             if (lane < 8) msg[8 + lane] = lane == 0 ? n_draws : 0u;
             VX_WAVE_SYNC();
@@ -303,10 +431,14 @@ __device__ __forceinline__ void quad_transcript(DevChannel* ch, const uint32_t* 
             const uint64_t ok = __ballot(wl < 2u * P && wh < 2u * P);
             VX_WAVE_SYNC();
             if ((ok & 0xFull) == 0xFull) {
-                if (lane < 4) alpha_dst[l] = canon(wl);
+                if (lane < 4) alpha_dst[4 * k + l] = canon(wl);
+                got = true;
                 break;
             }
-            if (tries == 63 && lane == 0) ch->failed = 1;
+        }
+        if (!got) {
+            if (lane == 0) ch->failed = 1;
+            break;
         }
     }
     if (lane == 0) ch->n_draws = n_draws;
@@ -377,8 +509,9 @@ __global__ __launch_bounds__(64 * MERKLE_STAGE_WAVES) void merkle_stage_kernel(S
 struct TailOut {
     uint32_t* root_dst;      // 8 words (e.g. inside the proof buffer); may be null
     DevChannel* channel;     // transcript to advance; may be null
-    uint32_t* alpha_dst;     // challenge drawn after mixing the root; may be null
+    uint32_t* alpha_dst;     // challenges drawn after mixing the root; may be null
     uint32_t init_channel;   // reset the transcript before mixing
+    uint32_t n_felts = 1;    // challenges drawn (4 words each)
 };
 
 constexpr uint32_t TAIL_THREADS = 1024;
@@ -433,7 +566,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src, uint
     VX_WAVE_SYNC();
     uint32_t* root = narrow_reduce(na, nb, n, lvl, item_base >> (lvl - item_level), st);
     if (out.root_dst && tid < 8) out.root_dst[8ull * blockIdx.x + tid] = root[tid];
-    if (out.channel) quad_transcript(out.channel, root, msg, out.init_channel != 0, out.alpha_dst);
+    if (out.channel) quad_transcript(out.channel, root, msg, out.init_channel != 0, out.alpha_dst, out.n_felts);
 }
 
 }  // namespace vx

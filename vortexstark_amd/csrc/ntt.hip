@@ -12,6 +12,11 @@
 // up to 2^14 words (64 KB LDS); passes over higher strides gather 2^m rows of
 // 2^c contiguous words (c = 5: 128-byte segments) so every HBM access stays
 // coalesced. A 2^26-point evaluate is 3 passes instead of 26.
+//
+// The same pass kernel runs the stwo-basis transform (cuda/circle_ntt_stwo.cu):
+// layer b >= 1 reads the flat stwo buffer at n/2 - n/2^b, and the circle layer
+// derives its twiddle for pair t from the first line layer as
+// [y, -y, -x, x][t % 4] with (x, y) = (T[2(t/4)], T[2(t/4) + 1]).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -64,7 +69,8 @@ static HostPt host_pow(HostPt p, uint64_t e) {
 struct NttPassArgs {
     uint32_t* dst;
     const uint32_t* src;
-    uint32_t* const* cols;  // batched: column blockIdx.y, in place
+    uint32_t* const* cols;        // batched: column blockIdx.y is written here
+    const uint32_t* const* srcs;  // batched: and read from here (null: in place)
     const uint32_t* line;
     const uint32_t* circle;
     uint32_t offs[32];
@@ -78,10 +84,25 @@ struct NttPassArgs {
 // accesses below free of bank conflicts.
 __device__ __forceinline__ uint32_t lpad(uint32_t e) { return e + (e >> 3); }
 
+// Twiddle h of the layer with stride bit b.
+template <bool STWO>
+__device__ __forceinline__ uint32_t ntt_twiddle(const NttPassArgs& a, uint32_t b, uint32_t h) {
+    if (b) return a.line[a.offs[b - 1] + h];
+    if (!STWO) return a.circle[h];
+    const uint32_t* pr = a.line + 2 * (h >> 2);
+    const uint32_t x = pr[0], y = pr[1];
+    switch (h & 3) {
+        case 0: return y;
+        case 1: return mneg(y);
+        case 2: return mneg(x);
+        default: return x;
+    }
+}
+
 // One radix-2^R round over row bits [j, j + R) of an LDS tile of 2^M rows x
 // 2^C columns. Each group of 2^R values (fixed column and other row bits) is
 // loaded into registers, transformed through R layers, and written back.
-template <bool FWD, int M, int C, int R>
+template <bool FWD, int M, int C, int R, bool STWO>
 __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi, uint32_t j) {
     constexpr uint32_t GROUPS = 1u << (M + C - R);
     for (uint32_t g = threadIdx.x; g < GROUPS; g += blockDim.x) {
@@ -97,13 +118,12 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
             const int t = FWD ? (R - 1 - tt) : tt;  // layer inside the group
             const uint32_t rb = j + t;                 // row bit of the layer
             const uint32_t b = a.s + rb;               // global stride bit
-            const uint32_t* tw = b ? a.line + a.offs[b - 1] : a.circle;
 #pragma unroll
             for (int d = 0; d < (1 << R); d++) {
                 if (d & (1 << t)) continue;
                 const uint32_t r0 = row_base + ((uint32_t)d << j);
                 const uint32_t h = (uint32_t)((hi << (M - rb - 1)) + (r0 >> (rb + 1)));
-                const uint32_t w = tw[h];
+                const uint32_t w = ntt_twiddle<STWO>(a, b, h);
                 uint32_t& x0 = v[d];
                 uint32_t& x1 = v[d | (1 << t)];
                 if (FWD) {
@@ -124,11 +144,11 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
 
 // Applies the M layers with stride bits [s, s + M) of a size-2^k transform to
 // the tile of 2^M rows (stride 2^s) by 2^C contiguous columns owned by this block.
-template <bool FWD, int M, int C>
+template <bool FWD, int M, int C, bool STWO>
 __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
     __shared__ uint32_t tile[(1u << (M + C)) + (1u << (M + C)) / 8];
     uint32_t* dst = a.cols ? a.cols[blockIdx.y] : a.dst;
-    const uint32_t* src = a.cols ? a.cols[blockIdx.y] : a.src;
+    const uint32_t* src = a.cols ? (a.srcs ? a.srcs[blockIdx.y] : a.cols[blockIdx.y]) : a.src;
     const uint32_t s = a.s;
     constexpr uint32_t TILE = 1u << (M + C);
     const uint32_t mid_bits = s - C;
@@ -148,11 +168,11 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
             int r = top >= 3 ? 3 : top;
             top -= r;
             if (r == 3)
-                ntt_round<FWD, M, C, (M >= 3 ? 3 : 1)>(tile, a, hi, (uint32_t)top);
+                ntt_round<FWD, M, C, (M >= 3 ? 3 : 1), STWO>(tile, a, hi, (uint32_t)top);
             else if (r == 2)
-                ntt_round<FWD, M, C, (M >= 2 ? 2 : 1)>(tile, a, hi, (uint32_t)top);
+                ntt_round<FWD, M, C, (M >= 2 ? 2 : 1), STWO>(tile, a, hi, (uint32_t)top);
             else
-                ntt_round<FWD, M, C, 1>(tile, a, hi, (uint32_t)top);
+                ntt_round<FWD, M, C, 1, STWO>(tile, a, hi, (uint32_t)top);
             __syncthreads();
         }
     } else {
@@ -160,11 +180,11 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
         while (lo < M) {
             int r = M - lo >= 3 ? 3 : M - lo;
             if (r == 3)
-                ntt_round<FWD, M, C, (M >= 3 ? 3 : 1)>(tile, a, hi, (uint32_t)lo);
+                ntt_round<FWD, M, C, (M >= 3 ? 3 : 1), STWO>(tile, a, hi, (uint32_t)lo);
             else if (r == 2)
-                ntt_round<FWD, M, C, (M >= 2 ? 2 : 1)>(tile, a, hi, (uint32_t)lo);
+                ntt_round<FWD, M, C, (M >= 2 ? 2 : 1), STWO>(tile, a, hi, (uint32_t)lo);
             else
-                ntt_round<FWD, M, C, 1>(tile, a, hi, (uint32_t)lo);
+                ntt_round<FWD, M, C, 1, STWO>(tile, a, hi, (uint32_t)lo);
             lo += r;
             __syncthreads();
         }
@@ -200,45 +220,49 @@ static std::vector<Pass> plan_passes(uint32_t k) {
     return ps;
 }
 
-template <bool FWD, int M, int C>
+template <bool FWD, int M, int C, bool STWO>
 static void launch_tile(const NttPassArgs& a, uint32_t k, uint32_t n_cols, hipStream_t st) {
     dim3 grid((uint32_t)(1ull << (k - M - C)), n_cols);
     constexpr uint32_t groups = 1u << (M + C > 3 ? M + C - 3 : 0);
     uint32_t T = groups >= 256 ? 256 : (groups < 64 ? 64 : groups);
-    ntt_pass_kernel<FWD, M, C><<<grid, T, 0, st>>>(a);
+    ntt_pass_kernel<FWD, M, C, STWO><<<grid, T, 0, st>>>(a);
 }
 
-template <bool FWD, int C, int M>
+template <bool FWD, int C, int M, bool STWO>
 static void dispatch_m(uint32_t m, const NttPassArgs& a, uint32_t k, uint32_t n_cols, hipStream_t st) {
     if constexpr (M >= 1) {
         if (m == (uint32_t)M) {
-            launch_tile<FWD, M, C>(a, k, n_cols, st);
+            launch_tile<FWD, M, C, STWO>(a, k, n_cols, st);
             return;
         }
-        dispatch_m<FWD, C, M - 1>(m, a, k, n_cols, st);
+        dispatch_m<FWD, C, M - 1, STWO>(m, a, k, n_cols, st);
     } else {
         set_error(-2, "ntt: unsupported pass shape");
     }
 }
 
-static void launch_pass(bool fwd, const NttPassArgs& a, uint32_t m, uint32_t c, uint32_t k, uint32_t n_cols,
-                        hipStream_t st) {
+// Only the contiguous pass (c == 0) contains the circle layer, so only it has a
+// stwo variant; strided passes differ by their layer offsets alone.
+static void launch_pass(bool fwd, bool stwo, const NttPassArgs& a, uint32_t m, uint32_t c, uint32_t k,
+                        uint32_t n_cols, hipStream_t st) {
     if (c == 0) {
         if (fwd)
-            dispatch_m<true, 0, NTT_LOW>(m, a, k, n_cols, st);
+            stwo ? dispatch_m<true, 0, NTT_LOW, true>(m, a, k, n_cols, st)
+                 : dispatch_m<true, 0, NTT_LOW, false>(m, a, k, n_cols, st);
         else
-            dispatch_m<false, 0, NTT_LOW>(m, a, k, n_cols, st);
+            stwo ? dispatch_m<false, 0, NTT_LOW, true>(m, a, k, n_cols, st)
+                 : dispatch_m<false, 0, NTT_LOW, false>(m, a, k, n_cols, st);
     } else {
         if (fwd)
-            dispatch_m<true, NTT_COLS, NTT_HIGH>(m, a, k, n_cols, st);
+            dispatch_m<true, NTT_COLS, NTT_HIGH, false>(m, a, k, n_cols, st);
         else
-            dispatch_m<false, NTT_COLS, NTT_HIGH>(m, a, k, n_cols, st);
+            dispatch_m<false, NTT_COLS, NTT_HIGH, false>(m, a, k, n_cols, st);
     }
     check_launch("ntt_pass_kernel");
 }
 
 static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n, uint32_t* const* cols,
-                    uint32_t n_cols, const NttTables& t, hipStream_t st) {
+                    const uint32_t* const* srcs, uint32_t n_cols, const NttTables& t, hipStream_t st) {
     uint32_t k = t.log_n;
     auto ps = plan_passes(k);
     uint32_t ex = (30u * k) % 31u;
@@ -247,6 +271,7 @@ static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n
         size_t pi = fwd ? ps.size() - 1 - i : i;
         NttPassArgs a = {};
         a.cols = cols;
+        a.srcs = i == 0 ? srcs : nullptr;
         a.dst = dst;
         a.src = i == 0 ? src : dst;
         a.src_n = i == 0 ? src_n : (1ull << k);
@@ -256,7 +281,7 @@ static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n
         a.s = ps[pi].s;
         a.k = k;
         a.scale = (!fwd && i + 1 == ps.size() && inv_n != 1u) ? inv_n : 0u;
-        launch_pass(fwd, a, ps[pi].m, ps[pi].c, k, n_cols, st);
+        launch_pass(fwd, t.stwo, a, ps[pi].m, ps[pi].c, k, n_cols, st);
     }
     if (!fwd && ps.empty() && src != dst && src_n) {
         check(hipMemcpyAsync(dst, src, 4, hipMemcpyDeviceToDevice, st), "ntt copy");
@@ -268,16 +293,29 @@ void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTa
         if (src != dst) check(hipMemcpyAsync(dst, src, 4, hipMemcpyDeviceToDevice, s), "ntt copy");
         return;
     }
-    run_ntt(true, dst, src, src_n, nullptr, 1, t, s);
+    run_ntt(true, dst, src, src_n, nullptr, nullptr, 1, t, s);
 }
 void ntt_inverse(uint32_t* dst, const uint32_t* src, const NttTables& t, hipStream_t s) {
-    run_ntt(false, dst, src, 1ull << t.log_n, nullptr, 1, t, s);
+    run_ntt(false, dst, src, 1ull << t.log_n, nullptr, nullptr, 1, t, s);
 }
-void ntt_forward_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& t, hipStream_t s) {
-    if (n_cols) run_ntt(true, nullptr, nullptr, 1ull << t.log_n, cols, n_cols, t, s);
+void ntt_forward_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& t, hipStream_t s, uint64_t src_n) {
+    if (n_cols && t.log_n)
+        run_ntt(true, nullptr, nullptr, src_n ? src_n : (1ull << t.log_n), cols, nullptr, n_cols, t, s);
 }
-void ntt_inverse_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& t, hipStream_t s) {
-    if (n_cols) run_ntt(false, nullptr, nullptr, 1ull << t.log_n, cols, n_cols, t, s);
+void ntt_inverse_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& t, hipStream_t s,
+                       const uint32_t* const* srcs) {
+    if (n_cols && t.log_n) run_ntt(false, nullptr, nullptr, 1ull << t.log_n, cols, srcs, n_cols, t, s);
+}
+
+NttTables stwo_tables(const uint32_t* flat, uint32_t log_n) {
+    NttTables t;
+    t.line = flat;
+    t.circle = nullptr;
+    t.log_n = log_n;
+    t.stwo = true;
+    uint64_t half = (1ull << log_n) / 2;
+    for (uint32_t b = 1; b < log_n && b <= 32; b++) t.offs[b - 1] = (uint32_t)(half - (half >> (b - 1)));
+    return t;
 }
 
 // ---------------------------------------------------------------- twiddle kernels

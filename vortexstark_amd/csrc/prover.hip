@@ -21,6 +21,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "commit.hpp"
 #include "merkle_engine.hpp"
 
 namespace vx {
@@ -207,71 +208,6 @@ __global__ __launch_bounds__(64) void decommit_kernel(const TreeDesc* trees, con
     }
 }
 
-// ---------------------------------------------------------------- Merkle driver
-static uint32_t pick_logc(uint64_t n) {
-    const uint64_t MIN_WAVES = 1024;
-    for (int c = (int)MERKLE_LOGC_MAX; c > 0; c--)
-        if ((n >> (7 + c)) >= MIN_WAVES) return (uint32_t)c;
-    return 0;
-}
-
-// Trees of at most this many items go straight to the tail kernel; larger
-// ones are reduced by stage kernels until this many nodes remain.
-constexpr uint32_t PROVER_TAIL_ITEMS = 1024;
-
-struct Scratch {
-    uint32_t* a;
-    uint32_t* b;
-};
-
-// Commits n_leaves (power of two >= 2) items of a leaf source.
-template <class Src, class Mark>
-static void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, Scratch sc, hipStream_t s,
-                        Mark&& mark) {
-    if (n_leaves <= PROVER_TAIL_ITEMS) {
-        merkle_tail_kernel<Src><<<1, TAIL_THREADS, 0, s>>>(src, 0, (uint32_t)n_leaves, st, out);
-        check_launch("merkle_tail_kernel");
-        mark("tail");
-        return;
-    }
-    uint32_t logc = pick_logc(n_leaves);
-    uint64_t waves = n_leaves >> (7 + logc);
-    merkle_stage_kernel<Src><<<ceil_div(waves, MERKLE_STAGE_WAVES), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
-        src, 0, n_leaves, logc, sc.a, st);
-    check_launch("merkle_stage_kernel");
-    mark("stage_leaf");
-    uint64_t n = waves;  // one node per wave
-    uint32_t lvl = 7 + logc;
-    uint32_t* cur = sc.a;
-    uint32_t* nxt = sc.b;
-    while (n > PROVER_TAIL_ITEMS) {
-        logc = pick_logc(n);
-        waves = n >> (7 + logc);
-        merkle_stage_kernel<SrcNodes><<<ceil_div(waves, MERKLE_STAGE_WAVES), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
-            SrcNodes{cur}, lvl, n, logc, nxt, st);
-        check_launch("merkle_stage_kernel<nodes>");
-        mark("stage_node");
-        n = waves;
-        lvl += 7 + logc;
-        uint32_t* tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-    }
-    merkle_tail_kernel<SrcNodes><<<1, TAIL_THREADS, 0, s>>>(SrcNodes{cur}, lvl, (uint32_t)n, st, out);
-    check_launch("merkle_tail_kernel<nodes>");
-    mark("tail");
-}
-
-static uint64_t max_stage_nodes(uint64_t n_leaves) {
-    if (n_leaves <= PROVER_TAIL_ITEMS) return 0;
-    uint64_t mx = 0, n = n_leaves;
-    while (n > PROVER_TAIL_ITEMS) {
-        n >>= 7 + pick_logc(n);
-        if (n > mx) mx = n;
-    }
-    return mx;
-}
-
 }  // namespace vx
 
 // ---------------------------------------------------------------- prover object
@@ -300,23 +236,8 @@ struct vx_prover {
     TreeDesc* d_trees = nullptr;
     // layout
     uint64_t off_fri_roots = 0, off_last = 0, off_queries = 0;
-    // optional launch timeline (events recorded on the prover's stream)
-    bool timing = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::string> labels;
-    size_t n_marks = 0;
-    void mark(const char* label) {
-        if (!timing) return;
-        if (n_marks == ev_pool.size()) {
-            hipEvent_t e;
-            if (!check(hipEventCreate(&e), "hipEventCreate")) return;
-            ev_pool.push_back(e);
-            labels.emplace_back();
-        }
-        labels[n_marks] = label;
-        check(hipEventRecord(ev_pool[n_marks], stream), "hipEventRecord");
-        n_marks++;
-    }
+    Timeline tl;  // optional launch timeline on the prover's stream
+    void mark(const std::string& label) { tl.mark(label, stream); }
 
     uint32_t* col(uint32_t j, uint32_t c) const { return layer[j] + (uint64_t)c * (E >> (j + 1)); }
     uint32_t* alpha_q() const { return alphas; }
@@ -329,7 +250,6 @@ struct vx_prover {
     }
     ~vx_prover() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void* p : allocs) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -523,13 +443,10 @@ int32_t vx_prover_prove_device(vx_prover* p, const uint32_t* d_trace, uint32_t a
     }
     hipStream_t s = p->stream;
     const uint64_t N = p->N, E = p->E;
-    p->n_marks = 0;
+    p->tl.n = 0;
     p->mark("start");
     std::string phase;
-    auto mk = [&](const char* what) {
-        std::string l = phase + ":" + what;
-        p->mark(l.c_str());
-    };
+    auto mk = [&](const char* what) { p->mark(phase + ":" + what); };
     // LDE: interpolate on half_coset(log_n), evaluate on half_coset(log_n + 2)
     ntt_inverse(p->coeffs, d_trace, p->trace_tab, s);
     p->mark("lde:interpolate");
@@ -592,21 +509,12 @@ uint64_t vx_prover_proof_copy(vx_prover* p, void* host_out, uint64_t cap) {
 
 int32_t vx_prover_set_timing(vx_prover* p, int32_t enable) {
     if (!p) return -1;
-    p->timing = enable != 0;
-    p->n_marks = 0;
+    p->tl.reset(enable != 0);
     return 0;
 }
-uint32_t vx_prover_timeline_count(const vx_prover* p) { return p ? (uint32_t)p->n_marks : 0; }
-const char* vx_prover_timeline_label(const vx_prover* p, uint32_t i) {
-    return (p && i < p->n_marks) ? p->labels[i].c_str() : "";
-}
-float vx_prover_timeline_ms(vx_prover* p, uint32_t i) {
-    if (!p || i == 0 || i >= p->n_marks) return 0.f;
-    float ms = 0.f;
-    if (!check(hipEventSynchronize(p->ev_pool[i]), "hipEventSynchronize")) return -1.f;
-    if (!check(hipEventElapsedTime(&ms, p->ev_pool[i - 1], p->ev_pool[i]), "hipEventElapsedTime")) return -1.f;
-    return ms;
-}
+uint32_t vx_prover_timeline_count(const vx_prover* p) { return p ? (uint32_t)p->tl.n : 0; }
+const char* vx_prover_timeline_label(const vx_prover* p, uint32_t i) { return p ? p->tl.label(i) : ""; }
+float vx_prover_timeline_ms(vx_prover* p, uint32_t i) { return p ? p->tl.ms(i) : 0.f; }
 
 int32_t vx_fibonacci_trace(uint32_t a, uint32_t b, uint32_t log_n, uint32_t* d_out, void* stream) {
     uint64_t n = 1ull << log_n;
