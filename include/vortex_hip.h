@@ -21,6 +21,9 @@
  * through the stwo-basis LDE, one lifted Blake2s tree and fused FRI
  * fold+commit layers.
  *
+ * Part 5 is a proof server: continuous batching of proof jobs of mixed sizes
+ * under an HBM budget, with per-job latency.
+ *
  * Conventions shared by every entry point:
  *  - device buffers are allocated by the caller; kernels never allocate,
  *    except the vx_prover_* and vx_wide_* objects, which own their workspace;
@@ -180,6 +183,12 @@ typedef struct vx_prover vx_prover;
  * workspace are allocated here once. n_queries / log_last_layer default to the
  * reference protocol (80, 3) when 0 is passed. Returns NULL on failure. */
 vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer);
+/* Same, with room for `capacity` proofs of this size per launch sequence: one
+ * launch of each kernel serves every proof of a batch (each proof owns an
+ * equal-size arena of device memory; vx_prover_device_bytes reports the total). */
+vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer, uint32_t capacity);
+uint32_t vx_prover_capacity(const vx_prover* p);
+uint64_t vx_prover_device_bytes(const vx_prover* p);
 void vx_prover_destroy(vx_prover* p);
 /* Stream the prover's launches go to (default: its own stream). */
 void* vx_prover_stream(vx_prover* p);
@@ -192,10 +201,27 @@ uint64_t vx_prover_proof_size(const vx_prover* p);
  * query sampling and decommitment. Nothing waits for the host; the KRAK bytes
  * are left in device memory (vx_prover_proof_device). Returns 0 when enqueued. */
 int32_t vx_prover_prove_device(vx_prover* p, const uint32_t* d_trace, uint32_t a, uint32_t b);
+/* Batched form: proofs 0 .. count-1 of the traces h_traces[i] (a (host) array of
+ * device pointers) with h_ab[2i], h_ab[2i+1] = (a, b) of proof i; count <= capacity. */
+int32_t vx_prover_prove_batch_device(vx_prover* p, const uint32_t* const* h_traces, const uint32_t* h_ab,
+                                     uint32_t count);
+/* Batched Fibonacci proofs with the traces generated on the device from (a, b). */
+int32_t vx_prove_fibonacci_batch(vx_prover* p, const uint32_t* h_ab, uint32_t count);
+/* Replay the launch sequence of a batch as one HIP graph (captured on first use
+ * per batch count) instead of individual launches. */
+int32_t vx_prover_set_graph(vx_prover* p, int32_t enable);
 /* Device address of the KRAK v1 bytes of the last proof (valid after the stream completes). */
 const void* vx_prover_proof_device(const vx_prover* p);
+const void* vx_prover_proof_device_at(const vx_prover* p, uint32_t index);
 /* Waits for the stream and copies the proof to host memory. Returns bytes written or 0. */
 uint64_t vx_prover_proof_copy(vx_prover* p, void* host_out, uint64_t cap);
+uint64_t vx_prover_proof_copy_at(vx_prover* p, uint32_t index, void* host_out, uint64_t cap);
+/* Enqueues on `stream` (NULL: the prover's) copies of proofs 0 .. count-1 to
+ * host_proofs (proof_size bytes each, back to back) and, when host_failed is
+ * not NULL, each proof's transcript failure flag (non-zero: draw_felt gave up). */
+int32_t vx_prover_enqueue_copy(vx_prover* p, uint32_t count, void* host_proofs, uint32_t* host_failed, void* stream);
+/* Size in bytes of the KRAK v1 proof of a 2^log_n trace (no device needed). */
+uint64_t vx_krak_proof_size(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer);
 
 /* Convenience: generate the Fibonacci trace on device (air.rs:30-45 semantics) and
  * prove, returning the proof bytes on the host (prove(a, b, log_n) + serialize_proof). */
@@ -256,6 +282,38 @@ int32_t vx_wide_set_timing(vx_wide* w, int32_t enable);
 uint32_t vx_wide_timeline_count(const vx_wide* w);
 const char* vx_wide_timeline_label(const vx_wide* w, uint32_t i);
 float vx_wide_timeline_ms(vx_wide* w, uint32_t i);
+
+/* ===================== Part 5: proof server ================================ */
+
+typedef struct vx_server vx_server;
+
+/* A server for Fibonacci proof jobs (n_queries / log_last_layer as for the
+ * prover; 0 selects the reference protocol). budget_bytes caps the device
+ * memory its batched provers hold (0: 90% of the free memory; a larger budget
+ * is clipped to that). Not thread-safe: one thread drives a server. */
+vx_server* vx_server_create(uint64_t budget_bytes, uint32_t n_queries, uint32_t log_last_layer);
+void vx_server_destroy(vx_server* s);
+/* Queues a job of n_cols columns of 2^log_n rows; column c is the Fibonacci trace
+ * with (a, b) = (ab[2c], ab[2c+1]) (host). Returns the job id or -1. */
+int64_t vx_server_submit(vx_server* s, uint32_t log_n, uint32_t n_cols, const uint32_t* ab);
+/* One scheduling step: retire finished batches, launch batches on idle provers,
+ * add provers within the budget. Returns the number of actions taken. */
+int32_t vx_server_poll(vx_server* s);
+int32_t vx_server_wait_all(vx_server* s);
+uint64_t vx_server_jobs(const vx_server* s);
+uint64_t vx_server_completed(const vx_server* s);
+/* 1 done, 0 pending, negative: error (-3 transcript, -4 does not fit the budget). */
+int32_t vx_server_job_status(const vx_server* s, int64_t job);
+double vx_server_job_latency_ms(const vx_server* s, int64_t job);
+uint64_t vx_server_proof_copy(const vx_server* s, int64_t job, uint32_t col, void* out, uint64_t cap);
+void vx_server_release_job(vx_server* s, int64_t job);  /* frees a finished job's host copy */
+/* Replays arrivals in real time (job i at arrival_s[i] seconds, ascending; its
+ * n_cols[i] (a, b) pairs follow each other in ab) and returns when all are done;
+ * latency_ms (optional, n_jobs entries) gets arrival-to-completion times. */
+int32_t vx_server_run_trace(vx_server* s, uint32_t n_jobs, const double* arrival_s, const uint32_t* log_n,
+                            const uint32_t* n_cols, const uint32_t* ab, double* latency_ms);
+/* [0] batches, [1] proofs, [2] jobs done, [3] device bytes held, [4] peak, [5] budget */
+void vx_server_stats(const vx_server* s, uint64_t out[6]);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,67 @@
+"""Proof server: jobs of mixed sizes, batched per size class under an HBM
+budget, give every proof bit-identical to proving it alone (and to the
+oracle). Runs on the MI355X."""
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+P = (1 << 31) - 1
+
+
+def jobs_spec(seed, n, sizes, max_cols):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        log_n = int(rng.choice(sizes))
+        cols = int(rng.integers(1, max_cols + 1))
+        out.append((log_n, [(int(rng.integers(0, P)), int(rng.integers(0, P))) for _ in range(cols)]))
+    return out
+
+
+def test_server_mixed_jobs_match_oracle(vx):
+    spec = jobs_spec(1, 12, [4, 6, 9, 11], 3)
+    srv = vx.Server(0, 64, 3)
+    ids = [srv.submit(log_n, ab) for log_n, ab in spec]
+    srv.wait_all()
+    solo = {}
+    for jid, (log_n, ab) in zip(ids, spec):
+        assert srv.status(jid) == 1
+        assert srv.latency_ms(jid) > 0
+        for c, (a, b) in enumerate(ab):
+            exp = O.prove(a, b, log_n, 64, 3)
+            assert srv.proof(jid, c, log_n) == exp, (jid, c)
+            if log_n not in solo:
+                solo[log_n] = vx.Prover(log_n, 64, 3)
+            assert solo[log_n].prove(a, b) == exp
+    st = srv.stats()
+    assert st["proofs"] == sum(len(ab) for _, ab in spec)
+    assert st["batches"] < st["proofs"]  # proofs of a class share launches
+    for p in solo.values():
+        p.close()
+    srv.close()
+
+
+def test_server_trace_replay(vx):
+    spec = jobs_spec(2, 16, [5, 8, 10], 4)
+    arrivals = np.sort(np.random.default_rng(3).uniform(0, 0.05, len(spec)))
+    srv = vx.Server(0, 80, 3)
+    lat = srv.run_trace(arrivals, [s[0] for s in spec], [len(s[1]) for s in spec],
+                        [v for s in spec for pair in s[1] for v in pair])
+    assert lat.shape == (len(spec),) and np.all(lat > 0)
+    for jid, (log_n, ab) in enumerate(spec):
+        for c, (a, b) in enumerate(ab):
+            assert srv.proof(jid, c, log_n) == O.prove(a, b, log_n, 80, 3), (jid, c)
+        srv.release(jid)
+    srv.close()
+
+
+def test_server_budget_too_small_fails_cleanly(vx):
+    srv = vx.Server(1 << 20, 80, 3)  # 1 MiB cannot hold a 2^12 prover
+    jid = srv.submit(12, [(1, 2)])
+    with pytest.raises(vx.HipError):
+        srv.wait_all()
+    assert srv.status(jid) == -4
+    srv.close()

@@ -5,6 +5,7 @@ Host-side mirror of the reference's prover interface (src/prover.rs:192-231):
     prove(a, b, log_n)              -> KRAK v1 proof bytes  (prove + serialize_proof)
     Prover(log_n).prove(a, b)       -> same, with the ProverCache reused
     WideCommit(log_sizes)           -> batched stwo-basis LDE + lifted tree + FRI commitments
+    Server(budget)                  -> continuous batching of proof jobs under an HBM budget
     DeviceBuffer                    -> device memory owned from Python
 
 Everything runs in libvortex_hip.so (HIP kernels for gfx950 behind a C ABI).
@@ -102,15 +103,18 @@ class Prover:
     log_last_layer default to the reference protocol (80 queries, last layer 2^3).
     """
 
-    def __init__(self, log_n: int, n_queries: int = N_QUERIES, log_last_layer: int = LOG_LAST_LAYER):
+    def __init__(self, log_n: int, n_queries: int = N_QUERIES, log_last_layer: int = LOG_LAST_LAYER,
+                 capacity: int = 1):
         self.log_n = log_n
         self.n_queries = n_queries
         self.log_last_layer = log_last_layer
-        self.handle = lib.vx_prover_create(log_n, n_queries, log_last_layer)
+        self.capacity = capacity
+        self.handle = lib.vx_prover_create_batch(log_n, n_queries, log_last_layer, capacity)
         if not self.handle:
             check("vx_prover_create")
             raise HipError("vx_prover_create failed")
         self.proof_size = int(lib.vx_prover_proof_size(self.handle))
+        self.device_bytes = int(lib.vx_prover_device_bytes(self.handle))
 
     @property
     def stream(self) -> int:
@@ -130,9 +134,30 @@ class Prover:
         if lib.vx_prover_prove_device(self.handle, C.c_void_p(d_trace_addr), a, b):
             check("vx_prover_prove_device")
 
-    def proof_bytes(self) -> bytes:
+    def prove_batch(self, ab) -> list:
+        """Fibonacci proofs of every (a, b) in one batch (traces generated on the device)."""
+        self.prove_batch_async(ab)
+        return [self.proof_bytes(i) for i in range(len(ab))]
+
+    def prove_batch_async(self, ab) -> None:
+        flat = (C.c_uint32 * (2 * len(ab)))(*[int(v) % P for pair in ab for v in pair])
+        if lib.vx_prove_fibonacci_batch(self.handle, flat, len(ab)):
+            check("vx_prove_fibonacci_batch")
+
+    def prove_batch_device(self, trace_addrs, ab) -> None:
+        """Enqueue proofs of traces already in device memory, one batch."""
+        ptrs = (C.c_void_p * len(trace_addrs))(*trace_addrs)
+        flat = (C.c_uint32 * (2 * len(ab)))(*[int(v) % P for pair in ab for v in pair])
+        if lib.vx_prover_prove_batch_device(self.handle, ptrs, flat, len(trace_addrs)):
+            check("vx_prover_prove_batch_device")
+
+    def set_graph(self, on: bool) -> None:
+        if lib.vx_prover_set_graph(self.handle, 1 if on else 0):
+            check("vx_prover_set_graph")
+
+    def proof_bytes(self, index: int = 0) -> bytes:
         buf = (C.c_uint8 * self.proof_size)()
-        n = lib.vx_prover_proof_copy(self.handle, buf, self.proof_size)
+        n = lib.vx_prover_proof_copy_at(self.handle, index, buf, self.proof_size)
         if n != self.proof_size:
             check("vx_prover_proof_copy")
             raise HipError("vx_prover_proof_copy failed")
@@ -263,3 +288,79 @@ def stwo_twiddles_device(log_n: int):
         check("vx_stwo_twiddles")
     synchronize()
     return fwd, inv
+
+
+class Server:
+    """Proof server (Part 5 of the C ABI): jobs of n Fibonacci columns of 2^log_n
+    rows, batched per size class on the device under an HBM budget."""
+
+    def __init__(self, budget_bytes: int = 0, n_queries: int = N_QUERIES, log_last_layer: int = LOG_LAST_LAYER):
+        self.handle = lib.vx_server_create(budget_bytes, n_queries, log_last_layer)
+        if not self.handle:
+            check("vx_server_create")
+            raise HipError("vx_server_create failed")
+        self.n_queries = n_queries
+        self.log_last_layer = log_last_layer
+
+    def submit(self, log_n: int, ab) -> int:
+        flat = (C.c_uint32 * (2 * len(ab)))(*[int(v) % P for pair in ab for v in pair])
+        job = lib.vx_server_submit(self.handle, log_n, len(ab), flat)
+        if job < 0:
+            check("vx_server_submit")
+            raise HipError("vx_server_submit failed")
+        return job
+
+    def poll(self) -> int:
+        return lib.vx_server_poll(self.handle)
+
+    def wait_all(self) -> None:
+        if lib.vx_server_wait_all(self.handle):
+            check("vx_server_wait_all")
+            raise HipError("vx_server_wait_all failed")
+
+    def status(self, job: int) -> int:
+        return lib.vx_server_job_status(self.handle, job)
+
+    def latency_ms(self, job: int) -> float:
+        return lib.vx_server_job_latency_ms(self.handle, job)
+
+    def proof(self, job: int, col: int, log_n: int) -> bytes:
+        size = int(lib.vx_krak_proof_size(log_n, self.n_queries, self.log_last_layer))
+        buf = (C.c_uint8 * size)()
+        if lib.vx_server_proof_copy(self.handle, job, col, buf, size) != size:
+            raise HipError(f"job {job}: no proof (status {self.status(job)})")
+        return bytes(buf)
+
+    def release(self, job: int) -> None:
+        lib.vx_server_release_job(self.handle, job)
+
+    def run_trace(self, arrival_s, log_n, n_cols, ab) -> np.ndarray:
+        """Replays arrivals in real time; returns per-job latency in ms."""
+        n = len(arrival_s)
+        arr_t = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        ln = np.ascontiguousarray(log_n, dtype=np.uint32)
+        nc = np.ascontiguousarray(n_cols, dtype=np.uint32)
+        flat = np.ascontiguousarray(np.asarray(ab, dtype=np.uint64) % P, dtype=np.uint32)
+        lat = np.zeros(n, dtype=np.float64)
+        if lib.vx_server_run_trace(self.handle, n, arr_t.ctypes.data, ln.ctypes.data, nc.ctypes.data,
+                                   flat.ctypes.data, lat.ctypes.data):
+            check("vx_server_run_trace")
+            raise HipError("vx_server_run_trace failed")
+        return lat
+
+    def stats(self) -> dict:
+        out = (C.c_uint64 * 6)()
+        lib.vx_server_stats(self.handle, out)
+        keys = ["batches", "proofs", "jobs_done", "device_bytes", "peak_device_bytes", "budget_bytes"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def close(self) -> None:
+        if self.handle:
+            lib.vx_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

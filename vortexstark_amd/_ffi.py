@@ -80,6 +80,16 @@ SIGNATURES = {
     "vx_get_stream": (vp, []),
     # Part 3: fused prover
     "vx_prover_create": (vp, [u32, u32, u32]),
+    "vx_prover_create_batch": (vp, [u32, u32, u32, u32]),
+    "vx_prover_capacity": (u32, [vp]),
+    "vx_prover_device_bytes": (u64, [vp]),
+    "vx_prover_prove_batch_device": (i32, [vp, vp, vp, u32]),
+    "vx_prove_fibonacci_batch": (i32, [vp, vp, u32]),
+    "vx_prover_set_graph": (i32, [vp, i32]),
+    "vx_prover_proof_device_at": (vp, [vp, u32]),
+    "vx_prover_proof_copy_at": (u64, [vp, u32, vp, u64]),
+    "vx_prover_enqueue_copy": (i32, [vp, u32, vp, vp, vp]),
+    "vx_krak_proof_size": (u64, [u32, u32, u32]),
     "vx_prover_destroy": (None, [vp]),
     "vx_prover_stream": (vp, [vp]),
     "vx_prover_proof_size": (u64, [vp]),
@@ -109,6 +119,20 @@ SIGNATURES = {
     "vx_wide_timeline_count": (u32, [vp]),
     "vx_wide_timeline_label": (C.c_char_p, [vp, u32]),
     "vx_wide_timeline_ms": (C.c_float, [vp, u32]),
+    # Part 5: proof server
+    "vx_server_create": (vp, [u64, u32, u32]),
+    "vx_server_destroy": (None, [vp]),
+    "vx_server_submit": (C.c_int64, [vp, u32, u32, vp]),
+    "vx_server_poll": (i32, [vp]),
+    "vx_server_wait_all": (i32, [vp]),
+    "vx_server_jobs": (u64, [vp]),
+    "vx_server_completed": (u64, [vp]),
+    "vx_server_job_status": (i32, [vp, C.c_int64]),
+    "vx_server_job_latency_ms": (C.c_double, [vp, C.c_int64]),
+    "vx_server_proof_copy": (u64, [vp, C.c_int64, u32, vp, u64]),
+    "vx_server_release_job": (None, [vp, C.c_int64]),
+    "vx_server_run_trace": (i32, [vp, u32, vp, vp, vp, vp, vp]),
+    "vx_server_stats": (None, [vp, vp]),
 }
 
 

@@ -23,20 +23,21 @@ struct Scratch {
     uint32_t* b;
 };
 
-// Commits n_leaves (power of two >= 2) items of a leaf source.
+// Commits n_leaves (power of two >= 2) items of a leaf source, for `batch`
+// proofs at once whose buffers are bs words apart (bs = 0 for one proof).
 template <class Src, class Mark>
 void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, Scratch sc, hipStream_t s,
-                        Mark&& mark) {
+                 Mark&& mark, uint32_t batch = 1, uint64_t bs = 0) {
     if (n_leaves <= PROVER_TAIL_ITEMS) {
-        merkle_tail_kernel<Src><<<1, TAIL_THREADS, 0, s>>>(src, 0, (uint32_t)n_leaves, st, out);
+        merkle_tail_kernel<Src><<<dim3(1, batch), TAIL_THREADS, 0, s>>>(src, 0, (uint32_t)n_leaves, st, out, bs);
         check_launch("merkle_tail_kernel");
         mark("tail");
         return;
     }
     uint32_t logc = pick_logc(n_leaves);
     uint64_t waves = n_leaves >> (7 + logc);
-    merkle_stage_kernel<Src><<<ceil_div(waves, MERKLE_STAGE_WAVES), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
-        src, 0, n_leaves, logc, sc.a, st);
+    merkle_stage_kernel<Src><<<dim3(ceil_div(waves, MERKLE_STAGE_WAVES), batch), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
+        src, 0, n_leaves, logc, sc.a, st, bs);
     check_launch("merkle_stage_kernel");
     mark("stage_leaf");
     uint64_t n = waves;  // one node per wave
@@ -46,8 +47,8 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
     while (n > PROVER_TAIL_ITEMS) {
         logc = pick_logc(n);
         waves = n >> (7 + logc);
-        merkle_stage_kernel<SrcNodes><<<ceil_div(waves, MERKLE_STAGE_WAVES), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
-            SrcNodes{cur}, lvl, n, logc, nxt, st);
+        merkle_stage_kernel<SrcNodes><<<dim3(ceil_div(waves, MERKLE_STAGE_WAVES), batch), 64 * MERKLE_STAGE_WAVES,
+                                        0, s>>>(SrcNodes{cur}, lvl, n, logc, nxt, st, bs);
         check_launch("merkle_stage_kernel<nodes>");
         mark("stage_node");
         n = waves;
@@ -56,7 +57,7 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
         cur = nxt;
         nxt = tmp;
     }
-    merkle_tail_kernel<SrcNodes><<<1, TAIL_THREADS, 0, s>>>(SrcNodes{cur}, lvl, (uint32_t)n, st, out);
+    merkle_tail_kernel<SrcNodes><<<dim3(1, batch), TAIL_THREADS, 0, s>>>(SrcNodes{cur}, lvl, (uint32_t)n, st, out, bs);
     check_launch("merkle_tail_kernel<nodes>");
     mark("tail");
 }

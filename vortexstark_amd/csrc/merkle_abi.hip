@@ -11,6 +11,7 @@ namespace vx {
 struct SrcGeneric {
     const uint32_t* const* cols;
     uint32_t n_cols;
+    __device__ __forceinline__ SrcGeneric at(uint64_t) const { return *this; }
     __device__ __forceinline__ void leaf(uint64_t i, uint32_t h[8]) const {
         uint32_t eff = n_cols > 16 ? 16 : n_cols;
         uint32_t m[16];
@@ -56,7 +57,7 @@ static void tail_blocks(Src src, uint32_t n_items, uint32_t blocks, uint32_t* ro
                         const char* what) {
     LevelStore none = {nullptr, 0, 0};
     TailOut out = {roots, nullptr, nullptr, 0};
-    merkle_tail_kernel<Src><<<blocks, TAIL_THREADS, 0, st>>>(src, 0, n_items, none, out);
+    merkle_tail_kernel<Src><<<blocks, TAIL_THREADS, 0, st>>>(src, 0, n_items, none, out, 0);
     check_launch(what);
 }
 

@@ -51,6 +51,9 @@ struct LevelStore {
         uint64_t off = (1ull << (depth - from + 1)) - (1ull << (depth - level + 1));
         return base + 8ull * (off + idx);
     }
+    __device__ __forceinline__ LevelStore shifted(uint64_t off) const {
+        return {base ? base + off : nullptr, depth, from};
+    }
 };
 __host__ __device__ inline uint64_t level_store_words(uint32_t depth, uint32_t from) {
     return from > depth ? 0 : 8ull * ((1ull << (depth - from + 1)) - 1);
@@ -77,9 +80,13 @@ __device__ __forceinline__ void load8(const uint32_t* src, uint32_t h[8]) {
 // ---------------------------------------------------------------- sources
 // pair(i, h): i even; h = parent of items i and i+1.
 // leaf(i, h) / values(i, v): used by decommitment (leaf sources only).
+// at(off): the same source for another proof of a batch, whose buffers sit
+// `off` words further (batched provers give every proof an equal-size arena;
+// shared tables such as twiddles do not move).
 
 struct SrcNodes {
     const uint32_t* nodes;
+    __device__ __forceinline__ SrcNodes at(uint64_t off) const { return {nodes + off}; }
     __device__ __forceinline__ void pair(uint64_t i, uint32_t h[8]) const {
         uint32_t l[8], r[8];
         load8(nodes + 8 * i, l);
@@ -92,6 +99,7 @@ struct SrcNodes {
 struct SrcCol1 {
     const uint32_t* col;
     static constexpr uint32_t WIDTH = 1;
+    __device__ __forceinline__ SrcCol1 at(uint64_t off) const { return {col + off}; }
     __device__ __forceinline__ void values(uint64_t i, uint32_t v[4]) const { v[0] = col[i]; }
     __device__ __forceinline__ void leaf(uint64_t i, uint32_t h[8]) const { b2s_leaf1(col[i], h); }
     __device__ __forceinline__ void pair(uint64_t i, uint32_t h[8]) const {
@@ -110,6 +118,7 @@ struct SrcSoa4 {
     const uint32_t* c2;
     const uint32_t* c3;
     static constexpr uint32_t WIDTH = 4;
+    __device__ __forceinline__ SrcSoa4 at(uint64_t off) const { return {c0 + off, c1 + off, c2 + off, c3 + off}; }
     __device__ __forceinline__ void values(uint64_t i, uint32_t v[4]) const {
         v[0] = c0[i];
         v[1] = c1[i];
@@ -141,6 +150,7 @@ struct SrcQuotient {
     uint64_t mask;
     const uint32_t* alpha;  // device, 4 words
     static constexpr uint32_t WIDTH = 4;
+    __device__ __forceinline__ SrcQuotient at(uint64_t off) const { return {t + off, mask, alpha + off}; }
     __device__ __forceinline__ QM31 q(uint64_t i, QM31 al) const {
         return quotient_at(t[i], t[(i + 1) & mask], t[(i + 2) & mask], al);
     }
@@ -178,6 +188,9 @@ struct SrcCircleFold {
     uint32_t* o1;
     uint32_t* o2;
     uint32_t* o3;
+    __device__ __forceinline__ SrcCircleFold at(uint64_t off) const {
+        return {t + off, mask, alpha_q + off, alpha_f + off, tw, o0 + off, o1 + off, o2 + off, o3 + off};
+    }
     __device__ __forceinline__ void pair(uint64_t i, uint32_t h[8]) const {
         QM31 aq = load_alpha(alpha_q), af = load_alpha(alpha_f);
         uint64_t j = 2 * i;  // quotient index of the first of four inputs
@@ -214,6 +227,9 @@ struct SrcLineFold {
     uint32_t* o1;
     uint32_t* o2;
     uint32_t* o3;
+    __device__ __forceinline__ SrcLineFold at(uint64_t off) const {
+        return {i0 + off, i1 + off, i2 + off, i3 + off, alpha + off, tw, o0 + off, o1 + off, o2 + off, o3 + off};
+    }
     __device__ __forceinline__ void pair(uint64_t i, uint32_t h[8]) const {
         QM31 al = load_alpha(alpha);
         uint64_t j = 2 * i;
@@ -257,6 +273,7 @@ __device__ __forceinline__ ConstLiftCol* const_table(const LiftCol* t) { return 
 struct SrcLifted {
     const LiftCol* cols;  // device, hashing order
     uint32_t n_cols;
+    __device__ __forceinline__ SrcLifted at(uint64_t) const { return *this; }
     // leaves i (even) and i + 1 read rows r and r + 1 of every column
     __device__ __forceinline__ void load_block(ConstLiftCol* tab, uint32_t c0, uint64_t i, uint2 v[16]) const {
         const uint32_t nw = n_cols - c0 < 16 ? n_cols - c0 : 16;
@@ -315,6 +332,7 @@ struct SrcWideFold {
     uint32_t* o1;
     uint32_t* o2;
     uint32_t* o3;
+    __device__ __forceinline__ SrcWideFold at(uint64_t) const { return *this; }
     __device__ __forceinline__ void pair(uint64_t i, uint32_t h[8]) const {
         const uint64_t j = 2 * i;  // f index of the first of four inputs (a multiple of 4)
         QM31 f[4];
@@ -450,11 +468,18 @@ __device__ __forceinline__ void quad_transcript(DevChannel* ch, const uint32_t* 
 constexpr uint32_t STAGE_SLOTS = MERKLE_LOGC_MAX + 1;
 constexpr uint32_t STAGE_WAVE_LDS = STAGE_SLOTS * 8 * 64;
 
+// blockIdx.y selects the proof of a batch: every per-proof buffer (items,
+// out_nodes, level store) lies blockIdx.y * bs words past proof 0's.
 template <class Src>
-__global__ __launch_bounds__(64 * MERKLE_STAGE_WAVES) void merkle_stage_kernel(Src src, uint32_t item_level,
+__global__ __launch_bounds__(64 * MERKLE_STAGE_WAVES) void merkle_stage_kernel(Src src0, uint32_t item_level,
                                                                              uint64_t n_items, uint32_t logc,
-                                                                             uint32_t* out_nodes, LevelStore st) {
+                                                                             uint32_t* out_nodes, LevelStore st0,
+                                                                             uint64_t bs) {
     __shared__ uint32_t lds[MERKLE_STAGE_WAVES * STAGE_WAVE_LDS];
+    const uint64_t boff = (uint64_t)blockIdx.y * bs;
+    const Src src = src0.at(boff);
+    const LevelStore st = st0.shifted(boff);
+    out_nodes += boff;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t* wl = lds + w * STAGE_WAVE_LDS;
     uint32_t* xchg = wl + MERKLE_LOGC_MAX * 8 * 64;
@@ -519,10 +544,20 @@ constexpr uint32_t TAIL_THREADS = 1024;
 // Block b reduces items [b * n_items, (b + 1) * n_items) to one root (the
 // tiled form used by the reference-ABI entry points runs many blocks; the
 // prover runs one). n_items is a power of two in [2, MERKLE_TAIL_MAX].
+// blockIdx.y selects the proof of a batch as in merkle_stage_kernel; the root,
+// transcript and challenge destinations move with it.
 template <class Src>
-__global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src, uint32_t item_level, uint32_t n_items,
-                                                                  LevelStore st, TailOut out) {
+__global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src0, uint32_t item_level, uint32_t n_items,
+                                                                  LevelStore st0, TailOut out, uint64_t bs) {
     constexpr uint32_t W = MERKLE_TAIL_MAX / 2;
+    const uint64_t boff = (uint64_t)blockIdx.y * bs;
+    const Src src = src0.at(boff);
+    const LevelStore st = st0.shifted(boff);
+    if (boff) {
+        if (out.root_dst) out.root_dst += boff;
+        if (out.channel) out.channel = (DevChannel*)((uint32_t*)out.channel + boff);
+        if (out.alpha_dst) out.alpha_dst += boff;
+    }
     __shared__ uint32_t nodes[8 * W];  // wide levels, word-major: word q of node j at q * W + j
     __shared__ uint32_t na[8 * 64], nb[8 * 64];
     __shared__ uint32_t msg[16];

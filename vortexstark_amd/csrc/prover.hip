@@ -68,18 +68,23 @@ __global__ void fib_trace_kernel(uint32_t a, uint32_t b, uint64_t n, uint32_t ch
 // ---------------------------------------------------------------- queries
 // Last layer into the proof, mix_felts(last layer), then n_q draw_number(E)
 // (prover.rs:1922-1927). After mix_felts the draws only differ in n_draws, so
-// every query is squeezed by its own lane.
+// every query is squeezed by its own lane. One block per proof of a batch
+// (buffers bs words apart); ab holds (a, b) per proof.
 constexpr uint32_t MAX_LAST_WORDS = 4096;
-__global__ __launch_bounds__(256) void queries_kernel(DevChannel* ch, const uint32_t* l0, const uint32_t* l1,
+__global__ __launch_bounds__(256) void queries_kernel(DevChannel* ch0, const uint32_t* l0, const uint32_t* l1,
                                                      const uint32_t* l2, const uint32_t* l3, uint32_t n_last,
-                                                     uint32_t* proof, uint64_t last_off, uint64_t q_off,
-                                                     uint32_t n_q, uint64_t E, uint32_t* queries, uint32_t a,
-                                                     uint32_t b) {
+                                                     uint32_t* proof0, uint64_t last_off, uint64_t q_off,
+                                                     uint32_t n_q, uint64_t E, uint32_t* queries0,
+                                                     const uint32_t* ab, uint64_t bs) {
     __shared__ uint32_t words[MAX_LAST_WORDS];
     __shared__ uint32_t st[8];
     const uint32_t tid = threadIdx.x;
+    const uint64_t off = (uint64_t)blockIdx.x * bs;
+    DevChannel* ch = (DevChannel*)((uint32_t*)ch0 + off);
+    uint32_t* proof = proof0 + off;
+    uint32_t* queries = queries0 + off;
     for (uint32_t i = tid; i < n_last; i += blockDim.x) {
-        uint32_t v[4] = {l0[i], l1[i], l2[i], l3[i]};
+        uint32_t v[4] = {l0[off + i], l1[off + i], l2[off + i], l3[off + i]};
         for (int c = 0; c < 4; c++) {
             words[4 * i + c] = v[c];
             proof[last_off + 4 * i + c] = v[c];
@@ -90,8 +95,8 @@ __global__ __launch_bounds__(256) void queries_kernel(DevChannel* ch, const uint
         ch_mix_felts(ch, words, 4 * n_last);
         for (int c = 0; c < 8; c++) st[c] = ch->state[c];
         ch->n_draws = n_q;
-        proof[3] = a;
-        proof[4] = b;
+        proof[3] = ab[2 * blockIdx.x];
+        proof[4] = ab[2 * blockIdx.x + 1];
     }
     __syncthreads();
     uint32_t s[8];
@@ -109,7 +114,8 @@ __global__ __launch_bounds__(256) void queries_kernel(DevChannel* ch, const uint
 // ---------------------------------------------------------------- decommitment
 // prover.rs:1184-1247 and 2171-2234: value at the query index and at its
 // sibling (index ^ 1), with both authentication paths (siblings from the leaf
-// level up, root excluded). One wave per (tree, query).
+// level up, root excluded). One wave per (query, tree, proof); tree
+// descriptors address proof 0 and move by blockIdx.z * bs words.
 struct TreeDesc {
     uint32_t kind;   // 0: one M31 column, 1: Fibonacci quotient of the LDE, 2: SoA4
     uint32_t depth;  // log2(leaves)
@@ -122,31 +128,35 @@ struct TreeDesc {
     uint64_t val_off, sib_off, path_off, sib_path_off;
 };
 
-__device__ void tree_values(const TreeDesc& t, uint64_t i, uint32_t v[4]) {
+__device__ void tree_values(const TreeDesc& t, uint64_t off, uint64_t i, uint32_t v[4]) {
     if (t.kind == 0) {
-        SrcCol1{t.c[0]}.values(i, v);
+        SrcCol1{t.c[0]}.at(off).values(i, v);
     } else if (t.kind == 1) {
-        SrcQuotient{t.c[0], t.mask, t.alpha}.values(i, v);
+        SrcQuotient{t.c[0], t.mask, t.alpha}.at(off).values(i, v);
     } else {
-        SrcSoa4{t.c[0], t.c[1], t.c[2], t.c[3]}.values(i, v);
+        SrcSoa4{t.c[0], t.c[1], t.c[2], t.c[3]}.at(off).values(i, v);
     }
 }
-__device__ void tree_leaf(const TreeDesc& t, uint64_t i, uint32_t h[8]) {
+__device__ void tree_leaf(const TreeDesc& t, uint64_t off, uint64_t i, uint32_t h[8]) {
     if (t.kind == 0) {
-        SrcCol1{t.c[0]}.leaf(i, h);
+        SrcCol1{t.c[0]}.at(off).leaf(i, h);
     } else if (t.kind == 1) {
-        SrcQuotient{t.c[0], t.mask, t.alpha}.leaf(i, h);
+        SrcQuotient{t.c[0], t.mask, t.alpha}.at(off).leaf(i, h);
     } else {
-        SrcSoa4{t.c[0], t.c[1], t.c[2], t.c[3]}.leaf(i, h);
+        SrcSoa4{t.c[0], t.c[1], t.c[2], t.c[3]}.at(off).leaf(i, h);
     }
 }
 
 constexpr uint32_t DECOMMIT_SUB = 5;  // recomputed subtree: 32 leaves
-__global__ __launch_bounds__(64) void decommit_kernel(const TreeDesc* trees, const uint32_t* queries, uint32_t* proof) {
+__global__ __launch_bounds__(64) void decommit_kernel(const TreeDesc* trees, const uint32_t* queries0,
+                                                      uint32_t* proof0, uint64_t bs) {
     __shared__ uint32_t lv[8 * 32];    // current level of the subtree, word-major
     __shared__ uint32_t path[8 * 32];  // sibling of the query's ancestor per level
     __shared__ uint32_t leaf_self[8];  // leaf hash of the query index itself
     const TreeDesc t = trees[blockIdx.y];
+    const uint64_t off = (uint64_t)blockIdx.z * bs;
+    const uint32_t* queries = queries0 + off;
+    uint32_t* proof = proof0 + off;
     const uint32_t q = blockIdx.x, lane = threadIdx.x;
     const uint64_t li = (uint64_t)queries[q] >> t.shift;
     const uint32_t D = t.depth;
@@ -156,8 +166,8 @@ __global__ __launch_bounds__(64) void decommit_kernel(const TreeDesc* trees, con
 
     if (lane == 0) {
         uint32_t v[4], s[4];
-        tree_values(t, li, v);
-        tree_values(t, li ^ 1, s);
+        tree_values(t, off, li, v);
+        tree_values(t, off, li ^ 1, s);
         for (uint32_t k = 0; k < width; k++) {
             proof[t.val_off + (uint64_t)q * width + k] = v[k];
             proof[t.sib_off + (uint64_t)q * width + k] = s[k];
@@ -166,7 +176,7 @@ __global__ __launch_bounds__(64) void decommit_kernel(const TreeDesc* trees, con
     uint32_t cnt = 1u << lam;
     if (lane < cnt) {
         uint32_t h[8];
-        tree_leaf(t, sub_base + lane, h);
+        tree_leaf(t, off, sub_base + lane, h);
         for (int w = 0; w < 8; w++) lv[w * 32 + lane] = h[w];
     }
     __syncthreads();
@@ -190,7 +200,7 @@ __global__ __launch_bounds__(64) void decommit_kernel(const TreeDesc* trees, con
         cnt = half;
         pos >>= 1;
     }
-    LevelStore st = {t.store, D, lam};
+    const LevelStore st = LevelStore{t.store, D, lam}.shifted(off);
     for (uint32_t L = lam; L < D; L++) {
         if (lane < 8) path[L * 8 + lane] = st.at(L, ((li >> L) ^ 1))[lane];
     }
@@ -208,49 +218,89 @@ __global__ __launch_bounds__(64) void decommit_kernel(const TreeDesc* trees, con
     }
 }
 
+// Fibonacci traces of a batch: proof blockIdx.y gets (a, b) = ab[2y], ab[2y+1]
+// and writes its trace bs words past proof 0's.
+__global__ void fib_trace_batch_kernel(const uint32_t* ab, uint64_t n, uint32_t chunk, uint32_t* out0, uint64_t bs) {
+    const uint32_t y = blockIdx.y;
+    uint64_t start = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * chunk;
+    if (start >= n) return;
+    uint32_t* out = out0 + (uint64_t)y * bs;
+    uint32_t f0, f1;
+    fib_at(ab[2 * y], ab[2 * y + 1], start, f0, f1);
+    uint64_t end = start + chunk < n ? start + chunk : n;
+    for (uint64_t i = start; i < end; i++) {
+        out[i] = f0;
+        uint32_t f2 = madd(f0, f1);
+        f0 = f1;
+        f1 = f2;
+    }
+}
+
 }  // namespace vx
 
 // ---------------------------------------------------------------- prover object
 using namespace vx;
 
+// A prover for traces of 2^log_n rows with room for `capacity` proofs per
+// launch sequence. Every proof owns one arena of R words (trace slot, LDE, FRI
+// layers, level stores, Merkle scratch, transcript, challenges, queries, KRAK
+// image); proof p's arena starts p * R words after proof 0's, so a single
+// stride moves every per-proof pointer and one launch serves the whole batch.
 struct vx_prover {
-    uint32_t log_n = 0, log_e = 0, n_q = 0, log_last = 0, J = 0;
+    uint32_t log_n = 0, log_e = 0, n_q = 0, log_last = 0, J = 0, capacity = 1;
     uint64_t N = 0, E = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
-    // twiddles
     NttTables trace_tab, eval_tab;
     std::vector<uint32_t*> fri_tw;
-    // work
-    uint32_t* trace_buf = nullptr;
-    uint32_t* coeffs = nullptr;
-    uint32_t* eval = nullptr;
-    std::vector<uint32_t*> layer;  // J layers, 4 columns each contiguous
-    std::vector<LevelStore> stores;
-    Scratch scratch = {nullptr, nullptr};
-    DevChannel* ch = nullptr;
-    uint32_t* alphas = nullptr;
-    uint32_t* queries = nullptr;
-    uint32_t* proof = nullptr;
+    // arena layout (word offsets inside one proof's arena)
+    uint64_t R = 0;
+    uint64_t o_trace = 0, o_eval = 0, o_scr_a = 0, o_scr_b = 0, o_ch = 0, o_alpha = 0, o_queries = 0, o_proof = 0;
+    std::vector<uint64_t> o_layer, o_store;
+    std::vector<LevelStore> stores;  // proof 0
+    uint32_t* arena = nullptr;
     uint64_t proof_words = 0;
-    TreeDesc* d_trees = nullptr;
-    // layout
     uint64_t off_fri_roots = 0, off_last = 0, off_queries = 0;
-    Timeline tl;  // optional launch timeline on the prover's stream
+    TreeDesc* d_trees = nullptr;
+    uint32_t** d_eval_ptrs = nullptr;   // capacity pointers: LDE of proof p
+    uint32_t** d_slot_ptrs = nullptr;   // capacity pointers: trace slot of proof p
+    // call parameters: trace pointers (capacity) then (a, b) per proof
+    uint64_t* d_call = nullptr;
+    uint64_t* h_call = nullptr;  // pinned staging
+    std::vector<uint64_t> last_call;
+    // graph replay of the launch sequence (one graph per batch count)
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_count = 0;
+    Timeline tl;
     void mark(const std::string& label) { tl.mark(label, stream); }
 
-    uint32_t* col(uint32_t j, uint32_t c) const { return layer[j] + (uint64_t)c * (E >> (j + 1)); }
-    uint32_t* alpha_q() const { return alphas; }
-    uint32_t* alpha_f(uint32_t j) const { return alphas + 4 * (1 + j); }
+    uint32_t* at(uint64_t o) const { return arena + o; }
+    uint32_t* col(uint32_t j, uint32_t c) const { return arena + o_layer[j] + (uint64_t)c * (E >> (j + 1)); }
+    uint32_t* alpha_q() const { return arena + o_alpha; }
+    uint32_t* alpha_f(uint32_t j) const { return arena + o_alpha + 4 * (1 + j); }
+    DevChannel* ch() const { return (DevChannel*)(arena + o_ch); }
+    uint32_t* proof(uint32_t p = 0) const { return arena + (uint64_t)p * R + o_proof; }
+    const uint32_t* ab_dev() const { return (const uint32_t*)(d_call + capacity); }
 
     bool alloc(void** p, uint64_t bytes) {
         if (!check(hipMalloc(p, bytes ? bytes : 4), "vx_prover: hipMalloc")) return false;
         allocs.push_back(*p);
         return true;
     }
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_count = 0;
+    }
     ~vx_prover() {
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
         for (void* p : allocs) (void)hipFree(p);
+        if (h_call) (void)hipHostFree(h_call);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -304,14 +354,140 @@ void build_layout(vx_prover* p, Layout& L) {
 
 CPt to_cpt(HostPt h) { return {h.x, h.y}; }
 
+// Word offsets of one arena (each region 256-byte aligned).
+void plan_arena(vx_prover* p) {
+    uint64_t o = 0;
+    auto take = [&](uint64_t words) {
+        uint64_t at = o;
+        o += (words + 63) & ~63ull;
+        return at;
+    };
+    p->o_trace = take(p->N);
+    p->o_eval = take(p->E);
+    p->o_layer.assign(p->J, 0);
+    for (uint32_t j = 0; j < p->J; j++) p->o_layer[j] = take(4 * (p->E >> (j + 1)));
+    p->o_store.clear();
+    p->stores.clear();
+    auto depth_of = [&](uint32_t t) { return t < 2 ? p->log_e : p->log_e - 1 - (t - 2); };
+    for (uint32_t t = 0; t < 2 + p->J; t++) {
+        uint32_t d = depth_of(t), from = DECOMMIT_SUB < d ? DECOMMIT_SUB : d;
+        p->o_store.push_back(take(level_store_words(d, from)));
+    }
+    const uint64_t mx = max_stage_nodes(p->E);
+    p->o_scr_a = take(8 * (mx ? mx : 1));
+    p->o_scr_b = take(8 * (mx ? mx : 1));
+    p->o_ch = take(sizeof(DevChannel) / 4);
+    p->o_alpha = take(4ull * (1 + p->J));
+    p->o_queries = take(p->n_q);
+    p->o_proof = take(p->proof_words);
+    p->R = o;
+}
+
+// Enqueues the whole proof sequence for proofs 0 .. count-1 (trace pointers and
+// (a, b) already in d_call).
+void enqueue_prove(vx_prover* p, uint32_t count) {
+    hipStream_t s = p->stream;
+    const uint64_t N = p->N, E = p->E, R = p->R;
+    p->mark("start");
+    std::string phase;
+    auto mk = [&](const char* what) { p->mark(phase + ":" + what); };
+    // LDE: interpolate on half_coset(log_n) into the first N words of the LDE
+    // buffer, then evaluate on half_coset(log_n + 2) in place (zero pad implicit)
+    ntt_inverse_batch(p->d_eval_ptrs, count, p->trace_tab, s, (const uint32_t* const*)p->d_call);
+    p->mark("lde:interpolate");
+    ntt_forward_batch(p->d_eval_ptrs, count, p->eval_tab, s, N);
+    p->mark("lde:evaluate");
+    Scratch sc = {p->at(p->o_scr_a), p->at(p->o_scr_b)};
+    uint32_t* pr = p->proof();
+    uint32_t* ev = p->at(p->o_eval);
+    phase = "trace_commit";
+    commit_tree(SrcCol1{ev}, E, p->stores[0], TailOut{pr + 5, p->ch(), p->alpha_q(), 1}, sc, s, mk, count, R);
+    phase = "quotient_commit";
+    commit_tree(SrcQuotient{ev, E - 1, p->alpha_q()}, E, p->stores[1], TailOut{pr + 13, p->ch(), p->alpha_f(0), 0},
+                sc, s, mk, count, R);
+    {
+        phase = "fri0_circle_fold_commit";
+        SrcCircleFold src = {ev,           E - 1,        p->alpha_q(), p->alpha_f(0), p->fri_tw[0],
+                             p->col(0, 0), p->col(0, 1), p->col(0, 2), p->col(0, 3)};
+        uint32_t* nxt = p->J > 1 ? p->alpha_f(1) : nullptr;
+        commit_tree(src, E / 2, p->stores[2], TailOut{pr + p->off_fri_roots, p->ch(), nxt, 0}, sc, s, mk, count, R);
+    }
+    for (uint32_t j = 1; j < p->J; j++) {
+        phase = "fri" + std::to_string(j) + "_line_fold_commit";
+        SrcLineFold src = {p->col(j - 1, 0), p->col(j - 1, 1), p->col(j - 1, 2), p->col(j - 1, 3), p->alpha_f(j),
+                           p->fri_tw[j],     p->col(j, 0),     p->col(j, 1),     p->col(j, 2),     p->col(j, 3)};
+        uint32_t* nxt = j + 1 < p->J ? p->alpha_f(j + 1) : nullptr;
+        commit_tree(src, E >> (j + 1), p->stores[2 + j], TailOut{pr + p->off_fri_roots + 8ull * j, p->ch(), nxt, 0},
+                    sc, s, mk, count, R);
+    }
+    const uint32_t jl = p->J - 1;
+    queries_kernel<<<count, 256, 0, s>>>(p->ch(), p->col(jl, 0), p->col(jl, 1), p->col(jl, 2), p->col(jl, 3),
+                                         1u << p->log_last, pr, p->off_last, p->off_queries, p->n_q, E,
+                                         p->at(p->o_queries), p->ab_dev(), R);
+    check_launch("queries_kernel");
+    p->mark("queries");
+    decommit_kernel<<<dim3(p->n_q, 2 + p->J, count), 64, 0, s>>>(p->d_trees, p->at(p->o_queries), pr, R);
+    check_launch("decommit_kernel");
+    p->mark("decommit");
+}
+
+// Uploads trace pointers and (a, b) of a call; waits for the stream first when
+// the pinned staging buffer must change (an earlier copy may still read it).
+bool upload_call(vx_prover* p, const uint32_t* const* traces, const uint32_t* ab, uint32_t count) {
+    std::vector<uint64_t> want(2ull * p->capacity, 0);
+    for (uint32_t i = 0; i < count; i++) {
+        want[i] = (uint64_t)(uintptr_t)traces[i];
+        ((uint32_t*)(want.data() + p->capacity))[2 * i] = ab[2 * i] % P;
+        ((uint32_t*)(want.data() + p->capacity))[2 * i + 1] = ab[2 * i + 1] % P;
+    }
+    if (want == p->last_call) return true;
+    if (!check(hipStreamSynchronize(p->stream), "vx_prover: upload")) return false;
+    std::memcpy(p->h_call, want.data(), 8 * want.size());
+    if (!check(hipMemcpyAsync(p->d_call, p->h_call, 8 * want.size(), hipMemcpyHostToDevice, p->stream), "H2D"))
+        return false;
+    p->last_call = want;
+    return true;
+}
+
+int32_t run_batch(vx_prover* p, uint32_t count) {
+    if (!p->use_graph) {
+        p->tl.n = 0;
+        enqueue_prove(p, count);
+        return peek_error() == 0 ? 0 : -1;
+    }
+    if (!p->graph_exec || p->graph_count != count) {
+        p->drop_graph();
+        const bool timing = p->tl.on;
+        p->tl.on = false;  // no event records inside a captured graph
+        if (!check(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+            return -1;
+        enqueue_prove(p, count);
+        hipGraph_t g = nullptr;
+        const bool ok = check(hipStreamEndCapture(p->stream, &g), "hipStreamEndCapture");
+        p->tl.on = timing;
+        if (!ok || peek_error()) {
+            if (g) (void)hipGraphDestroy(g);
+            return -1;
+        }
+        p->graph = g;
+        if (!check(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0), "hipGraphInstantiate")) {
+            p->drop_graph();
+            return -1;
+        }
+        p->graph_count = count;
+    }
+    return check(hipGraphLaunch(p->graph_exec, p->stream), "hipGraphLaunch") ? 0 : -1;
+}
+
 }  // namespace
 
 extern "C" {
 
-vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer) {
+vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer, uint32_t capacity) {
     if (n_queries == 0) n_queries = 80;       // prover.rs:90
     if (log_last_layer == 0) log_last_layer = 3;  // fri.rs:265
-    if (log_n < 2 || log_n + 2 > 30 || log_last_layer > log_n + 1 || log_last_layer > 10 || n_queries > 4096) {
+    if (log_n < 2 || log_n + 2 > 30 || log_last_layer > log_n + 1 || log_last_layer > 10 || n_queries > 4096 ||
+        capacity < 1 || capacity > 65535) {
         set_error(-2, "vx_prover_create: unsupported parameters");
         return nullptr;
     }
@@ -320,6 +496,7 @@ vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_las
     p->log_e = log_n + 2;  // BLOWUP_BITS = 2 (prover.rs:86)
     p->n_q = n_queries;
     p->log_last = log_last_layer;
+    p->capacity = capacity;
     p->N = 1ull << log_n;
     p->E = 1ull << p->log_e;
     p->J = p->log_e - log_last_layer;
@@ -327,7 +504,7 @@ vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_las
     bool ok = check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
     hipStream_t s = p->stream;
 
-    // twiddle tables (ProverCache, prover.rs:159-176)
+    // twiddle tables (ProverCache, prover.rs:159-176), shared by the batch
     uint32_t *t_line, *t_circle, *t_iline, *t_icircle, *e_line, *e_circle;
     ok = ok && p->alloc((void**)&t_line, 4 * (N - 1)) && p->alloc((void**)&t_circle, 4 * (N / 2)) &&
          p->alloc((void**)&t_iline, 4 * (N - 1)) && p->alloc((void**)&t_icircle, 4 * (N / 2)) &&
@@ -362,46 +539,38 @@ vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_las
         build_fold_twiddles(to_cpt(di), to_cpt(dstp), k, j == 0, p->fri_tw[j], tw_scratch, s);
     }
 
-    // work buffers
-    ok = ok && p->alloc((void**)&p->trace_buf, 4 * N) && p->alloc((void**)&p->coeffs, 4 * N) &&
-         p->alloc((void**)&p->eval, 4 * E);
-    p->layer.assign(p->J, nullptr);
-    for (uint32_t j = 0; ok && j < p->J; j++) ok = p->alloc((void**)&p->layer[j], 16 * (E >> (j + 1)));
-    // level stores: trace, quotient, FRI layers
-    p->stores.clear();
-    auto add_store = [&](uint32_t depth) {
-        uint32_t from = DECOMMIT_SUB < depth ? DECOMMIT_SUB : depth;
-        uint32_t* b = nullptr;
-        ok = ok && p->alloc((void**)&b, 4 * level_store_words(depth, from));
-        p->stores.push_back(LevelStore{b, depth, from});
-    };
-    add_store(p->log_e);
-    add_store(p->log_e);
-    for (uint32_t j = 0; j < p->J; j++) add_store(p->log_e - 1 - j);
-    uint64_t mx = max_stage_nodes(E);
-    ok = ok && p->alloc((void**)&p->scratch.a, 32 * (mx ? mx : 1)) && p->alloc((void**)&p->scratch.b, 32 * (mx ? mx : 1));
-    ok = ok && p->alloc((void**)&p->ch, sizeof(DevChannel)) && p->alloc((void**)&p->alphas, 16 * (1 + p->J)) &&
-         p->alloc((void**)&p->queries, 4 * p->n_q);
-
     Layout L;
     build_layout(p, L);
     p->proof_words = L.words.size();
     p->off_fri_roots = L.fri_roots;
     p->off_last = L.last;
     p->off_queries = L.queries;
-    ok = ok && p->alloc((void**)&p->proof, 4 * p->proof_words);
-    ok = ok && p->alloc((void**)&p->d_trees, sizeof(TreeDesc) * L.trees.size());
+    plan_arena(p);
+    ok = ok && p->alloc((void**)&p->arena, 4 * p->R * capacity);
+    ok = ok && p->alloc((void**)&p->d_trees, sizeof(TreeDesc) * L.trees.size()) &&
+         p->alloc((void**)&p->d_eval_ptrs, 8ull * capacity) && p->alloc((void**)&p->d_slot_ptrs, 8ull * capacity) &&
+         p->alloc((void**)&p->d_call, 16ull * capacity);
+    ok = ok && check(hipHostMalloc((void**)&p->h_call, 16ull * capacity), "hipHostMalloc");
     if (ok) {
-        ok = check(hipMemcpyAsync(p->proof, L.words.data(), 4 * p->proof_words, hipMemcpyHostToDevice, s), "H2D");
+        for (uint32_t t = 0; t < 2 + p->J; t++) {
+            uint32_t d = t < 2 ? p->log_e : p->log_e - 1 - (t - 2);
+            uint32_t from = DECOMMIT_SUB < d ? DECOMMIT_SUB : d;
+            p->stores.push_back(LevelStore{p->at(p->o_store[t]), d, from});
+        }
+        // KRAK template into every proof slot
+        for (uint32_t i = 0; ok && i < capacity; i++)
+            ok = check(hipMemcpyAsync(p->proof(i), L.words.data(), 4 * p->proof_words, hipMemcpyHostToDevice, s),
+                       "H2D");
         std::vector<TreeDesc>& T = L.trees;
+        uint32_t* ev = p->at(p->o_eval);
         T[0].kind = 0;
         T[0].depth = p->log_e;
         T[0].shift = 0;
-        T[0].c[0] = p->eval;
+        T[0].c[0] = ev;
         T[1].kind = 1;
         T[1].depth = p->log_e;
         T[1].shift = 0;
-        T[1].c[0] = p->eval;
+        T[1].c[0] = ev;
         T[1].mask = E - 1;
         T[1].alpha = p->alpha_q();
         for (uint32_t j = 0; j < p->J; j++) {
@@ -415,7 +584,15 @@ vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_las
             T[i].from = p->stores[i].from;
             T[i].store = p->stores[i].base;
         }
+        std::vector<uint64_t> evp(capacity), slots(capacity);
+        for (uint32_t i = 0; i < capacity; i++) {
+            evp[i] = (uint64_t)(uintptr_t)(p->arena + (uint64_t)i * p->R + p->o_eval);
+            slots[i] = (uint64_t)(uintptr_t)(p->arena + (uint64_t)i * p->R + p->o_trace);
+        }
         ok = ok && check(hipMemcpyAsync(p->d_trees, T.data(), sizeof(TreeDesc) * T.size(), hipMemcpyHostToDevice, s),
+                         "H2D");
+        ok = ok && check(hipMemcpyAsync(p->d_eval_ptrs, evp.data(), 8ull * capacity, hipMemcpyHostToDevice, s), "H2D");
+        ok = ok && check(hipMemcpyAsync(p->d_slot_ptrs, slots.data(), 8ull * capacity, hipMemcpyHostToDevice, s),
                          "H2D");
         ok = ok && check(hipStreamSynchronize(s), "vx_prover_create");
     }
@@ -426,14 +603,38 @@ vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_las
     return p;
 }
 
+vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer) {
+    return vx_prover_create_batch(log_n, n_queries, log_last_layer, 1);
+}
+
 void vx_prover_destroy(vx_prover* p) { delete p; }
 void* vx_prover_stream(vx_prover* p) { return p ? (void*)p->stream : nullptr; }
+uint32_t vx_prover_capacity(const vx_prover* p) { return p ? p->capacity : 0; }
 uint64_t vx_prover_proof_size(const vx_prover* p) { return p ? 4 * p->proof_words : 0; }
-const void* vx_prover_proof_device(const vx_prover* p) { return p ? p->proof : nullptr; }
-const uint32_t* vx_prover_eval_device(const vx_prover* p) { return p ? p->eval : nullptr; }
+uint64_t vx_prover_device_bytes(const vx_prover* p) { return p ? 4 * p->R * p->capacity : 0; }
+const void* vx_prover_proof_device(const vx_prover* p) { return p ? p->proof(0) : nullptr; }
+const void* vx_prover_proof_device_at(const vx_prover* p, uint32_t index) {
+    return (p && index < p->capacity) ? p->proof(index) : nullptr;
+}
+const uint32_t* vx_prover_eval_device(const vx_prover* p) { return p ? p->at(p->o_eval) : nullptr; }
 const uint32_t* vx_prover_fri_layer_device(const vx_prover* p, uint32_t j, uint32_t column) {
     if (!p || j >= p->J || column >= 4) return nullptr;
     return p->col(j, column);
+}
+
+int32_t vx_prover_prove_batch_device(vx_prover* p, const uint32_t* const* h_traces, const uint32_t* h_ab,
+                                     uint32_t count) {
+    if (!p || !h_traces || !h_ab || count == 0 || count > p->capacity) {
+        set_error(-2, "vx_prover_prove_batch_device: bad arguments");
+        return -1;
+    }
+    for (uint32_t i = 0; i < count; i++)
+        if (!h_traces[i]) {
+            set_error(-2, "vx_prover_prove_batch_device: null trace");
+            return -1;
+        }
+    if (!upload_call(p, h_traces, h_ab, count)) return -1;
+    return run_batch(p, count);
 }
 
 int32_t vx_prover_prove_device(vx_prover* p, const uint32_t* d_trace, uint32_t a, uint32_t b) {
@@ -441,70 +642,83 @@ int32_t vx_prover_prove_device(vx_prover* p, const uint32_t* d_trace, uint32_t a
         set_error(-2, "vx_prover_prove_device: null argument");
         return -1;
     }
-    hipStream_t s = p->stream;
-    const uint64_t N = p->N, E = p->E;
-    p->tl.n = 0;
-    p->mark("start");
-    std::string phase;
-    auto mk = [&](const char* what) { p->mark(phase + ":" + what); };
-    // LDE: interpolate on half_coset(log_n), evaluate on half_coset(log_n + 2)
-    ntt_inverse(p->coeffs, d_trace, p->trace_tab, s);
-    p->mark("lde:interpolate");
-    ntt_forward(p->eval, p->coeffs, N, p->eval_tab, s);
-    p->mark("lde:evaluate");
-    // trace commitment -> alpha (quotient)
-    phase = "trace_commit";
-    commit_tree(SrcCol1{p->eval}, E, p->stores[0], TailOut{p->proof + 5, p->ch, p->alpha_q(), 1}, p->scratch, s, mk);
-    // quotient commitment -> alpha_0 (circle fold)
-    phase = "quotient_commit";
-    commit_tree(SrcQuotient{p->eval, E - 1, p->alpha_q()}, E, p->stores[1],
-                TailOut{p->proof + 13, p->ch, p->alpha_f(0), 0}, p->scratch, s, mk);
-    // FRI layer 0: circle fold fused with its commitment
-    {
-        phase = "fri0_circle_fold_commit";
-        SrcCircleFold src = {p->eval, E - 1, p->alpha_q(), p->alpha_f(0), p->fri_tw[0],
-                             p->col(0, 0), p->col(0, 1), p->col(0, 2), p->col(0, 3)};
-        uint32_t* nxt = p->J > 1 ? p->alpha_f(1) : nullptr;
-        commit_tree(src, E / 2, p->stores[2], TailOut{p->proof + p->off_fri_roots, p->ch, nxt, 0}, p->scratch, s, mk);
-    }
-    // FRI layers 1..J-1: line folds fused with their commitments
-    for (uint32_t j = 1; j < p->J; j++) {
-        phase = "fri" + std::to_string(j) + "_line_fold_commit";
-        SrcLineFold src = {p->col(j - 1, 0), p->col(j - 1, 1), p->col(j - 1, 2), p->col(j - 1, 3), p->alpha_f(j),
-                           p->fri_tw[j],     p->col(j, 0),     p->col(j, 1),     p->col(j, 2),     p->col(j, 3)};
-        uint32_t* nxt = j + 1 < p->J ? p->alpha_f(j + 1) : nullptr;
-        commit_tree(src, E >> (j + 1), p->stores[2 + j],
-                    TailOut{p->proof + p->off_fri_roots + 8ull * j, p->ch, nxt, 0}, p->scratch, s, mk);
-    }
-    // queries and decommitment
-    uint32_t jl = p->J - 1;
-    queries_kernel<<<1, 256, 0, s>>>(p->ch, p->col(jl, 0), p->col(jl, 1), p->col(jl, 2), p->col(jl, 3),
-                                     1u << p->log_last, p->proof, p->off_last, p->off_queries, p->n_q, E, p->queries,
-                                     a, b);
-    check_launch("queries_kernel");
-    p->mark("queries");
-    decommit_kernel<<<dim3(p->n_q, 2 + p->J), 64, 0, s>>>(p->d_trees, p->queries, p->proof);
-    check_launch("decommit_kernel");
-    p->mark("decommit");
-    return peek_error() == 0 ? 0 : -1;
+    const uint32_t ab[2] = {a, b};
+    return vx_prover_prove_batch_device(p, &d_trace, ab, 1);
 }
 
-uint64_t vx_prover_proof_copy(vx_prover* p, void* host_out, uint64_t cap) {
-    if (!p) return 0;
+int32_t vx_prove_fibonacci_batch(vx_prover* p, const uint32_t* h_ab, uint32_t count) {
+    if (!p || !h_ab || count == 0 || count > p->capacity) {
+        set_error(-2, "vx_prove_fibonacci_batch: bad arguments");
+        return -1;
+    }
+    std::vector<const uint32_t*> slots(count);
+    for (uint32_t i = 0; i < count; i++) slots[i] = p->arena + (uint64_t)i * p->R + p->o_trace;
+    if (!upload_call(p, slots.data(), h_ab, count)) return -1;
+    const uint32_t chunk = 256;
+    const uint64_t threads = (p->N + chunk - 1) / chunk;
+    fib_trace_batch_kernel<<<dim3(ceil_div(threads, 256), count), 256, 0, p->stream>>>(
+        p->ab_dev(), p->N, chunk, p->arena + p->o_trace, p->R);
+    if (!check_launch("fib_trace_batch_kernel")) return -1;
+    return run_batch(p, count);
+}
+
+uint64_t vx_prover_proof_copy_at(vx_prover* p, uint32_t index, void* host_out, uint64_t cap) {
+    if (!p || index >= p->capacity) return 0;
     uint64_t bytes = 4 * p->proof_words;
     if (!host_out || cap < bytes) {
         set_error(-2, "vx_prover_proof_copy: buffer too small");
         return 0;
     }
-    if (!check(hipMemcpyAsync(host_out, p->proof, bytes, hipMemcpyDeviceToHost, p->stream), "D2H")) return 0;
-    if (!check(hipStreamSynchronize(p->stream), "vx_prover_proof_copy")) return 0;
     DevChannel ch;
-    if (!check(hipMemcpy(&ch, p->ch, sizeof(ch), hipMemcpyDeviceToHost), "D2H")) return 0;
+    const DevChannel* dch = (const DevChannel*)((const uint32_t*)p->ch() + (uint64_t)index * p->R);
+    if (!check(hipMemcpyAsync(host_out, p->proof(index), bytes, hipMemcpyDeviceToHost, p->stream), "D2H")) return 0;
+    if (!check(hipMemcpyAsync(&ch, dch, sizeof(ch), hipMemcpyDeviceToHost, p->stream), "D2H")) return 0;
+    if (!check(hipStreamSynchronize(p->stream), "vx_prover_proof_copy")) return 0;
     if (ch.failed) {
         set_error(-3, "transcript: draw_felt exhausted its retry budget");
         return 0;
     }
     return bytes;
+}
+uint64_t vx_prover_proof_copy(vx_prover* p, void* host_out, uint64_t cap) {
+    return vx_prover_proof_copy_at(p, 0, host_out, cap);
+}
+
+int32_t vx_prover_enqueue_copy(vx_prover* p, uint32_t count, void* host_proofs, uint32_t* host_failed, void* stream) {
+    if (!p || count == 0 || count > p->capacity || !host_proofs) {
+        set_error(-2, "vx_prover_enqueue_copy: bad arguments");
+        return -1;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    const uint64_t pb = 4 * p->proof_words, pitch = 4 * p->R;
+    // one strided copy per field: proof i sits i * R words after proof 0
+    if (!check(hipMemcpy2DAsync(host_proofs, pb, p->proof(0), pitch, pb, count, hipMemcpyDeviceToHost, s), "D2H"))
+        return -1;
+    if (host_failed &&
+        !check(hipMemcpy2DAsync(host_failed, 4, &p->ch()->failed, pitch, 4, count, hipMemcpyDeviceToHost, s), "D2H"))
+        return -1;
+    return 0;
+}
+
+uint64_t vx_krak_proof_size(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer) {
+    if (log_n < 2 || log_n + 2 > 30 || log_last_layer > log_n + 1) return 0;
+    vx_prover shape;  // layout only: no device resources
+    shape.log_n = log_n;
+    shape.log_e = log_n + 2;
+    shape.n_q = n_queries ? n_queries : 80;
+    shape.log_last = log_last_layer ? log_last_layer : 3;
+    shape.J = shape.log_e - shape.log_last;
+    Layout L;
+    build_layout(&shape, L);
+    return 4ull * L.words.size();
+}
+
+int32_t vx_prover_set_graph(vx_prover* p, int32_t enable) {
+    if (!p) return -1;
+    if (!check(hipStreamSynchronize(p->stream), "vx_prover_set_graph")) return -1;
+    p->use_graph = enable != 0;
+    if (!p->use_graph) p->drop_graph();
+    return 0;
 }
 
 int32_t vx_prover_set_timing(vx_prover* p, int32_t enable) {
@@ -526,8 +740,8 @@ int32_t vx_fibonacci_trace(uint32_t a, uint32_t b, uint32_t log_n, uint32_t* d_o
 
 uint64_t vx_prove_fibonacci(vx_prover* p, uint32_t a, uint32_t b, void* host_out, uint64_t cap) {
     if (!p) return 0;
-    if (vx_fibonacci_trace(a, b, p->log_n, p->trace_buf, p->stream)) return 0;
-    if (vx_prover_prove_device(p, p->trace_buf, a, b)) return 0;
+    const uint32_t ab[2] = {a, b};
+    if (vx_prove_fibonacci_batch(p, ab, 1)) return 0;
     return vx_prover_proof_copy(p, host_out, cap);
 }
 

@@ -1,0 +1,369 @@
+// Proof server: continuous batching of Fibonacci proof jobs of mixed sizes on
+// one GPU, admitted against an HBM budget (Part 5 of the C ABI).
+//
+// A job is n_cols Fibonacci columns of 2^log_n rows; each column becomes one
+// KRAK v1 proof, bit-identical to proving it alone (every proof of a batch
+// owns its arena, so nothing is shared between proofs but the twiddles).
+//
+// Scheduling, all on the caller's thread (vx_server_poll is one step):
+//  * every size class (log_n) keeps a FIFO of pending proofs and up to
+//    MAX_PROVERS_PER_CLASS batched provers, each on its own stream;
+//  * an idle prover takes up to its capacity of pending proofs at once, so
+//    requests that arrive while a batch runs join the next batch of their
+//    class, and batches of different classes overlap on the device;
+//  * a prover is created when its class has work and its arena fits the HBM
+//    budget; idle provers of classes without work are released (oldest first)
+//    to make room;
+//  * a finished batch is noticed through an event query; its proofs were
+//    already copied to pinned host memory on the prover's stream.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+constexpr uint32_t MAX_PROVERS_PER_CLASS = 2;
+constexpr uint32_t M31_P = 0x7FFFFFFFu;
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// proofs per batch: enough LDE rows per launch to fill the chip, at most 128
+uint32_t class_capacity(uint32_t log_n) {
+    if (log_n >= 19) return 2;
+    return std::min<uint32_t>(128, std::max<uint32_t>(2, 1u << (19 - log_n)));
+}
+
+struct Job {
+    uint32_t log_n = 0, n_cols = 0, remaining = 0;
+    double arrival = 0, done = -1;
+    int32_t error = 0;
+    std::vector<uint8_t> proofs;  // n_cols * proof_size
+    uint64_t proof_size = 0;
+};
+
+struct Item {
+    int64_t job;
+    uint32_t col;
+    uint32_t a, b;
+};
+
+struct Slot {
+    vx_prover* prover = nullptr;
+    uint32_t capacity = 0;
+    uint64_t bytes = 0;
+    uint64_t proof_size = 0;
+    hipEvent_t done = nullptr;
+    uint8_t* h_proofs = nullptr;  // pinned: capacity * proof_size
+    uint32_t* h_failed = nullptr;  // pinned: capacity words
+    std::vector<Item> running;
+    double last_used = 0;
+    bool busy() const { return !running.empty(); }
+};
+
+struct Class {
+    std::deque<Item> pending;
+    std::vector<std::unique_ptr<Slot>> slots;
+};
+
+}  // namespace
+
+struct vx_server {
+    uint64_t budget = 0, used = 0;
+    uint32_t n_q = 80, log_last = 3;
+    std::map<uint32_t, Class> classes;
+    std::vector<Job> jobs;
+    uint64_t completed = 0, batches = 0, proofs_done = 0, peak_used = 0;
+
+    ~vx_server() {
+        for (auto& kv : classes)
+            for (auto& s : kv.second.slots) release(*s);
+    }
+    void release(Slot& s) {
+        if (s.prover) {
+            (void)hipStreamSynchronize((hipStream_t)vx_prover_stream(s.prover));
+            vx_prover_destroy(s.prover);
+            used -= s.bytes;
+        }
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.h_proofs) (void)hipHostFree(s.h_proofs);
+        if (s.h_failed) (void)hipHostFree(s.h_failed);
+        s = Slot();
+    }
+    bool make_slot(uint32_t log_n, Slot& s) {
+        const uint32_t cap = class_capacity(log_n);
+        vx_prover* p = vx_prover_create_batch(log_n, n_q, log_last, cap);
+        if (!p) return false;
+        s.prover = p;
+        s.capacity = cap;
+        s.bytes = vx_prover_device_bytes(p);
+        s.proof_size = vx_prover_proof_size(p);
+        used += s.bytes;
+        peak_used = std::max(peak_used, used);
+        bool ok = vx::check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate") &&
+                  vx::check(hipHostMalloc((void**)&s.h_proofs, s.proof_size * cap), "hipHostMalloc") &&
+                  vx::check(hipHostMalloc((void**)&s.h_failed, 4ull * cap), "hipHostMalloc");
+        if (!ok) release(s);
+        return ok;
+    }
+    // estimate before creating: capacity * arena; a created prover reports its real size
+    static uint64_t estimate_bytes(uint32_t log_n) {
+        const uint64_t E = 1ull << (log_n + 2);
+        return (uint64_t)class_capacity(log_n) * (28 * E + (4ull << log_n)) + (16 * E);
+    }
+    bool reserve_room(uint64_t need, uint32_t for_class) {
+        while (used + need > budget) {
+            // release the idle prover that was used least recently, in a class without pending work
+            Slot* victim = nullptr;
+            uint32_t victim_class = 0;
+            for (auto& kv : classes) {
+                if (kv.first == for_class || !kv.second.pending.empty()) continue;
+                for (auto& s : kv.second.slots)
+                    if (s->prover && !s->busy() && (!victim || s->last_used < victim->last_used)) {
+                        victim = s.get();
+                        victim_class = kv.first;
+                    }
+            }
+            if (!victim) return false;
+            release(*victim);
+            auto& v = classes[victim_class].slots;
+            v.erase(std::remove_if(v.begin(), v.end(), [](const std::unique_ptr<Slot>& x) { return !x->prover; }),
+                    v.end());
+        }
+        return true;
+    }
+    void launch(uint32_t log_n, Class& c, Slot& s) {
+        const uint32_t count = (uint32_t)std::min<size_t>(s.capacity, c.pending.size());
+        std::vector<uint32_t> ab(2 * count);
+        s.running.clear();
+        for (uint32_t i = 0; i < count; i++) {
+            s.running.push_back(c.pending.front());
+            c.pending.pop_front();
+            ab[2 * i] = s.running[i].a;
+            ab[2 * i + 1] = s.running[i].b;
+        }
+        hipStream_t st = (hipStream_t)vx_prover_stream(s.prover);
+        bool ok = vx_prove_fibonacci_batch(s.prover, ab.data(), count) == 0;
+        // proofs and transcript flags to pinned memory on the same stream
+        ok = ok && vx_prover_enqueue_copy(s.prover, count, s.h_proofs, s.h_failed, st) == 0;
+        ok = ok && vx::check(hipEventRecord(s.done, st), "hipEventRecord");
+        s.last_used = now_s();
+        batches++;
+        if (!ok) fail_running(s, -1);
+        (void)log_n;
+    }
+    void fail_running(Slot& s, int32_t code) {
+        for (const Item& it : s.running) {
+            Job& j = jobs[it.job];
+            j.error = code;
+            if (--j.remaining == 0) finish(j);
+        }
+        s.running.clear();
+    }
+    void finish(Job& j) {
+        j.done = now_s();
+        completed++;
+    }
+    void retire(Slot& s) {
+        for (uint32_t i = 0; i < s.running.size(); i++) {
+            const Item& it = s.running[i];
+            Job& j = jobs[it.job];
+            std::memcpy(j.proofs.data() + (uint64_t)it.col * j.proof_size, s.h_proofs + (uint64_t)i * s.proof_size,
+                        s.proof_size);
+            if (s.h_failed[i]) j.error = -3;
+            proofs_done++;
+            if (--j.remaining == 0) finish(j);
+        }
+        s.running.clear();
+        s.last_used = now_s();
+    }
+    bool any_busy() const {
+        for (auto& kv : classes)
+            for (auto& s : kv.second.slots)
+                if (s->busy()) return true;
+        return false;
+    }
+    // one scheduler step; returns the number of batches launched or retired
+    int32_t poll() {
+        int32_t acted = 0;
+        for (auto& kv : classes) {
+            for (auto& s : kv.second.slots) {
+                if (!s->busy()) continue;
+                hipError_t q = hipEventQuery(s->done);
+                if (q == hipSuccess) {
+                    retire(*s);
+                    acted++;
+                } else if (q != hipErrorNotReady) {
+                    vx::check(q, "vx_server: batch failed");
+                    fail_running(*s, (int32_t)q);
+                    acted++;
+                }
+            }
+        }
+        for (auto& kv : classes) {
+            Class& c = kv.second;
+            if (c.pending.empty()) continue;
+            for (auto& s : c.slots)
+                if (!s->busy() && !c.pending.empty()) {
+                    launch(kv.first, c, *s);
+                    acted++;
+                }
+            // more proofs waiting than the idle provers take: add a prover if the budget allows
+            if (!c.pending.empty() && c.slots.size() < MAX_PROVERS_PER_CLASS) {
+                const uint64_t need = estimate_bytes(kv.first);
+                if (reserve_room(need, kv.first)) {
+                    auto s = std::make_unique<Slot>();
+                    if (make_slot(kv.first, *s)) {
+                        c.slots.push_back(std::move(s));
+                        launch(kv.first, c, *c.slots.back());
+                        acted++;
+                    } else {
+                        (void)vx_get_last_error();  // out of device memory: retry when something frees up
+                    }
+                } else if (c.slots.empty() && !any_busy()) {
+                    // nothing will free memory: this class can never run under the budget
+                    vx::set_error(-4, "vx_server: a proof of this size does not fit the HBM budget");
+                    for (const Item& it : c.pending) {
+                        Job& j = jobs[it.job];
+                        j.error = -4;
+                        if (--j.remaining == 0) finish(j);
+                    }
+                    c.pending.clear();
+                    acted++;
+                }
+            }
+        }
+        return acted;
+    }
+};
+
+extern "C" {
+
+vx_server* vx_server_create(uint64_t budget_bytes, uint32_t n_queries, uint32_t log_last_layer) {
+    size_t free_b = 0, total_b = 0;
+    if (!vx::check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo")) return nullptr;
+    vx_server* s = new vx_server();
+    const uint64_t avail = (uint64_t)(0.9 * (double)free_b);
+    s->budget = budget_bytes ? std::min<uint64_t>(budget_bytes, avail) : avail;
+    s->n_q = n_queries ? n_queries : 80;
+    s->log_last = log_last_layer ? log_last_layer : 3;
+    return s;
+}
+
+void vx_server_destroy(vx_server* s) { delete s; }
+
+int64_t vx_server_submit(vx_server* s, uint32_t log_n, uint32_t n_cols, const uint32_t* ab) {
+    if (!s || !ab || n_cols == 0 || log_n < 2 || log_n + 2 > 30 || s->log_last > log_n + 1) {
+        vx::set_error(-2, "vx_server_submit: bad job");
+        return -1;
+    }
+    const int64_t id = (int64_t)s->jobs.size();
+    Job j;
+    j.log_n = log_n;
+    j.n_cols = n_cols;
+    j.remaining = n_cols;
+    j.arrival = now_s();
+    // proof size of this class: the KRAK layout depends only on (log_n, n_q, log_last)
+    Class& c = s->classes[log_n];
+    uint64_t ps = 0;
+    for (auto& sl : c.slots)
+        if (sl->prover) ps = sl->proof_size;
+    if (!ps) ps = vx_krak_proof_size(log_n, s->n_q, s->log_last);
+    j.proof_size = ps;
+    j.proofs.assign(ps * n_cols, 0);
+    s->jobs.push_back(std::move(j));
+    for (uint32_t k = 0; k < n_cols; k++) c.pending.push_back(Item{id, k, ab[2 * k] % M31_P, ab[2 * k + 1] % M31_P});
+    return id;
+}
+
+int32_t vx_server_poll(vx_server* s) { return s ? s->poll() : -1; }
+
+uint64_t vx_server_completed(const vx_server* s) { return s ? s->completed : 0; }
+uint64_t vx_server_jobs(const vx_server* s) { return s ? s->jobs.size() : 0; }
+
+int32_t vx_server_wait_all(vx_server* s) {
+    if (!s) return -1;
+    while (s->completed < s->jobs.size()) {
+        if (s->poll() == 0) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        if (vx::peek_error()) return -1;
+    }
+    return 0;
+}
+
+void vx_server_release_job(vx_server* s, int64_t job) {
+    if (!s || job < 0 || job >= (int64_t)s->jobs.size() || s->jobs[job].done < 0) return;
+    std::vector<uint8_t>().swap(s->jobs[job].proofs);
+}
+
+int32_t vx_server_job_status(const vx_server* s, int64_t job) {
+    if (!s || job < 0 || job >= (int64_t)s->jobs.size()) return -2;
+    const Job& j = s->jobs[job];
+    if (j.error) return j.error;
+    return j.done >= 0 ? 1 : 0;
+}
+
+double vx_server_job_latency_ms(const vx_server* s, int64_t job) {
+    if (!s || job < 0 || job >= (int64_t)s->jobs.size() || s->jobs[job].done < 0) return -1.0;
+    return 1e3 * (s->jobs[job].done - s->jobs[job].arrival);
+}
+
+uint64_t vx_server_proof_copy(const vx_server* s, int64_t job, uint32_t col, void* out, uint64_t cap) {
+    if (!s || job < 0 || job >= (int64_t)s->jobs.size()) return 0;
+    const Job& j = s->jobs[job];
+    if (j.done < 0 || j.error || col >= j.n_cols || cap < j.proof_size || !out) return 0;
+    std::memcpy(out, j.proofs.data() + (uint64_t)col * j.proof_size, j.proof_size);
+    return j.proof_size;
+}
+
+// Replays an arrival trace in real time: job i arrives arrival_s[i] seconds after
+// the call starts (ascending), with n_cols[i] columns whose (a, b) pairs follow
+// each other in ab. Returns when every job is done; latency_ms[i] (optional)
+// gets arrival-to-completion times.
+int32_t vx_server_run_trace(vx_server* s, uint32_t n_jobs, const double* arrival_s, const uint32_t* log_n,
+                            const uint32_t* n_cols, const uint32_t* ab, double* latency_ms) {
+    if (!s || (n_jobs && (!arrival_s || !log_n || !n_cols || !ab))) {
+        vx::set_error(-2, "vx_server_run_trace: bad arguments");
+        return -1;
+    }
+    const double t0 = now_s();
+    const int64_t first = (int64_t)s->jobs.size();
+    uint32_t next = 0;
+    uint64_t ab_pos = 0;
+    while (next < n_jobs || s->completed < s->jobs.size()) {
+        const double t = now_s() - t0;
+        while (next < n_jobs && arrival_s[next] <= t) {
+            if (vx_server_submit(s, log_n[next], n_cols[next], ab + 2 * ab_pos) < 0) return -1;
+            s->jobs.back().arrival = t0 + arrival_s[next];
+            ab_pos += n_cols[next];
+            next++;
+        }
+        if (s->poll() == 0) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        if (vx::peek_error()) return -1;
+    }
+    if (latency_ms)
+        for (uint32_t i = 0; i < n_jobs; i++) latency_ms[i] = vx_server_job_latency_ms(s, first + i);
+    return 0;
+}
+
+// Counters: [0] batches launched, [1] proofs done, [2] jobs done, [3] HBM bytes
+// held by provers now, [4] peak, [5] budget.
+void vx_server_stats(const vx_server* s, uint64_t out[6]) {
+    if (!s || !out) return;
+    out[0] = s->batches;
+    out[1] = s->proofs_done;
+    out[2] = s->completed;
+    out[3] = s->used;
+    out[4] = s->peak_used;
+    out[5] = s->budget;
+}
+
+}  // extern "C"
