@@ -1,16 +1,23 @@
-"""Benchmark of the plain commit pipeline on one MI355X.
+"""Benchmark of the commit pipeline on one MI355X.
 
-One step = one full proof of the plain GPU workload: a single Fibonacci M31
-column of 2^24 rows already resident in HBM -> circle LDE on the 2^26 domain
--> Blake2s trace commitment -> quotient commitment -> circle fold and 22 line
-folds, each fused with its Blake2s commitment -> Fiat-Shamir queries (100)
--> decommitment, leaving the KRAK v1 proof bytes in HBM. All steps are
-enqueued back to back on the prover's stream and timed between device
-synchronisations.
+Headline (value): the plain GPU workload. One step = one full proof of a
+single Fibonacci M31 column of 2^24 rows already resident in HBM -> circle LDE
+on the 2^26 domain -> Blake2s trace commitment -> quotient commitment ->
+circle fold and 22 line folds, each fused with its Blake2s commitment ->
+Fiat-Shamir queries (100) -> decommitment, leaving the KRAK v1 proof bytes in
+HBM. All steps are enqueued back to back on the prover's stream and timed
+between device synchronisations.
 
-Prints one JSON line (see README of the contract in DESIGN.md).
+The same JSON line carries the other workloads of the metric:
+  wide    34 uniform M31 columns (4 x 2^14, 10 x 2^18, 20 x 2^20), stwo LDE
+          blowup 4, one lifted Blake2s tree, fused FRI fold+commit to 2^4;
+  batch   256 jobs of 1-8 Fibonacci columns, log_n uniform in [10, 18]:
+          batched proving (graph replay) vs the same proofs one at a time;
+  serve   Poisson arrivals of jobs with 1-34 columns, log_n uniform in
+          [12, 26], admitted under the HBM budget: proofs/s and p50/p99.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--log-n 24] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+                    [--skip wide,batch,serve]
 """
 
 from __future__ import annotations
@@ -22,6 +29,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -50,36 +59,46 @@ def golden_case(label):
     return None
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--log-n", type=int, default=24)
-    ap.add_argument("--queries", type=int, default=100)
-    ap.add_argument("--log-last", type=int, default=4)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-log-n", type=int, default=24)
-    ap.add_argument("--cpu-reps", type=int, default=2)
-    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
-                    help="also run a device-to-device copy of this many MiB (counter calibration)")
-    args = ap.parse_args()
+def stage_compressions(n_items: int, leaf_blocks: int) -> int:
+    """Blake2s compressions of the first stage kernel of a tree: leaf hashes of
+    n_items leaves (leaf_blocks each) and every node below each wave's output
+    (a wave reduces 128 * 2^logc items to one node; logc as the library picks it)."""
+    logc = 5
+    while logc > 0 and (n_items >> (7 + logc)) < 1024:
+        logc -= 1
+    waves = n_items >> (7 + logc)
+    return n_items * leaf_blocks + (n_items - waves)
 
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(independent proofs shard by job; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
-        return 2
 
-    import vortexstark_amd as vx
+def valu_roofline(kernel: str, compressions: int, ms: float) -> dict:
+    ops = compressions * OPS_PER_COMPRESSION
+    achieved = ops / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "Tops/s (int32 lane-ops)", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+            "avg_ms": round(ms, 4), "ops_per_launch": ops}
 
-    assert vx.gpu_available(), "no HIP device"
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        return ""
+
+
+def quantiles(xs):
+    xs = sorted(xs)
+    return {"p50": round(xs[len(xs) // 2], 3), "p99": round(xs[min(len(xs) - 1, int(round(0.99 * (len(xs) - 1))))], 3)}
+
+
+def bench_plain(vx, args) -> dict:
     log_n, nq, last = args.log_n, args.queries, args.log_last
     golden = golden_case("gpu_plain") if (log_n, nq, last) == (24, 100, 4) else None
     if golden:
         a, b = golden["a"], golden["b"]
     else:
         a, b = 1, 1
-    log(f"bench: log_n={log_n} queries={nq} last=2^{last} a={a} b={b}")
+    log(f"plain: log_n={log_n} queries={nq} last=2^{last} a={a} b={b}")
     t0 = time.time()
     prover = vx.Prover(log_n, nq, last)
     log(f"prover ready ({time.time() - t0:.2f}s: twiddles + workspace)")
@@ -109,8 +128,7 @@ def main() -> int:
     elapsed = time.perf_counter() - t_start
     vx.check("timed steps")
     ms_per_step = 1e3 * elapsed / args.steps
-    n_rows = 1 << log_n
-    melem_s = n_rows * args.steps / elapsed / 1e6
+    melem_s = (1 << log_n) * args.steps / elapsed / 1e6
     log(f"timed: {args.steps} proofs in {elapsed:.3f}s -> {ms_per_step:.3f} ms/proof, {melem_s:.1f} Melem/s")
 
     proof = prover.proof_bytes()
@@ -126,9 +144,6 @@ def main() -> int:
         prover.prove_device(trace.addr, a, b)
         vx.lib.vx_stream_synchronize(stream)
         lat.append(1e3 * (time.perf_counter() - t))
-    lat.sort()
-    p50 = lat[len(lat) // 2]
-    p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
 
     # launch timeline of one proof (HIP events on the prover's stream)
     vx.lib.vx_prover_set_timing(prover.handle, 1)
@@ -144,32 +159,18 @@ def main() -> int:
 
     E = 1 << (log_n + 2)
     # dominant kernel: the first commit stage of the quotient tree (Blake2s over E leaves)
-    dom = "quotient_commit:stage_leaf"
-    dom_ms = phases.get(dom)
-    roofline = None
-    if dom_ms:
-        logc = 5
-        while logc > 0 and (E >> (7 + logc)) < 1024:
-            logc -= 1
-        compressions = E + (E - (E >> (logc + 1)))  # leaf hashes + nodes inside the stage
-        ops = compressions * OPS_PER_COMPRESSION
-        achieved = ops / (dom_ms * 1e-3) / 1e12
-        roofline = {"bound": "valu", "kernel": "merkle_stage_kernel<SrcQuotient>", "achieved": round(achieved, 3),
-                    "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s (int32 lane-ops)",
-                    "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-                    "avg_ms": round(dom_ms, 4), "ops_per_launch": ops}
+    dom_ms = phases.get("quotient_commit:stage_leaf")
+    roofline = valu_roofline("merkle_stage_kernel<SrcQuotient>", stage_compressions(E, 1), dom_ms) if dom_ms else None
     ntt_ms = phases.get("lde:evaluate")
     roofline_hbm = None
     if ntt_ms:
-        n = 1 << log_n
-        byts = 4 * n + 4 * E  # read coefficients once, write the LDE once
+        byts = 4 * (1 << log_n) + 4 * E  # read coefficients once, write the LDE once
         achieved = byts / (ntt_ms * 1e-3) / 1e9
         roofline_hbm = {"bound": "hbm", "kernel": "ntt_pass_kernel<true> x3 (evaluate 2^%d)" % (log_n + 2),
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "phase_ms": round(ntt_ms, 4)}
     # pipeline-level bound: 6E Blake2s compressions at the VALU peak
-    total_compressions = 6 * E
-    t_valu_ms = total_compressions * OPS_PER_COMPRESSION / (VALU_PEAK_TOPS * 1e12) * 1e3
+    t_valu_ms = 6 * E * OPS_PER_COMPRESSION / (VALU_PEAK_TOPS * 1e12) * 1e3
 
     cpu = None
     if not args.no_cpu:
@@ -182,25 +183,251 @@ def main() -> int:
         log(f"cpu baseline: oracle prove at log_n={cl}, {args.cpu_reps} reps, {threads} threads")
         secs = L.orc_time_prove_fibonacci(a % vx.P, b % vx.P, cl, nq, last, args.cpu_reps)
         cpu_rate = (1 << cl) * args.cpu_reps / secs / 1e6
-        model = ""
-        try:
-            with open("/proc/cpuinfo") as f:
-                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-        except OSError:
-            pass
         cpu = {"value": round(cpu_rate, 4), "unit": "Melem/s", "cores": threads, "kind": "port",
                "sample": f"oracle (C++ restatement) full proof of a 2^{cl}-row Fibonacci column, {nq} queries, "
-                         f"last layer 2^{last}, {args.cpu_reps} reps after twiddle precompute, {secs:.2f}s; {model}"}
+                         f"last layer 2^{last}, {args.cpu_reps} reps after twiddle precompute, {secs:.2f}s; "
+                         f"{cpu_model()}"}
         log(f"cpu baseline: {cpu_rate:.3f} Melem/s ({secs:.2f}s)")
+    prover.close()
+    return {"melem_s": melem_s, "ms_per_step": ms_per_step, "elapsed": elapsed, "proofs_per_s": args.steps / elapsed,
+            "latency": quantiles(lat), "roofline": roofline, "roofline_hbm": roofline_hbm,
+            "t_valu_ms": t_valu_ms, "phases": phases, "parity": parity, "cpu": cpu}
 
+
+def bench_wide(vx, args) -> dict:
+    import inputs
+
+    logs = inputs.WIDE_LOG_SIZES
+    cols = inputs.m31_columns(inputs.WIDE_SEED, logs)
+    w = vx.WideCommit(logs, 2, 4)
+    bufs = [vx.DeviceBuffer.from_host(c) for c in cols]
+    addrs = [b.addr for b in bufs]
+    for _ in range(3):
+        w.commit_device(addrs)
+    vx.synchronize_stream(w.stream)
+    K = 10
+    t = time.perf_counter()
+    for _ in range(K):
+        w.commit_device(addrs)
+    vx.synchronize_stream(w.stream)
+    dt = (time.perf_counter() - t) / K
+    root, fri, last, beta = w.results()
+    g = None
+    path = os.path.join(ROOT, "tests", "golden", "wide.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            g = next((c for c in json.load(f) if c["label"] == "wide_config"), None)
+    parity = None if g is None else (root.tolist() == g["root"] and fri.tolist() == g["fri_roots"]
+                                     and last.tolist() == g["last_layer"])
+    w.set_timing(True)
+    w.commit_device(addrs)
+    vx.synchronize_stream(w.stream)
+    phases = w.timeline()
+    w.set_timing(False)
+    trace_elems = sum(1 << l for l in logs)
+    S = w.lifting_log_size
+    dom_ms = phases.get("lifted_commit:stage_leaf")
+    blocks = (len(logs) + 15) // 16
+    roofline = valu_roofline("merkle_stage_kernel<SrcLifted>", stage_compressions(1 << S, blocks), dom_ms) \
+        if dom_ms else None
+    log(f"wide: {1e3 * dt:.3f} ms per commitment, {trace_elems / dt / 1e6:.1f} Melem/s, parity {parity}")
+    cpu = None
+    if not args.no_cpu:
+        import oracle_lib
+
+        threads = min(16, os.cpu_count() or 1)
+        L = oracle_lib.lib()
+        L.orc_set_threads(threads)
+        ptrs = (C.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+        lg = np.array(logs, np.uint32)
+        secs = L.orc_time_wide_commit(ptrs, lg.ctypes.data, len(cols), 2, 4)
+        cpu = {"value": round(trace_elems / secs / 1e6, 4), "unit": "Melem/s", "cores": threads, "kind": "port",
+               "sample": f"oracle wide commitment of the same 34 columns, one run, {secs:.2f}s"}
+        log(f"wide cpu baseline: {trace_elems / secs / 1e6:.2f} Melem/s ({secs:.2f}s)")
+    w.close()
+    return {"workload": "34 uniform M31 columns (4x2^14, 10x2^18, 20x2^20, splitmix64 seed 3), stwo LDE blowup 4, "
+                        "lifted Blake2s tree over all columns, FRI on the beta-combination to 2^4",
+            "ms_per_commit": round(1e3 * dt, 4), "melem_s": round(trace_elems / dt / 1e6, 2),
+            "lde_melem_s": round(sum(1 << (l + 2) for l in logs) / dt / 1e6, 2), "parity_vs_oracle": parity,
+            "roofline": roofline, "phases_ms": {k: round(v, 4) for k, v in phases.items()}, "cpu_baseline": cpu}
+
+
+def batch_jobs(seed: int, n_jobs: int, min_log: int, max_log: int, max_cols: int):
+    import inputs
+
+    r = inputs.splitmix64(seed, 2 * n_jobs)
+    jobs, pos = [], 0
+    for j in range(n_jobs):
+        log_n = min_log + int(r[2 * j] % np.uint64(max_log - min_log + 1))
+        cols = 1 + int(r[2 * j + 1] % np.uint64(max_cols))
+        v = inputs.splitmix64(seed + 1000, 2 * cols, pos) % np.uint64(vx_p())
+        pos += 2 * cols
+        jobs.append((log_n, [(int(v[2 * c]), int(v[2 * c + 1])) for c in range(cols)]))
+    return jobs
+
+
+def vx_p() -> int:
+    return (1 << 31) - 1
+
+
+def bench_batch(vx, args) -> dict:
+    jobs = batch_jobs(4, 256, 10, 18, 8)
+    n_proofs = sum(len(ab) for _, ab in jobs)
+    rows = sum((1 << l) * len(ab) for l, ab in jobs)
+    nq, last = 80, 3
+    # one job at a time on the plain path: one prover per size, each proof to host bytes
+    solo = {l: vx.Prover(l, nq, last) for l in sorted({l for l, _ in jobs})}
+    for l, p in solo.items():
+        p.prove(1, 1)
+    t = time.perf_counter()
+    solo_out = []
+    for l, ab in jobs:
+        solo_out.append([solo[l].prove(a, b) for a, b in ab])
+    t_solo = time.perf_counter() - t
+    for p in solo.values():
+        p.close()
+    log(f"batch: {n_proofs} proofs one at a time: {t_solo:.3f}s ({n_proofs / t_solo:.1f} proofs/s)")
+    # batched: all jobs submitted at once; provers created and graphs captured by a first pass
+    srv = vx.Server(0, nq, last, result_bytes=4 << 30)
+    srv.set_graphs(True)
+    warm = [srv.submit(l, ab) for l, ab in jobs]
+    srv.wait_all()
+    for i in warm:
+        srv.release(i)
+    t = time.perf_counter()
+    ids = [srv.submit(l, ab) for l, ab in jobs]
+    srv.wait_all()
+    t_batch = time.perf_counter() - t
+    lat = [srv.latency_ms(i) for i in ids]
+    same = all(srv.proof(i, c, l) == solo_out[k][c] for k, (i, (l, ab)) in enumerate(zip(ids, jobs))
+               for c in range(len(ab)))
+    st = srv.stats()
+    srv.close()
+    log(f"batch: batched {t_batch:.3f}s ({n_proofs / t_batch:.1f} proofs/s), {t_solo / t_batch:.2f}x, "
+        f"identical to one-at-a-time: {same}")
+    cpu = None
+    if not args.no_cpu:
+        import oracle_lib
+
+        threads = min(16, os.cpu_count() or 1)
+        oracle_lib.lib().orc_set_threads(threads)
+        sample = [(l, a, b) for l, ab in jobs[:12] for a, b in ab][:32]
+        t = time.perf_counter()
+        for l, a, b in sample:
+            oracle_lib.prove(a, b, l, nq, last)
+        secs = time.perf_counter() - t
+        cpu = {"value": round(len(sample) / secs, 3), "unit": "proofs/s", "cores": threads, "kind": "port",
+               "sample": f"oracle proofs of the first {len(sample)} proofs of the same job list, {secs:.2f}s"}
+    return {"workload": "256 jobs of 1-8 Fibonacci columns, log_n uniform in [10,18] (splitmix64 seed 4), "
+                        "blowup 4, 80 queries, last layer 2^3; every column one KRAK proof",
+            "n_proofs": n_proofs, "rows": rows, "batched_proofs_per_s": round(n_proofs / t_batch, 2),
+            "one_at_a_time_proofs_per_s": round(n_proofs / t_solo, 2),
+            "speedup_vs_one_at_a_time": round(t_solo / t_batch, 3), "batched_s": round(t_batch, 4),
+            "one_at_a_time_s": round(t_solo, 4), "batched_melem_s": round(rows / t_batch / 1e6, 2),
+            "job_latency_ms": quantiles(lat), "identical_to_one_at_a_time": same,
+            "batches": st["batches"], "peak_device_bytes": st["peak_device_bytes"], "cpu_baseline": cpu}
+
+
+def bench_serve(vx, args) -> dict:
+    """Two replays of the same job list: at the offered rate (latency) and with
+    every job arriving within a fraction of a second (throughput at saturation)."""
+    import inputs
+
+    jobs = batch_jobs(5, args.serve_jobs, 12, 26, 34)
+    u = (inputs.splitmix64(6, len(jobs)) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    gaps = -np.log1p(-u)
+    nq, last = 80, 3
+    srv = vx.Server(288 << 30, nq, last, result_bytes=8 << 30)
+    # warm-up: one single-column job per size creates the provers (outside the replays)
+    for l in sorted({l for l, _ in jobs}):
+        srv.submit(l, [(1, 1)])
+    srv.wait_all()
+    for i in range(srv.stats()["jobs_done"]):
+        srv.release(i)
+    n_proofs = sum(len(ab) for _, ab in jobs)
+    rows = sum((1 << l) * len(ab) for l, ab in jobs)
+    runs = {}
+    for name, rate in (("offered", args.serve_rate), ("saturated", args.serve_rate * 50)):
+        arrivals = np.cumsum(gaps / rate)
+        arrivals -= arrivals[0]
+        first = srv.stats()["jobs_done"]
+        t = time.perf_counter()
+        lat = srv.run_trace(arrivals, [l for l, _ in jobs], [len(ab) for _, ab in jobs],
+                            [v for _, ab in jobs for pair in ab for v in pair])
+        span = time.perf_counter() - t
+        ok = all(srv.status(first + i) == 1 for i in range(len(jobs)))
+        for i in range(len(jobs)):
+            srv.release(first + i)
+        runs[name] = {"rate_jobs_per_s": rate, "makespan_s": round(span, 4), "proofs_per_s": round(n_proofs / span, 2),
+                      "melem_s": round(rows / span / 1e6, 2), "latency_ms": quantiles(list(lat)), "all_done": ok}
+        log(f"serve ({name}, {rate:.0f} jobs/s): {n_proofs} proofs in {span:.3f}s, {rows / span / 1e6:.0f} Melem/s, "
+            f"p50 {np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms")
+    st = srv.stats()
+    srv.close()
+    # the same proofs one at a time on the plain path (time per size measured once, then summed)
+    per = {}
+    for l in sorted({l for l, _ in jobs}):
+        p = vx.Prover(l, nq, last)
+        p.prove(1, 1)
+        t = time.perf_counter()
+        p.prove(2, 3)
+        per[l] = time.perf_counter() - t
+        p.close()
+    t_serial = sum(per[l] * len(ab) for l, ab in jobs)
+    sat = runs["saturated"]["makespan_s"]
+    log(f"serve: one at a time {t_serial:.3f}s -> {t_serial / sat:.2f}x at saturation")
+    return {"workload": f"{len(jobs)} jobs, Poisson arrivals, 1-34 Fibonacci columns, log_n uniform in [12,26] "
+                        "(splitmix64 seeds 5, 6), 80 queries, last layer 2^3, HBM budget 288 GB",
+            "n_proofs": n_proofs, "rows": rows, **runs,
+            "one_at_a_time_s": round(t_serial, 4), "speedup_vs_one_at_a_time": round(t_serial / sat, 3),
+            "one_at_a_time_ms_per_proof": {str(l): round(1e3 * v, 3) for l, v in per.items()},
+            "peak_device_bytes": st["peak_device_bytes"], "batches": st["batches"]}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log-n", type=int, default=24)
+    ap.add_argument("--queries", type=int, default=100)
+    ap.add_argument("--log-last", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--cpu-log-n", type=int, default=24)
+    ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--skip", default="", help="comma list of workloads to skip: wide,batch,serve")
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="offered load, jobs/s")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run a device-to-device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(independent proofs shard by job; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import vortexstark_amd as vx
+
+    assert vx.gpu_available(), "no HIP device"
+    skip = {x for x in args.skip.split(",") if x}
+    plain = bench_plain(vx, args)
+    extra = {}
+    if "wide" not in skip:
+        extra["wide"] = bench_wide(vx, args)
+    if "batch" not in skip:
+        extra["batch"] = bench_batch(vx, args)
+    if "serve" not in skip:
+        extra["serve"] = bench_serve(vx, args)
+    log_n, nq, last = args.log_n, args.queries, args.log_last
     out = {
         "metric": METRIC,
-        "value": round(melem_s, 2),
+        "value": round(plain["melem_s"], 2),
         "unit": "Melem/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(plain["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -209,18 +436,18 @@ def main() -> int:
         "config": {"workload": "plain: Fibonacci M31 column 2^%d rows, blowup 4 (2^%d LDE), Blake2s Merkle, "
                                "FRI to 2^%d, %d queries, full proof in HBM" % (log_n, log_n + 2, last, nq),
                    "log_n": log_n, "blowup": 4, "n_queries": nq, "log_last_layer": last, "parallelism": "none"},
-        "proofs_per_s": round(args.steps / elapsed, 3),
-        "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3)},
-        "roofline": roofline,
-        "roofline_hbm": roofline_hbm,
-        "pipeline_valu_bound_ms": round(t_valu_ms, 3),
-        "pipeline_frac_of_valu_bound": round(t_valu_ms / ms_per_step, 4),
-        "phases_ms": {k: round(v, 4) for k, v in phases.items()},
-        "parity_vs_oracle": parity,
-        "cpu_baseline": cpu,
+        "proofs_per_s": round(plain["proofs_per_s"], 3),
+        "latency_ms": plain["latency"],
+        "roofline": plain["roofline"],
+        "roofline_hbm": plain["roofline_hbm"],
+        "pipeline_valu_bound_ms": round(plain["t_valu_ms"], 3),
+        "pipeline_frac_of_valu_bound": round(plain["t_valu_ms"] / plain["ms_per_step"], 4),
+        "phases_ms": {k: round(v, 4) for k, v in plain["phases"].items()},
+        "parity_vs_oracle": plain["parity"],
+        "cpu_baseline": plain["cpu"],
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
-    prover.close()
     return 0
 
 

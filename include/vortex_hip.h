@@ -290,8 +290,11 @@ typedef struct vx_server vx_server;
 /* A server for Fibonacci proof jobs (n_queries / log_last_layer as for the
  * prover; 0 selects the reference protocol). budget_bytes caps the device
  * memory its batched provers hold (0: 90% of the free memory; a larger budget
- * is clipped to that). Not thread-safe: one thread drives a server. */
-vx_server* vx_server_create(uint64_t budget_bytes, uint32_t n_queries, uint32_t log_last_layer);
+ * is clipped to that). Finished proofs wait in a pinned host ring of
+ * result_bytes (0: 1 GiB) until their job is released. Not thread-safe: one
+ * thread drives a server. */
+vx_server* vx_server_create(uint64_t budget_bytes, uint64_t result_bytes, uint32_t n_queries,
+                            uint32_t log_last_layer);
 void vx_server_destroy(vx_server* s);
 /* Queues a job of n_cols columns of 2^log_n rows; column c is the Fibonacci trace
  * with (a, b) = (ab[2c], ab[2c+1]) (host). Returns the job id or -1. */
@@ -299,14 +302,17 @@ int64_t vx_server_submit(vx_server* s, uint32_t log_n, uint32_t n_cols, const ui
 /* One scheduling step: retire finished batches, launch batches on idle provers,
  * add provers within the budget. Returns the number of actions taken. */
 int32_t vx_server_poll(vx_server* s);
+/* Batches replay captured HIP graphs (one per prover and batch count). */
+int32_t vx_server_set_graphs(vx_server* s, int32_t enable);
 int32_t vx_server_wait_all(vx_server* s);
 uint64_t vx_server_jobs(const vx_server* s);
 uint64_t vx_server_completed(const vx_server* s);
-/* 1 done, 0 pending, negative: error (-3 transcript, -4 does not fit the budget). */
+/* 1 done, 0 pending, negative: error (-3 transcript, -4 does not fit the budget,
+ * -5 result ring full while nothing runs: release finished jobs). */
 int32_t vx_server_job_status(const vx_server* s, int64_t job);
 double vx_server_job_latency_ms(const vx_server* s, int64_t job);
 uint64_t vx_server_proof_copy(const vx_server* s, int64_t job, uint32_t col, void* out, uint64_t cap);
-void vx_server_release_job(vx_server* s, int64_t job);  /* frees a finished job's host copy */
+void vx_server_release_job(vx_server* s, int64_t job);  /* frees a finished job's ring space */
 /* Replays arrivals in real time (job i at arrival_s[i] seconds, ascending; its
  * n_cols[i] (a, b) pairs follow each other in ab) and returns when all are done;
  * latency_ms (optional, n_jobs entries) gets arrival-to-completion times. */

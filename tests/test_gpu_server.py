@@ -65,3 +65,35 @@ def test_server_budget_too_small_fails_cleanly(vx):
         srv.wait_all()
     assert srv.status(jid) == -4
     srv.close()
+
+
+def test_server_small_result_ring_with_releases(vx):
+    # room for three proofs: batches wait for finished jobs to be released
+    size = int(vx.lib.vx_krak_proof_size(6, 80, 3))
+    srv = vx.Server(0, 80, 3, result_bytes=3 * size)
+    spec = [(6, [(i + 1, 2 * i + 5)]) for i in range(10)]
+    ids = [srv.submit(l, ab) for l, ab in spec]
+    checked = set()
+    for _ in range(200000):
+        srv.poll()
+        for jid in ids:
+            if jid not in checked and srv.status(jid) == 1:
+                (a, b), = spec[jid][1]
+                assert srv.proof(jid, 0, 6) == O.prove(a, b, 6, 80, 3)
+                srv.release(jid)
+                checked.add(jid)
+        if len(checked) == len(ids):
+            break
+    assert len(checked) == len(ids)
+    srv.close()
+
+
+def test_server_ring_full_without_release_fails_cleanly(vx):
+    size = int(vx.lib.vx_krak_proof_size(6, 80, 3))
+    srv = vx.Server(0, 80, 3, result_bytes=2 * size)
+    ids = [srv.submit(6, [(1, 1)]) for _ in range(5)]
+    with pytest.raises(vx.HipError):
+        srv.wait_all()
+    st = [srv.status(i) for i in ids]
+    assert st.count(1) == 2 and all(x == -5 for x in st if x != 1)
+    srv.close()

@@ -294,8 +294,9 @@ class Server:
     """Proof server (Part 5 of the C ABI): jobs of n Fibonacci columns of 2^log_n
     rows, batched per size class on the device under an HBM budget."""
 
-    def __init__(self, budget_bytes: int = 0, n_queries: int = N_QUERIES, log_last_layer: int = LOG_LAST_LAYER):
-        self.handle = lib.vx_server_create(budget_bytes, n_queries, log_last_layer)
+    def __init__(self, budget_bytes: int = 0, n_queries: int = N_QUERIES, log_last_layer: int = LOG_LAST_LAYER,
+                 result_bytes: int = 0):
+        self.handle = lib.vx_server_create(budget_bytes, result_bytes, n_queries, log_last_layer)
         if not self.handle:
             check("vx_server_create")
             raise HipError("vx_server_create failed")
@@ -312,6 +313,10 @@ class Server:
 
     def poll(self) -> int:
         return lib.vx_server_poll(self.handle)
+
+    def set_graphs(self, on: bool) -> None:
+        if lib.vx_server_set_graphs(self.handle, 1 if on else 0):
+            check("vx_server_set_graphs")
 
     def wait_all(self) -> None:
         if lib.vx_server_wait_all(self.handle):

@@ -120,10 +120,11 @@ SIGNATURES = {
     "vx_wide_timeline_label": (C.c_char_p, [vp, u32]),
     "vx_wide_timeline_ms": (C.c_float, [vp, u32]),
     # Part 5: proof server
-    "vx_server_create": (vp, [u64, u32, u32]),
+    "vx_server_create": (vp, [u64, u64, u32, u32]),
     "vx_server_destroy": (None, [vp]),
     "vx_server_submit": (C.c_int64, [vp, u32, u32, vp]),
     "vx_server_poll": (i32, [vp]),
+    "vx_server_set_graphs": (i32, [vp, i32]),
     "vx_server_wait_all": (i32, [vp]),
     "vx_server_jobs": (u64, [vp]),
     "vx_server_completed": (u64, [vp]),

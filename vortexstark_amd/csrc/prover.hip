@@ -15,6 +15,7 @@
 // The KRAK v1 byte image is assembled in device memory: a template with the
 // static words (magic, counts, path depths) is written once when the prover is
 // created and every proof overwrites all the data words.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -268,11 +269,16 @@ struct vx_prover {
     uint64_t* d_call = nullptr;
     uint64_t* h_call = nullptr;  // pinned staging
     std::vector<uint64_t> last_call;
-    // graph replay of the launch sequence (one graph per batch count)
+    // graph replay of the launch sequence: one instantiated graph per batch
+    // count, the most recent GRAPH_CACHE counts kept
+    static constexpr size_t GRAPH_CACHE = 8;
+    struct Captured {
+        uint32_t count;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
     bool use_graph = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    uint32_t graph_count = 0;
+    std::vector<Captured> graphs;  // most recently used last
     Timeline tl;
     void mark(const std::string& label) { tl.mark(label, stream); }
 
@@ -289,16 +295,17 @@ struct vx_prover {
         allocs.push_back(*p);
         return true;
     }
-    void drop_graph() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
-        graph_count = 0;
+    static void destroy(Captured& c) {
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        if (c.graph) (void)hipGraphDestroy(c.graph);
+    }
+    void drop_graphs() {
+        for (Captured& c : graphs) destroy(c);
+        graphs.clear();
     }
     ~vx_prover() {
         if (stream) (void)hipStreamSynchronize(stream);
-        drop_graph();
+        drop_graphs();
         for (void* p : allocs) (void)hipFree(p);
         if (h_call) (void)hipHostFree(h_call);
         if (stream) (void)hipStreamDestroy(stream);
@@ -455,28 +462,31 @@ int32_t run_batch(vx_prover* p, uint32_t count) {
         enqueue_prove(p, count);
         return peek_error() == 0 ? 0 : -1;
     }
-    if (!p->graph_exec || p->graph_count != count) {
-        p->drop_graph();
+    auto& G = p->graphs;
+    auto it = std::find_if(G.begin(), G.end(), [&](const vx_prover::Captured& c) { return c.count == count; });
+    if (it != G.end()) {
+        std::rotate(it, it + 1, G.end());  // most recently used last
+    } else {
+        if (G.size() == vx_prover::GRAPH_CACHE) {
+            vx_prover::destroy(G.front());
+            G.erase(G.begin());
+        }
         const bool timing = p->tl.on;
         p->tl.on = false;  // no event records inside a captured graph
         if (!check(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
             return -1;
         enqueue_prove(p, count);
-        hipGraph_t g = nullptr;
-        const bool ok = check(hipStreamEndCapture(p->stream, &g), "hipStreamEndCapture");
+        vx_prover::Captured c = {count, nullptr, nullptr};
+        const bool ok = check(hipStreamEndCapture(p->stream, &c.graph), "hipStreamEndCapture");
         p->tl.on = timing;
-        if (!ok || peek_error()) {
-            if (g) (void)hipGraphDestroy(g);
+        if (!ok || peek_error() || !check(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0),
+                                          "hipGraphInstantiate")) {
+            vx_prover::destroy(c);
             return -1;
         }
-        p->graph = g;
-        if (!check(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0), "hipGraphInstantiate")) {
-            p->drop_graph();
-            return -1;
-        }
-        p->graph_count = count;
+        G.push_back(c);
     }
-    return check(hipGraphLaunch(p->graph_exec, p->stream), "hipGraphLaunch") ? 0 : -1;
+    return check(hipGraphLaunch(G.back().exec, p->stream), "hipGraphLaunch") ? 0 : -1;
 }
 
 }  // namespace
@@ -717,7 +727,7 @@ int32_t vx_prover_set_graph(vx_prover* p, int32_t enable) {
     if (!p) return -1;
     if (!check(hipStreamSynchronize(p->stream), "vx_prover_set_graph")) return -1;
     p->use_graph = enable != 0;
-    if (!p->use_graph) p->drop_graph();
+    if (!p->use_graph) p->drop_graphs();
     return 0;
 }
 

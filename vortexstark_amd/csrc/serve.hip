@@ -12,10 +12,12 @@
 //    requests that arrive while a batch runs join the next batch of their
 //    class, and batches of different classes overlap on the device;
 //  * a prover is created when its class has work and its arena fits the HBM
-//    budget; idle provers of classes without work are released (oldest first)
-//    to make room;
-//  * a finished batch is noticed through an event query; its proofs were
-//    already copied to pinned host memory on the prover's stream.
+//    budget; idle provers of classes without work are released (least
+//    recently used first) to make room;
+//  * finished proofs go by one strided DMA copy per batch into a pinned host
+//    ring, where they stay until the job is released; a batch waits while the
+//    ring has no room for it;
+//  * a finished batch is noticed through an event query on its stream.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -31,24 +33,75 @@ namespace {
 
 constexpr uint32_t MAX_PROVERS_PER_CLASS = 2;
 constexpr uint32_t M31_P = 0x7FFFFFFFu;
+constexpr uint64_t DEFAULT_RING_BYTES = 1ull << 30;
 
 double now_s() {
     using namespace std::chrono;
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// proofs per batch: enough LDE rows per launch to fill the chip, at most 128
+// proofs per batch: about 2^24 LDE rows per launch (the stage kernels then run
+// thousands of waves), at least 2 and at most 256
 uint32_t class_capacity(uint32_t log_n) {
-    if (log_n >= 19) return 2;
-    return std::min<uint32_t>(128, std::max<uint32_t>(2, 1u << (19 - log_n)));
+    if (log_n >= 21) return 2;
+    return std::min<uint32_t>(256, std::max<uint32_t>(2, 1u << (22 - log_n)));
 }
+
+// Pinned host memory for finished proofs, allocated in batch-sized regions in
+// ring order; a region is reused once every proof in it has been released.
+struct ResultRing {
+    uint8_t* base = nullptr;
+    uint64_t size = 0;
+    struct Region {
+        uint64_t off, bytes;
+        uint32_t live;
+    };
+    std::deque<Region> regions;  // allocation order
+    uint64_t first_id = 0;       // id of regions.front()
+
+    // returns the region id, or -1 when there is no contiguous room now
+    int64_t alloc(uint64_t bytes, uint32_t live) {
+        if (!bytes || bytes > size) return -1;
+        uint64_t off = 0;
+        if (!regions.empty()) {
+            const uint64_t tail = regions.front().off, head = regions.back().off + regions.back().bytes;
+            if (head > tail) {  // live data in [tail, head)
+                if (head + bytes <= size)
+                    off = head;
+                else if (bytes <= tail)
+                    off = 0;
+                else
+                    return -1;
+            } else {  // wrapped: live data in [tail, size) and [0, head)
+                if (head + bytes <= tail)
+                    off = head;
+                else
+                    return -1;
+            }
+        }
+        regions.push_back({off, bytes, live});
+        return (int64_t)(first_id + regions.size() - 1);
+    }
+    uint8_t* at(int64_t id) const { return base + regions[id - first_id].off; }
+    void release(int64_t id) {
+        if (id < (int64_t)first_id || id >= (int64_t)(first_id + regions.size())) return;
+        Region& r = regions[id - first_id];
+        if (r.live) r.live--;
+        while (!regions.empty() && regions.front().live == 0) {
+            regions.pop_front();
+            first_id++;
+        }
+    }
+};
 
 struct Job {
     uint32_t log_n = 0, n_cols = 0, remaining = 0;
     double arrival = 0, done = -1;
     int32_t error = 0;
-    std::vector<uint8_t> proofs;  // n_cols * proof_size
+    bool released = false;
     uint64_t proof_size = 0;
+    std::vector<int64_t> region;  // per column: ring region holding its proof
+    std::vector<uint8_t*> where;  // per column: address of its proof in the ring
 };
 
 struct Item {
@@ -63,8 +116,8 @@ struct Slot {
     uint64_t bytes = 0;
     uint64_t proof_size = 0;
     hipEvent_t done = nullptr;
-    uint8_t* h_proofs = nullptr;  // pinned: capacity * proof_size
     uint32_t* h_failed = nullptr;  // pinned: capacity words
+    int64_t region = -1;
     std::vector<Item> running;
     double last_used = 0;
     bool busy() const { return !running.empty(); }
@@ -80,6 +133,8 @@ struct Class {
 struct vx_server {
     uint64_t budget = 0, used = 0;
     uint32_t n_q = 80, log_last = 3;
+    bool graphs = false;  // batches replay captured graphs
+    ResultRing ring;
     std::map<uint32_t, Class> classes;
     std::vector<Job> jobs;
     uint64_t completed = 0, batches = 0, proofs_done = 0, peak_used = 0;
@@ -87,6 +142,7 @@ struct vx_server {
     ~vx_server() {
         for (auto& kv : classes)
             for (auto& s : kv.second.slots) release(*s);
+        if (ring.base) (void)hipHostFree(ring.base);
     }
     void release(Slot& s) {
         if (s.prover) {
@@ -95,7 +151,6 @@ struct vx_server {
             used -= s.bytes;
         }
         if (s.done) (void)hipEventDestroy(s.done);
-        if (s.h_proofs) (void)hipHostFree(s.h_proofs);
         if (s.h_failed) (void)hipHostFree(s.h_failed);
         s = Slot();
     }
@@ -105,24 +160,25 @@ struct vx_server {
         if (!p) return false;
         s.prover = p;
         s.capacity = cap;
+        if (graphs) vx_prover_set_graph(p, 1);
         s.bytes = vx_prover_device_bytes(p);
         s.proof_size = vx_prover_proof_size(p);
         used += s.bytes;
         peak_used = std::max(peak_used, used);
         bool ok = vx::check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate") &&
-                  vx::check(hipHostMalloc((void**)&s.h_proofs, s.proof_size * cap), "hipHostMalloc") &&
                   vx::check(hipHostMalloc((void**)&s.h_failed, 4ull * cap), "hipHostMalloc");
         if (!ok) release(s);
         return ok;
     }
-    // estimate before creating: capacity * arena; a created prover reports its real size
+    // device bytes of a prover of this class, estimated before creating it
+    // (arena ~ 27 E + 4 N bytes per proof plus ~16 E of shared twiddles)
     static uint64_t estimate_bytes(uint32_t log_n) {
         const uint64_t E = 1ull << (log_n + 2);
         return (uint64_t)class_capacity(log_n) * (28 * E + (4ull << log_n)) + (16 * E);
     }
     bool reserve_room(uint64_t need, uint32_t for_class) {
         while (used + need > budget) {
-            // release the idle prover that was used least recently, in a class without pending work
+            // release the idle prover used least recently, in a class without pending work
             Slot* victim = nullptr;
             uint32_t victim_class = 0;
             for (auto& kv : classes) {
@@ -141,10 +197,16 @@ struct vx_server {
         }
         return true;
     }
-    void launch(uint32_t log_n, Class& c, Slot& s) {
-        const uint32_t count = (uint32_t)std::min<size_t>(s.capacity, c.pending.size());
+    // Launches up to capacity pending proofs of the class on an idle prover.
+    // Returns false when the result ring has no room for even one proof.
+    bool launch(Class& c, Slot& s) {
+        uint32_t count = (uint32_t)std::min<size_t>(s.capacity, c.pending.size());
+        int64_t region = -1;
+        while (count && (region = ring.alloc(count * s.proof_size, count)) < 0) count /= 2;
+        if (region < 0) return false;
         std::vector<uint32_t> ab(2 * count);
         s.running.clear();
+        s.region = region;
         for (uint32_t i = 0; i < count; i++) {
             s.running.push_back(c.pending.front());
             c.pending.pop_front();
@@ -153,37 +215,41 @@ struct vx_server {
         }
         hipStream_t st = (hipStream_t)vx_prover_stream(s.prover);
         bool ok = vx_prove_fibonacci_batch(s.prover, ab.data(), count) == 0;
-        // proofs and transcript flags to pinned memory on the same stream
-        ok = ok && vx_prover_enqueue_copy(s.prover, count, s.h_proofs, s.h_failed, st) == 0;
+        // proofs (into the ring) and transcript flags, on the same stream
+        ok = ok && vx_prover_enqueue_copy(s.prover, count, ring.at(region), s.h_failed, st) == 0;
         ok = ok && vx::check(hipEventRecord(s.done, st), "hipEventRecord");
         s.last_used = now_s();
         batches++;
         if (!ok) fail_running(s, -1);
-        (void)log_n;
+        return true;
     }
     void fail_running(Slot& s, int32_t code) {
         for (const Item& it : s.running) {
             Job& j = jobs[it.job];
             j.error = code;
+            ring.release(s.region);
             if (--j.remaining == 0) finish(j);
         }
         s.running.clear();
+        s.region = -1;
     }
     void finish(Job& j) {
         j.done = now_s();
         completed++;
     }
     void retire(Slot& s) {
+        uint8_t* base = ring.at(s.region);
         for (uint32_t i = 0; i < s.running.size(); i++) {
             const Item& it = s.running[i];
             Job& j = jobs[it.job];
-            std::memcpy(j.proofs.data() + (uint64_t)it.col * j.proof_size, s.h_proofs + (uint64_t)i * s.proof_size,
-                        s.proof_size);
+            j.region[it.col] = s.region;
+            j.where[it.col] = base + (uint64_t)i * s.proof_size;
             if (s.h_failed[i]) j.error = -3;
             proofs_done++;
             if (--j.remaining == 0) finish(j);
         }
         s.running.clear();
+        s.region = -1;
         s.last_used = now_s();
     }
     bool any_busy() const {
@@ -191,6 +257,14 @@ struct vx_server {
             for (auto& s : kv.second.slots)
                 if (s->busy()) return true;
         return false;
+    }
+    void fail_pending(Class& c, int32_t code) {
+        for (const Item& it : c.pending) {
+            Job& j = jobs[it.job];
+            j.error = code;
+            if (--j.remaining == 0) finish(j);
+        }
+        c.pending.clear();
     }
     // one scheduler step; returns the number of batches launched or retired
     int32_t poll() {
@@ -209,38 +283,43 @@ struct vx_server {
                 }
             }
         }
+        bool ring_full = false;
         for (auto& kv : classes) {
             Class& c = kv.second;
             if (c.pending.empty()) continue;
             for (auto& s : c.slots)
                 if (!s->busy() && !c.pending.empty()) {
-                    launch(kv.first, c, *s);
-                    acted++;
+                    if (launch(c, *s))
+                        acted++;
+                    else
+                        ring_full = true;
                 }
             // more proofs waiting than the idle provers take: add a prover if the budget allows
-            if (!c.pending.empty() && c.slots.size() < MAX_PROVERS_PER_CLASS) {
+            if (!c.pending.empty() && !ring_full && c.slots.size() < MAX_PROVERS_PER_CLASS) {
                 const uint64_t need = estimate_bytes(kv.first);
                 if (reserve_room(need, kv.first)) {
                     auto s = std::make_unique<Slot>();
                     if (make_slot(kv.first, *s)) {
                         c.slots.push_back(std::move(s));
-                        launch(kv.first, c, *c.slots.back());
-                        acted++;
+                        if (launch(c, *c.slots.back()))
+                            acted++;
+                        else
+                            ring_full = true;
                     } else {
                         (void)vx_get_last_error();  // out of device memory: retry when something frees up
                     }
                 } else if (c.slots.empty() && !any_busy()) {
                     // nothing will free memory: this class can never run under the budget
                     vx::set_error(-4, "vx_server: a proof of this size does not fit the HBM budget");
-                    for (const Item& it : c.pending) {
-                        Job& j = jobs[it.job];
-                        j.error = -4;
-                        if (--j.remaining == 0) finish(j);
-                    }
-                    c.pending.clear();
+                    fail_pending(c, -4);
                     acted++;
                 }
             }
+        }
+        if (ring_full && !acted && !any_busy()) {
+            // nothing runs and no finished job has been released: waiting cannot help
+            vx::set_error(-5, "vx_server: result ring full; release finished jobs");
+            for (auto& kv : classes) fail_pending(kv.second, -5);
         }
         return acted;
     }
@@ -248,7 +327,8 @@ struct vx_server {
 
 extern "C" {
 
-vx_server* vx_server_create(uint64_t budget_bytes, uint32_t n_queries, uint32_t log_last_layer) {
+vx_server* vx_server_create(uint64_t budget_bytes, uint64_t result_bytes, uint32_t n_queries,
+                            uint32_t log_last_layer) {
     size_t free_b = 0, total_b = 0;
     if (!vx::check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo")) return nullptr;
     vx_server* s = new vx_server();
@@ -256,6 +336,11 @@ vx_server* vx_server_create(uint64_t budget_bytes, uint32_t n_queries, uint32_t 
     s->budget = budget_bytes ? std::min<uint64_t>(budget_bytes, avail) : avail;
     s->n_q = n_queries ? n_queries : 80;
     s->log_last = log_last_layer ? log_last_layer : 3;
+    s->ring.size = result_bytes ? result_bytes : DEFAULT_RING_BYTES;
+    if (!vx::check(hipHostMalloc((void**)&s->ring.base, s->ring.size), "vx_server: pinned result ring")) {
+        delete s;
+        return nullptr;
+    }
     return s;
 }
 
@@ -272,20 +357,25 @@ int64_t vx_server_submit(vx_server* s, uint32_t log_n, uint32_t n_cols, const ui
     j.n_cols = n_cols;
     j.remaining = n_cols;
     j.arrival = now_s();
-    // proof size of this class: the KRAK layout depends only on (log_n, n_q, log_last)
-    Class& c = s->classes[log_n];
-    uint64_t ps = 0;
-    for (auto& sl : c.slots)
-        if (sl->prover) ps = sl->proof_size;
-    if (!ps) ps = vx_krak_proof_size(log_n, s->n_q, s->log_last);
-    j.proof_size = ps;
-    j.proofs.assign(ps * n_cols, 0);
+    j.proof_size = vx_krak_proof_size(log_n, s->n_q, s->log_last);
+    j.region.assign(n_cols, -1);
+    j.where.assign(n_cols, nullptr);
     s->jobs.push_back(std::move(j));
+    Class& c = s->classes[log_n];
     for (uint32_t k = 0; k < n_cols; k++) c.pending.push_back(Item{id, k, ab[2 * k] % M31_P, ab[2 * k + 1] % M31_P});
     return id;
 }
 
 int32_t vx_server_poll(vx_server* s) { return s ? s->poll() : -1; }
+
+int32_t vx_server_set_graphs(vx_server* s, int32_t enable) {
+    if (!s) return -1;
+    s->graphs = enable != 0;
+    for (auto& kv : s->classes)
+        for (auto& sl : kv.second.slots)
+            if (sl->prover && vx_prover_set_graph(sl->prover, enable)) return -1;
+    return 0;
+}
 
 uint64_t vx_server_completed(const vx_server* s) { return s ? s->completed : 0; }
 uint64_t vx_server_jobs(const vx_server* s) { return s ? s->jobs.size() : 0; }
@@ -300,8 +390,13 @@ int32_t vx_server_wait_all(vx_server* s) {
 }
 
 void vx_server_release_job(vx_server* s, int64_t job) {
-    if (!s || job < 0 || job >= (int64_t)s->jobs.size() || s->jobs[job].done < 0) return;
-    std::vector<uint8_t>().swap(s->jobs[job].proofs);
+    if (!s || job < 0 || job >= (int64_t)s->jobs.size()) return;
+    Job& j = s->jobs[job];
+    if (j.done < 0 || j.released) return;
+    j.released = true;
+    for (int64_t r : j.region)
+        if (r >= 0) s->ring.release(r);
+    j.where.assign(j.n_cols, nullptr);
 }
 
 int32_t vx_server_job_status(const vx_server* s, int64_t job) {
@@ -319,15 +414,12 @@ double vx_server_job_latency_ms(const vx_server* s, int64_t job) {
 uint64_t vx_server_proof_copy(const vx_server* s, int64_t job, uint32_t col, void* out, uint64_t cap) {
     if (!s || job < 0 || job >= (int64_t)s->jobs.size()) return 0;
     const Job& j = s->jobs[job];
-    if (j.done < 0 || j.error || col >= j.n_cols || cap < j.proof_size || !out) return 0;
-    std::memcpy(out, j.proofs.data() + (uint64_t)col * j.proof_size, j.proof_size);
+    if (j.done < 0 || j.error || j.released || col >= j.n_cols || cap < j.proof_size || !out || !j.where[col])
+        return 0;
+    std::memcpy(out, j.where[col], j.proof_size);
     return j.proof_size;
 }
 
-// Replays an arrival trace in real time: job i arrives arrival_s[i] seconds after
-// the call starts (ascending), with n_cols[i] columns whose (a, b) pairs follow
-// each other in ab. Returns when every job is done; latency_ms[i] (optional)
-// gets arrival-to-completion times.
 int32_t vx_server_run_trace(vx_server* s, uint32_t n_jobs, const double* arrival_s, const uint32_t* log_n,
                             const uint32_t* n_cols, const uint32_t* ab, double* latency_ms) {
     if (!s || (n_jobs && (!arrival_s || !log_n || !n_cols || !ab))) {
@@ -354,8 +446,6 @@ int32_t vx_server_run_trace(vx_server* s, uint32_t n_jobs, const double* arrival
     return 0;
 }
 
-// Counters: [0] batches launched, [1] proofs done, [2] jobs done, [3] HBM bytes
-// held by provers now, [4] peak, [5] budget.
 void vx_server_stats(const vx_server* s, uint64_t out[6]) {
     if (!s || !out) return;
     out[0] = s->batches;
