@@ -395,7 +395,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--cpu-log-n", type=int, default=24)
     ap.add_argument("--cpu-reps", type=int, default=2)
-    ap.add_argument("--skip", default="", help="comma list of workloads to skip: wide,batch,serve")
+    ap.add_argument("--skip", default="", help="comma list of workloads to skip: plain,wide,batch,serve "
+                                               "(skipping plain leaves value null: for profiling runs)")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="offered load, jobs/s")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -411,8 +412,14 @@ def main() -> int:
 
     assert vx.gpu_available(), "no HIP device"
     skip = {x for x in args.skip.split(",") if x}
-    plain = bench_plain(vx, args)
     extra = {}
+    if "plain" in skip:  # profiling runs of the other workloads only
+        for name, fn in (("wide", bench_wide), ("batch", bench_batch), ("serve", bench_serve)):
+            if name not in skip:
+                extra[name] = fn(vx, args)
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "Melem/s", "n_gpus": 1, **extra}), flush=True)
+        return 0
+    plain = bench_plain(vx, args)
     if "wide" not in skip:
         extra["wide"] = bench_wide(vx, args)
     if "batch" not in skip:
