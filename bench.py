@@ -44,6 +44,20 @@ VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 OPS_PER_COMPRESSION = 10 * 8 * 12 + 16
 
 
+def load_traffic() -> dict:
+    """HBM bytes per launch of the rooflined kernels, measured offline with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy-kernel calibration);
+    see the summary the file names."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_per_launch.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get("kernels", {})
+
+
+TRAFFIC = load_traffic()
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -70,12 +84,18 @@ def stage_compressions(n_items: int, leaf_blocks: int) -> int:
     return n_items * leaf_blocks + (n_items - waves)
 
 
+MEASURED = {}  # filled at start-up: measured VALU peak, NTT plans
+
+
 def valu_roofline(kernel: str, compressions: int, ms: float) -> dict:
     ops = compressions * OPS_PER_COMPRESSION
     achieved = ops / (ms * 1e-3) / 1e12
-    return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-            "unit": "Tops/s (int32 lane-ops)", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-            "avg_ms": round(ms, 4), "ops_per_launch": ops}
+    out = {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
+           "unit": "Tops/s (int32 lane-ops)", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+           "traffic": TRAFFIC.get(kernel), "avg_ms": round(ms, 4), "ops_per_launch": ops}
+    if "valu_peak" in MEASURED:
+        out["peak_measured"] = round(MEASURED["valu_peak"], 3)
+    return out
 
 
 def cpu_model() -> str:
@@ -166,9 +186,10 @@ def bench_plain(vx, args) -> dict:
     if ntt_ms:
         byts = 4 * (1 << log_n) + 4 * E  # read coefficients once, write the LDE once
         achieved = byts / (ntt_ms * 1e-3) / 1e9
-        roofline_hbm = {"bound": "hbm", "kernel": "ntt_pass_kernel<true> x3 (evaluate 2^%d)" % (log_n + 2),
+        roofline_hbm = {"bound": "hbm", "kernel": "ntt_pass_kernel<true> (evaluate 2^%d, all passes)" % (log_n + 2),
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "phase_ms": round(ntt_ms, 4)}
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get("lde:evaluate"),
+                        "phase_ms": round(ntt_ms, 4), "passes": MEASURED.get("ntt_plans", {}).get(str(log_n + 2))}
     # pipeline-level bound: 6E Blake2s compressions at the VALU peak
     t_valu_ms = 6 * E * OPS_PER_COMPRESSION / (VALU_PEAK_TOPS * 1e12) * 1e3
 
@@ -399,6 +420,7 @@ def main() -> int:
                                                "(skipping plain leaves value null: for profiling runs)")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="offered load, jobs/s")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default NTT pass split")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -412,6 +434,13 @@ def main() -> int:
 
     assert vx.gpu_available(), "no HIP device"
     skip = {x for x in args.skip.split(",") if x}
+    MEASURED["valu_peak"] = vx.measure_valu_peak()
+    log(f"measured int32 VALU peak: {MEASURED['valu_peak']:.2f} T lane-ops/s (nominal {VALU_PEAK_TOPS:.2f})")
+    if not args.no_tune:
+        t = time.time()
+        MEASURED["ntt_plans"] = {str(k): {"ms": round(ms, 4), "passes": plan}
+                                 for k, (ms, plan) in vx.ntt_autotune(10, 28).items()}
+        log(f"ntt autotune 2^10..2^28: {time.time() - t:.1f}s")
     extra = {}
     if "plain" in skip:  # profiling runs of the other workloads only
         for name, fn in (("wide", bench_wide), ("batch", bench_batch), ("serve", bench_serve)):
@@ -452,6 +481,8 @@ def main() -> int:
         "phases_ms": {k: round(v, 4) for k, v in plain["phases"].items()},
         "parity_vs_oracle": plain["parity"],
         "cpu_baseline": plain["cpu"],
+        "valu_peak_measured": round(MEASURED["valu_peak"], 3),
+        "ntt_plans": MEASURED.get("ntt_plans"),
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

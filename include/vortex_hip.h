@@ -174,6 +174,21 @@ int32_t vx_stream_destroy(void* stream);
 void vx_set_stream(void* stream);                /* per-thread stream for Part 1 launches */
 void* vx_get_stream(void);
 
+/* Pass plans of the circle NTT (native and stwo basis). A transform of 2^log_n
+ * points runs as passes over LDS tiles: a contiguous pass of m[0] layers, then
+ * strided passes of m[i] layers moving 2^c[i] contiguous words per row
+ * (c in {4, 5}). vx_ntt_autotune times every candidate split of each size on
+ * the device and installs the fastest for this process; plan_set installs a
+ * given one (n_passes = 0: the default split); plan_get reports the one in use
+ * (returns the number of passes). Any plan gives bit-identical results. */
+int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms);
+/* Measured int32 VALU issue rate of the current device in T lane-ops/s (a
+ * grid of v_add3 / v_xor / v_alignbit chains, the instructions of Blake2s). */
+int32_t vx_measure_valu_peak(double* tops, double* ms);
+int32_t vx_ntt_plan_set(uint32_t log_n, uint32_t n_passes, const uint32_t* m, const uint32_t* c);
+uint32_t vx_ntt_plan_get(uint32_t log_n, uint32_t* m, uint32_t* c);
+void vx_ntt_plan_reset(void);
+
 /* ===================== Part 3: fused device-resident prover ================= */
 
 typedef struct vx_prover vx_prover;

@@ -369,3 +369,37 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def ntt_autotune(log_min: int, log_max: int) -> dict:
+    """Times every candidate pass split of the NTT for sizes 2^log_min .. 2^log_max
+    and installs the fastest; returns {log_n: (ms, [(layers, segment_log), ...])}."""
+    best = np.zeros(log_max - log_min + 1, np.float32)
+    if lib.vx_ntt_autotune(log_min, log_max, best.ctypes.data):
+        check("vx_ntt_autotune")
+        raise HipError("vx_ntt_autotune failed")
+    return {k: (float(best[k - log_min]), ntt_plan(k)) for k in range(log_min, log_max + 1)}
+
+
+def ntt_plan(log_n: int) -> list:
+    m, c = np.zeros(8, np.uint32), np.zeros(8, np.uint32)
+    n = lib.vx_ntt_plan_get(log_n, m.ctypes.data, c.ctypes.data)
+    return [(int(m[i]), int(c[i])) for i in range(n)]
+
+
+def ntt_plan_set(log_n: int, passes) -> None:
+    """passes: [(layers, segment_log), ...] with segment_log 0 for the first pass; [] = default."""
+    m = np.array([p[0] for p in passes] or [0], np.uint32)
+    c = np.array([p[1] for p in passes] or [0], np.uint32)
+    if lib.vx_ntt_plan_set(log_n, len(passes), m.ctypes.data, c.ctypes.data):
+        check("vx_ntt_plan_set")
+        raise HipError("vx_ntt_plan_set failed")
+
+
+def measure_valu_peak() -> float:
+    """int32 VALU issue rate of the device, T lane-ops/s (microbenchmark)."""
+    tops = C.c_double(0.0)
+    if lib.vx_measure_valu_peak(C.byref(tops), None):
+        check("vx_measure_valu_peak")
+        raise HipError("vx_measure_valu_peak failed")
+    return tops.value

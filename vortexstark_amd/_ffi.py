@@ -77,6 +77,11 @@ SIGNATURES = {
     "vx_stream_synchronize": (i32, [vp]),
     "vx_stream_destroy": (i32, [vp]),
     "vx_set_stream": (None, [vp]),
+    "vx_ntt_autotune": (i32, [u32, u32, vp]),
+    "vx_measure_valu_peak": (i32, [vp, vp]),
+    "vx_ntt_plan_set": (i32, [u32, u32, vp, vp]),
+    "vx_ntt_plan_get": (u32, [u32, vp, vp]),
+    "vx_ntt_plan_reset": (None, []),
     "vx_get_stream": (vp, []),
     # Part 3: fused prover
     "vx_prover_create": (vp, [u32, u32, u32]),

@@ -45,8 +45,12 @@ struct CM31 {
 };
 __device__ __forceinline__ CM31 cadd(CM31 x, CM31 y) { return {madd(x.a, y.a), madd(x.b, y.b)}; }
 __device__ __forceinline__ CM31 csub(CM31 x, CM31 y) { return {msub(x.a, y.a), msub(x.b, y.b)}; }
+// (a + bi)(c + di) with three products: the imaginary part is
+// (a + b)(c + d) - ac - bd (Karatsuba; equal to ad + bc mod p).
 __device__ __forceinline__ CM31 cmul(CM31 x, CM31 y) {
-    return {msub(mmul(x.a, y.a), mmul(x.b, y.b)), madd(mmul(x.a, y.b), mmul(x.b, y.a))};
+    const uint32_t ac = mmul(x.a, y.a), bd = mmul(x.b, y.b);
+    const uint32_t s = mmul(madd(x.a, x.b), madd(y.a, y.b));
+    return {msub(ac, bd), msub(msub(s, ac), bd)};
 }
 
 // QM31 = a + b*u with u^2 = 2 + i; stored as {a.a, a.b, b.a, b.b}.
@@ -62,13 +66,16 @@ __device__ __forceinline__ QM31 qsub(QM31 x, QM31 y) {
 __device__ __forceinline__ QM31 qmul_m31(QM31 x, uint32_t s) {
     return {{mmul(x.v[0], s), mmul(x.v[1], s), mmul(x.v[2], s), mmul(x.v[3], s)}};
 }
+// (A + Bu)(C + Du) = (AC + (2 + i) BD) + ((A + B)(C + D) - AC - BD) u: three
+// CM31 products (nine M31 products) instead of four (sixteen).
 __device__ __forceinline__ QM31 qmul(QM31 x, QM31 y) {
-    CM31 xa = {x.v[0], x.v[1]}, xb = {x.v[2], x.v[3]};
-    CM31 ya = {y.v[0], y.v[1]}, yb = {y.v[2], y.v[3]};
-    CM31 ac = cmul(xa, ya), bd = cmul(xb, yb), ad = cmul(xa, yb), bc = cmul(xb, ya);
+    const CM31 xa = {x.v[0], x.v[1]}, xb = {x.v[2], x.v[3]};
+    const CM31 ya = {y.v[0], y.v[1]}, yb = {y.v[2], y.v[3]};
+    const CM31 ac = cmul(xa, ya), bd = cmul(xb, yb);
+    const CM31 s = cmul(cadd(xa, xb), cadd(ya, yb));
     // bd * (2 + i) = (2p - q) + (p + 2q) i
-    CM31 bdu = {msub(mdbl(bd.a), bd.b), madd(bd.a, mdbl(bd.b))};
-    CM31 ra = cadd(ac, bdu), rb = cadd(ad, bc);
+    const CM31 bdu = {msub(mdbl(bd.a), bd.b), madd(bd.a, mdbl(bd.b))};
+    const CM31 ra = cadd(ac, bdu), rb = csub(csub(s, ac), bd);
     return {{ra.a, ra.b, rb.a, rb.b}};
 }
 

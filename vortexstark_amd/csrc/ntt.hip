@@ -17,6 +17,9 @@
 // layer b >= 1 reads the flat stwo buffer at n/2 - n/2^b, and the circle layer
 // derives its twiddle for pair t from the first line layer as
 // [y, -y, -x, x][t % 4] with (x, y) = (T[2(t/4)], T[2(t/4) + 1]).
+#include <algorithm>
+#include <mutex>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -196,14 +199,27 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
     }
 }
 
-constexpr uint32_t NTT_LOW = 12;   // layers of the contiguous (lowest-stride) pass
-constexpr uint32_t NTT_HIGH = 7;   // max layers of a strided pass
-constexpr uint32_t NTT_COLS = 5;   // strided passes move 2^5 contiguous words per row
+// Pass shapes. The contiguous pass (c = 0) holds up to 2^14 words in LDS;
+// strided passes move 2^c contiguous words per row (c = 4: 64-byte, c = 5:
+// 128-byte segments) of up to 2^14-word tiles. Which split a size uses comes
+// from the plan table: the default below, or what the autotuner measured.
+constexpr uint32_t NTT_LOW_MAX = 14;   // most layers of the contiguous pass
+constexpr uint32_t NTT_HIGH_MAX5 = 9;  // most layers of a strided pass, c = 5
+constexpr uint32_t NTT_HIGH_MAX4 = 10; // most layers of a strided pass, c = 4
+constexpr uint32_t NTT_LOW = 12;       // default split: contiguous pass
+constexpr uint32_t NTT_HIGH = 7;       // default split: most layers per strided pass
+constexpr uint32_t NTT_COLS = 5;       // default split: strided row segment
 
 struct Pass {
     uint32_t s, m, c;
 };
-static std::vector<Pass> plan_passes(uint32_t k) {
+
+namespace {
+std::mutex g_plan_mu;
+std::vector<Pass> g_plan[32];  // tuned or set plans; empty: default split
+}  // namespace
+
+static std::vector<Pass> default_plan(uint32_t k) {
     std::vector<Pass> ps;
     if (k == 0) return ps;
     uint32_t m0 = k < NTT_LOW ? k : NTT_LOW;
@@ -218,6 +234,32 @@ static std::vector<Pass> plan_passes(uint32_t k) {
         }
     }
     return ps;
+}
+
+// A plan is valid when its passes cover layers 0 .. k-1 in order, the first is
+// contiguous, every strided pass starts at a stride of at least its segment
+// and every shape has a compiled kernel.
+static bool plan_valid(uint32_t k, const std::vector<Pass>& ps) {
+    if (k == 0) return ps.empty();
+    if (ps.empty() || ps.size() > 8 || ps[0].c != 0 || ps[0].s != 0) return false;
+    uint32_t s = 0;
+    for (const Pass& p : ps) {
+        if (p.s != s || p.m == 0) return false;
+        if (p.c == 0 && (p.s != 0 || p.m > NTT_LOW_MAX)) return false;
+        if (p.c == 5 && (p.m > NTT_HIGH_MAX5 || p.s < 5)) return false;
+        if (p.c == 4 && (p.m > NTT_HIGH_MAX4 || p.s < 4)) return false;
+        if (p.c != 0 && p.c != 4 && p.c != 5) return false;
+        s += p.m;
+    }
+    return s == k;
+}
+
+static std::vector<Pass> plan_passes(uint32_t k) {
+    {
+        std::lock_guard<std::mutex> g(g_plan_mu);
+        if (k < 32 && !g_plan[k].empty()) return g_plan[k];
+    }
+    return default_plan(k);
 }
 
 template <bool FWD, int M, int C, bool STWO>
@@ -243,28 +285,33 @@ static void dispatch_m(uint32_t m, const NttPassArgs& a, uint32_t k, uint32_t n_
 
 // Only the contiguous pass (c == 0) contains the circle layer, so only it has a
 // stwo variant; strided passes differ by their layer offsets alone.
+template <bool FWD>
+static void launch_pass_dir(bool stwo, const NttPassArgs& a, uint32_t m, uint32_t c, uint32_t k, uint32_t n_cols,
+                            hipStream_t st) {
+    if (c == 0) {
+        if (stwo)
+            dispatch_m<FWD, 0, NTT_LOW_MAX, true>(m, a, k, n_cols, st);
+        else
+            dispatch_m<FWD, 0, NTT_LOW_MAX, false>(m, a, k, n_cols, st);
+    } else if (c == 4) {
+        dispatch_m<FWD, 4, NTT_HIGH_MAX4, false>(m, a, k, n_cols, st);
+    } else {
+        dispatch_m<FWD, 5, NTT_HIGH_MAX5, false>(m, a, k, n_cols, st);
+    }
+}
 static void launch_pass(bool fwd, bool stwo, const NttPassArgs& a, uint32_t m, uint32_t c, uint32_t k,
                         uint32_t n_cols, hipStream_t st) {
-    if (c == 0) {
-        if (fwd)
-            stwo ? dispatch_m<true, 0, NTT_LOW, true>(m, a, k, n_cols, st)
-                 : dispatch_m<true, 0, NTT_LOW, false>(m, a, k, n_cols, st);
-        else
-            stwo ? dispatch_m<false, 0, NTT_LOW, true>(m, a, k, n_cols, st)
-                 : dispatch_m<false, 0, NTT_LOW, false>(m, a, k, n_cols, st);
-    } else {
-        if (fwd)
-            dispatch_m<true, NTT_COLS, NTT_HIGH, false>(m, a, k, n_cols, st);
-        else
-            dispatch_m<false, NTT_COLS, NTT_HIGH, false>(m, a, k, n_cols, st);
-    }
+    if (fwd)
+        launch_pass_dir<true>(stwo, a, m, c, k, n_cols, st);
+    else
+        launch_pass_dir<false>(stwo, a, m, c, k, n_cols, st);
     check_launch("ntt_pass_kernel");
 }
 
-static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n, uint32_t* const* cols,
-                    const uint32_t* const* srcs, uint32_t n_cols, const NttTables& t, hipStream_t st) {
+static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n, uint32_t* const* cols,
+                         const uint32_t* const* srcs, uint32_t n_cols, const NttTables& t, hipStream_t st,
+                         const std::vector<Pass>& ps) {
     uint32_t k = t.log_n;
-    auto ps = plan_passes(k);
     uint32_t ex = (30u * k) % 31u;
     uint32_t inv_n = ex == 0 ? 1u : (1u << ex);
     for (size_t i = 0; i < ps.size(); i++) {
@@ -286,6 +333,10 @@ static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n
     if (!fwd && ps.empty() && src != dst && src_n) {
         check(hipMemcpyAsync(dst, src, 4, hipMemcpyDeviceToDevice, st), "ntt copy");
     }
+}
+static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n, uint32_t* const* cols,
+                    const uint32_t* const* srcs, uint32_t n_cols, const NttTables& t, hipStream_t st) {
+    run_ntt_plan(fwd, dst, src, src_n, cols, srcs, n_cols, t, st, plan_passes(t.log_n));
 }
 
 void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTables& t, hipStream_t s) {
@@ -480,6 +531,46 @@ __global__ void bit_reverse_kernel(uint32_t* d, uint32_t log_n, uint32_t n) {
     }
 }
 
+// Candidate splits of a size-2^k transform for the autotuner: every contiguous
+// pass depth that leaves at most three strided passes, with 64- or 128-byte
+// row segments and the fewest (or one more) balanced strided passes.
+static std::vector<std::vector<Pass>> plan_candidates(uint32_t k) {
+    std::vector<std::vector<Pass>> out;
+    auto add = [&](const std::vector<Pass>& ps) {
+        if (!plan_valid(k, ps)) return;
+        for (const auto& q : out)
+            if (q.size() == ps.size() &&
+                std::equal(q.begin(), q.end(), ps.begin(),
+                           [](const Pass& x, const Pass& y) { return x.m == y.m && x.c == y.c; }))
+                return;
+        out.push_back(ps);
+    };
+    const uint32_t lo = std::max<uint32_t>(k >= 16 ? 8 : 1, k > 3 * NTT_HIGH_MAX4 ? k - 3 * NTT_HIGH_MAX4 : 1);
+    for (uint32_t m0 = lo; m0 <= std::min(k, NTT_LOW_MAX); m0++) {
+        const uint32_t R = k - m0;
+        if (R == 0) {
+            add({{0, m0, 0}});
+            continue;
+        }
+        for (uint32_t c : {4u, 5u}) {
+            const uint32_t hmax = c == 4 ? NTT_HIGH_MAX4 : NTT_HIGH_MAX5;
+            const uint32_t npmin = (R + hmax - 1) / hmax;
+            for (uint32_t np = npmin; np <= npmin + 1 && np <= 7 && np <= R; np++) {
+                std::vector<Pass> ps = {{0, m0, 0}};
+                uint32_t s0 = m0;
+                for (uint32_t i = 0; i < np; i++) {
+                    const uint32_t mi = R / np + (i < R % np ? 1 : 0);
+                    ps.push_back({s0, mi, c});
+                    s0 += mi;
+                }
+                add(ps);
+            }
+        }
+    }
+    add(default_plan(k));
+    return out;
+}
+
 static bool tables_from_abi(const uint32_t* tw, const uint32_t* circ, const uint32_t* h_offs, uint32_t n_line,
                             uint32_t n, NttTables& t) {
     if (!is_pow2(n)) {
@@ -549,6 +640,96 @@ void cuda_circle_ntt_layer(uint32_t* d_data, const uint32_t* d_twiddles, uint32_
                                                                          forward);
     check_launch("cuda_circle_ntt_layer");
 }
+int32_t vx_ntt_plan_set(uint32_t log_n, uint32_t n_passes, const uint32_t* m, const uint32_t* c) {
+    if (log_n >= 32 || n_passes > 8 || (n_passes && (!m || !c))) {
+        set_error(-2, "vx_ntt_plan_set: bad arguments");
+        return -1;
+    }
+    std::vector<Pass> ps;
+    uint32_t s0 = 0;
+    for (uint32_t i = 0; i < n_passes; i++) {
+        ps.push_back({s0, m[i], c[i]});
+        s0 += m[i];
+    }
+    if (n_passes && !plan_valid(log_n, ps)) {
+        set_error(-2, "vx_ntt_plan_set: passes must cover every layer with supported shapes");
+        return -1;
+    }
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    g_plan[log_n] = ps;  // empty: back to the default split
+    return 0;
+}
+
+uint32_t vx_ntt_plan_get(uint32_t log_n, uint32_t* m, uint32_t* c) {
+    if (log_n >= 32) return 0;
+    const auto ps = plan_passes(log_n);
+    for (size_t i = 0; i < ps.size() && i < 8; i++) {
+        if (m) m[i] = ps[i].m;
+        if (c) c[i] = ps[i].c;
+    }
+    return (uint32_t)ps.size();
+}
+
+void vx_ntt_plan_reset(void) {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    for (auto& p : g_plan) p.clear();
+}
+
+int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms) {
+    if (log_min < 2 || log_max < log_min || log_max > 28) {
+        set_error(-2, "vx_ntt_autotune: sizes must satisfy 2 <= log_min <= log_max <= 28");
+        return -1;
+    }
+    const uint64_t N = 1ull << log_max;
+    hipStream_t st = nullptr;
+    uint32_t *data = nullptr, *line = nullptr, *circle = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate") &&
+              check(hipMalloc((void**)&data, 4 * N), "hipMalloc") &&
+              check(hipMalloc((void**)&line, 4 * N), "hipMalloc") &&
+              check(hipMalloc((void**)&circle, 4 * N / 2), "hipMalloc") && check(hipEventCreate(&e0), "hipEvent") &&
+              check(hipEventCreate(&e1), "hipEvent");
+    for (uint32_t k = log_min; ok && k <= log_max; k++) {
+        HostPt hi, hs;
+        host_coset(1, k, hi, hs);
+        NttTables t;
+        t.line = line;
+        t.circle = circle;
+        t.log_n = k;
+        build_ntt_tables(CPt{hi.x, hi.y}, CPt{hs.x, hs.y}, k, line, circle, nullptr, nullptr, t.offs, st);
+        ok = check(hipMemsetAsync(data, 0x11, 4ull << k, st), "hipMemsetAsync");  // canonical M31 words
+        const auto cands = plan_candidates(k);
+        float best = 1e30f;
+        size_t bi = 0;
+        for (size_t ci = 0; ok && ci < cands.size(); ci++) {
+            run_ntt_plan(true, data, data, 1ull << k, nullptr, nullptr, 1, t, st, cands[ci]);  // warm-up
+            const int reps = 3;
+            ok = check(hipEventRecord(e0, st), "hipEventRecord");
+            for (int r = 0; r < reps; r++) run_ntt_plan(true, data, data, 1ull << k, nullptr, nullptr, 1, t, st, cands[ci]);
+            ok = ok && check(hipEventRecord(e1, st), "hipEventRecord") && check(hipEventSynchronize(e1), "sync");
+            float ms = 0.f;
+            ok = ok && check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime") && peek_error() == 0;
+            if (ok && ms / reps < best) {
+                best = ms / reps;
+                bi = ci;
+            }
+        }
+        if (ok) {
+            std::lock_guard<std::mutex> g(g_plan_mu);
+            g_plan[k] = cands[bi];
+        }
+        if (best_ms) best_ms[k - log_min] = best;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamSynchronize(st);
+    if (data) (void)hipFree(data);
+    if (line) (void)hipFree(line);
+    if (circle) (void)hipFree(circle);
+    if (st) (void)hipStreamDestroy(st);
+    return ok ? 0 : -1;
+}
+
 void cuda_bit_reverse_m31(uint32_t* data, uint32_t log_n) {
     uint32_t n = 1u << log_n;
     bit_reverse_kernel<<<ceil_div(n, 256), 256, 0, current_stream()>>>(data, log_n, n);
