@@ -33,10 +33,11 @@ void ntt_inverse(uint32_t* dst, const uint32_t* src, const NttTables& t, hipStre
 // Batched forms over a device array of column pointers (one launch per pass for
 // all columns). Forward works in place and reads src_n words of each column (the
 // rest taken as zero; 0 = all). Inverse reads d_srcs when given, else in place.
+// aligned: every column pointer is 16-byte aligned (tiles then move as uint4).
 void ntt_forward_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables& t, hipStream_t s,
-                       uint64_t src_n = 0);
+                       uint64_t src_n = 0, bool aligned = false);
 void ntt_inverse_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables& t, hipStream_t s,
-                       const uint32_t* const* d_srcs = nullptr);
+                       const uint32_t* const* d_srcs = nullptr, bool aligned = false);
 
 // Builds the twiddles of coset (initial, step) of size 2^log_n directly on the
 // device: line (2^log_n - 1 words), circle (2^(log_n-1) words) and, when the

@@ -81,6 +81,7 @@ struct NttPassArgs {
     uint32_t s;      // lowest global stride bit of this pass
     uint32_t k;      // log2 of the transform size
     uint32_t scale;  // multiply on store when non-zero
+    uint32_t vec;    // every column pointer is 16-byte aligned: move tiles 4 words per lane
 };
 
 // LDS word index with one pad word per 8, which keeps the strided radix-8
@@ -102,31 +103,41 @@ __device__ __forceinline__ uint32_t ntt_twiddle(const NttPassArgs& a, uint32_t b
     }
 }
 
-// One radix-2^R round over row bits [j, j + R) of an LDS tile of 2^M rows x
+// One radix-2^R round over row bits [J, J + R) of an LDS tile of 2^M rows x
 // 2^C columns. Each group of 2^R values (fixed column and other row bits) is
 // loaded into registers, transformed through R layers, and written back.
-template <bool FWD, int M, int C, int R, bool STWO>
-__device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi, uint32_t j) {
+// J and R are compile-time, so the 2^R LDS addresses of a group are one base
+// plus constant offsets, and the twiddle of butterfly d in layer t is one
+// per-layer base index plus the constant d >> (t + 1).
+template <bool FWD, int M, int C, int R, int J, bool STWO>
+__device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi) {
     constexpr uint32_t GROUPS = 1u << (M + C - R);
+    constexpr uint32_t STEP = 1u << (J + C);  // tile index distance between rows d and d + 1
     for (uint32_t g = threadIdx.x; g < GROUPS; g += blockDim.x) {
         const uint32_t q = g & ((1u << C) - 1);
         const uint32_t rest = g >> C;
-        const uint32_t row_lo = rest & ((1u << j) - 1);
-        const uint32_t row_base = ((rest >> j) << (j + R)) | row_lo;
+        const uint32_t row_lo = rest & ((1u << J) - 1);
+        const uint32_t row_hi = rest >> J;
+        const uint32_t e0 = ((((row_hi << (J + R)) | row_lo)) << C) + q;
         uint32_t v[1 << R];
+        uint32_t addr[1 << R];
 #pragma unroll
-        for (int d = 0; d < (1 << R); d++) v[d] = tile[lpad(((row_base + ((uint32_t)d << j)) << C) + q)];
+        for (int d = 0; d < (1 << R); d++) {
+            // lpad is linear in d once the row step is a multiple of 8 words
+            addr[d] = (J + C >= 3) ? lpad(e0) + (uint32_t)d * (STEP + STEP / 8) : lpad(e0 + (uint32_t)d * STEP);
+            v[d] = tile[addr[d]];
+        }
 #pragma unroll
         for (int tt = 0; tt < R; tt++) {
             const int t = FWD ? (R - 1 - tt) : tt;  // layer inside the group
-            const uint32_t rb = j + t;                 // row bit of the layer
+            const uint32_t rb = J + t;                 // row bit of the layer
             const uint32_t b = a.s + rb;               // global stride bit
+            // h(d) = (hi << (M - rb - 1)) + (row >> (rb + 1)) = hb + (d >> (t + 1))
+            const uint32_t hb = (uint32_t)(hi << (M - rb - 1)) + (row_hi << (R - t - 1));
 #pragma unroll
             for (int d = 0; d < (1 << R); d++) {
                 if (d & (1 << t)) continue;
-                const uint32_t r0 = row_base + ((uint32_t)d << j);
-                const uint32_t h = (uint32_t)((hi << (M - rb - 1)) + (r0 >> (rb + 1)));
-                const uint32_t w = ntt_twiddle<STWO>(a, b, h);
+                const uint32_t w = ntt_twiddle<STWO>(a, b, hb + ((uint32_t)d >> (t + 1)));
                 uint32_t& x0 = v[d];
                 uint32_t& x1 = v[d | (1 << t)];
                 if (FWD) {
@@ -141,12 +152,35 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
             }
         }
 #pragma unroll
-        for (int d = 0; d < (1 << R); d++) tile[lpad(((row_base + ((uint32_t)d << j)) << C) + q)] = v[d];
+        for (int d = 0; d < (1 << R); d++) tile[addr[d]] = v[d];
+    }
+}
+
+// Radix-8 rounds over the M layers of a tile, top down (forward) or bottom up
+// (inverse), unrolled at compile time.
+template <bool FWD, int M, int C, bool STWO, int TOP>
+__device__ __forceinline__ void fwd_rounds(uint32_t* tile, const NttPassArgs& a, uint64_t hi) {
+    if constexpr (TOP > 0) {
+        constexpr int R = TOP >= 3 ? 3 : TOP;
+        ntt_round<FWD, M, C, R, TOP - R, STWO>(tile, a, hi);
+        __syncthreads();
+        fwd_rounds<FWD, M, C, STWO, TOP - R>(tile, a, hi);
+    }
+}
+template <bool FWD, int M, int C, bool STWO, int LO>
+__device__ __forceinline__ void inv_rounds(uint32_t* tile, const NttPassArgs& a, uint64_t hi) {
+    if constexpr (LO < M) {
+        constexpr int R = M - LO >= 3 ? 3 : M - LO;
+        ntt_round<FWD, M, C, R, LO, STWO>(tile, a, hi);
+        __syncthreads();
+        inv_rounds<FWD, M, C, STWO, LO + R>(tile, a, hi);
     }
 }
 
 // Applies the M layers with stride bits [s, s + M) of a size-2^k transform to
 // the tile of 2^M rows (stride 2^s) by 2^C contiguous columns owned by this block.
+// Tiles move between HBM and LDS 4 words per lane (rows of >= 4 contiguous
+// words, or the contiguous tile of the first pass) when the buffers allow it.
 template <bool FWD, int M, int C, bool STWO>
 __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
     __shared__ uint32_t tile[(1u << (M + C)) + (1u << (M + C)) / 8];
@@ -159,43 +193,44 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
     const uint64_t mid = blk & ((1ull << mid_bits) - 1);
     const uint64_t hi = blk >> mid_bits;
     const uint64_t base = (hi << (s + M)) + (mid << C);
+    constexpr bool VEC_OK = (C >= 2 || C == 0) && TILE >= 4;
+    const bool vec = VEC_OK && a.vec;
 
-    for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
-        uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
-        tile[lpad(e)] = g < a.src_n ? src[g] : 0u;
-    }
-    __syncthreads();
-    if (FWD) {
-        int top = M;
-        while (top > 0) {
-            int r = top >= 3 ? 3 : top;
-            top -= r;
-            if (r == 3)
-                ntt_round<FWD, M, C, (M >= 3 ? 3 : 1), STWO>(tile, a, hi, (uint32_t)top);
-            else if (r == 2)
-                ntt_round<FWD, M, C, (M >= 2 ? 2 : 1), STWO>(tile, a, hi, (uint32_t)top);
-            else
-                ntt_round<FWD, M, C, 1, STWO>(tile, a, hi, (uint32_t)top);
-            __syncthreads();
+    if (vec) {
+        for (uint32_t e = 4 * threadIdx.x; e < TILE; e += 4 * blockDim.x) {
+            const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+            const uint4 x = g < a.src_n ? *(const uint4*)(src + g) : make_uint4(0, 0, 0, 0);
+            const uint32_t p = lpad(e);  // e % 4 == 0: the four words stay within one padded group
+            tile[p] = x.x;
+            tile[p + 1] = x.y;
+            tile[p + 2] = x.z;
+            tile[p + 3] = x.w;
         }
     } else {
-        int lo = 0;
-        while (lo < M) {
-            int r = M - lo >= 3 ? 3 : M - lo;
-            if (r == 3)
-                ntt_round<FWD, M, C, (M >= 3 ? 3 : 1), STWO>(tile, a, hi, (uint32_t)lo);
-            else if (r == 2)
-                ntt_round<FWD, M, C, (M >= 2 ? 2 : 1), STWO>(tile, a, hi, (uint32_t)lo);
-            else
-                ntt_round<FWD, M, C, 1, STWO>(tile, a, hi, (uint32_t)lo);
-            lo += r;
-            __syncthreads();
+        for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
+            const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+            tile[lpad(e)] = g < a.src_n ? src[g] : 0u;
         }
     }
-    for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
-        uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
-        uint32_t v = tile[lpad(e)];
-        dst[g] = a.scale ? mmul(v, a.scale) : v;
+    __syncthreads();
+    if constexpr (FWD)
+        fwd_rounds<FWD, M, C, STWO, M>(tile, a, hi);
+    else
+        inv_rounds<FWD, M, C, STWO, 0>(tile, a, hi);
+    if (vec) {
+        for (uint32_t e = 4 * threadIdx.x; e < TILE; e += 4 * blockDim.x) {
+            const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+            const uint32_t p = lpad(e);
+            uint4 x = make_uint4(tile[p], tile[p + 1], tile[p + 2], tile[p + 3]);
+            if (a.scale) x = make_uint4(mmul(x.x, a.scale), mmul(x.y, a.scale), mmul(x.z, a.scale), mmul(x.w, a.scale));
+            *(uint4*)(dst + g) = x;
+        }
+    } else {
+        for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
+            const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+            uint32_t v = tile[lpad(e)];
+            dst[g] = a.scale ? mmul(v, a.scale) : v;
+        }
     }
 }
 
@@ -308,9 +343,11 @@ static void launch_pass(bool fwd, bool stwo, const NttPassArgs& a, uint32_t m, u
     check_launch("ntt_pass_kernel");
 }
 
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
 static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n, uint32_t* const* cols,
                          const uint32_t* const* srcs, uint32_t n_cols, const NttTables& t, hipStream_t st,
-                         const std::vector<Pass>& ps) {
+                         const std::vector<Pass>& ps, bool vec) {
     uint32_t k = t.log_n;
     uint32_t ex = (30u * k) % 31u;
     uint32_t inv_n = ex == 0 ? 1u : (1u << ex);
@@ -327,6 +364,7 @@ static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t 
         for (int l = 0; l < 32; l++) a.offs[l] = t.offs[l];
         a.s = ps[pi].s;
         a.k = k;
+        a.vec = vec ? 1u : 0u;
         a.scale = (!fwd && i + 1 == ps.size() && inv_n != 1u) ? inv_n : 0u;
         launch_pass(fwd, t.stwo, a, ps[pi].m, ps[pi].c, k, n_cols, st);
     }
@@ -335,8 +373,8 @@ static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t 
     }
 }
 static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n, uint32_t* const* cols,
-                    const uint32_t* const* srcs, uint32_t n_cols, const NttTables& t, hipStream_t st) {
-    run_ntt_plan(fwd, dst, src, src_n, cols, srcs, n_cols, t, st, plan_passes(t.log_n));
+                    const uint32_t* const* srcs, uint32_t n_cols, const NttTables& t, hipStream_t st, bool vec) {
+    run_ntt_plan(fwd, dst, src, src_n, cols, srcs, n_cols, t, st, plan_passes(t.log_n), vec);
 }
 
 void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTables& t, hipStream_t s) {
@@ -344,18 +382,19 @@ void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTa
         if (src != dst) check(hipMemcpyAsync(dst, src, 4, hipMemcpyDeviceToDevice, s), "ntt copy");
         return;
     }
-    run_ntt(true, dst, src, src_n, nullptr, nullptr, 1, t, s);
+    run_ntt(true, dst, src, src_n, nullptr, nullptr, 1, t, s, aligned16(dst) && aligned16(src));
 }
 void ntt_inverse(uint32_t* dst, const uint32_t* src, const NttTables& t, hipStream_t s) {
-    run_ntt(false, dst, src, 1ull << t.log_n, nullptr, nullptr, 1, t, s);
+    run_ntt(false, dst, src, 1ull << t.log_n, nullptr, nullptr, 1, t, s, aligned16(dst) && aligned16(src));
 }
-void ntt_forward_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& t, hipStream_t s, uint64_t src_n) {
+void ntt_forward_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& t, hipStream_t s, uint64_t src_n,
+                       bool aligned) {
     if (n_cols && t.log_n)
-        run_ntt(true, nullptr, nullptr, src_n ? src_n : (1ull << t.log_n), cols, nullptr, n_cols, t, s);
+        run_ntt(true, nullptr, nullptr, src_n ? src_n : (1ull << t.log_n), cols, nullptr, n_cols, t, s, aligned);
 }
 void ntt_inverse_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& t, hipStream_t s,
-                       const uint32_t* const* srcs) {
-    if (n_cols && t.log_n) run_ntt(false, nullptr, nullptr, 1ull << t.log_n, cols, srcs, n_cols, t, s);
+                       const uint32_t* const* srcs, bool aligned) {
+    if (n_cols && t.log_n) run_ntt(false, nullptr, nullptr, 1ull << t.log_n, cols, srcs, n_cols, t, s, aligned);
 }
 
 NttTables stwo_tables(const uint32_t* flat, uint32_t log_n) {
@@ -702,10 +741,11 @@ int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms) {
         float best = 1e30f;
         size_t bi = 0;
         for (size_t ci = 0; ok && ci < cands.size(); ci++) {
-            run_ntt_plan(true, data, data, 1ull << k, nullptr, nullptr, 1, t, st, cands[ci]);  // warm-up
+            run_ntt_plan(true, data, data, 1ull << k, nullptr, nullptr, 1, t, st, cands[ci], true);  // warm-up
             const int reps = 3;
             ok = check(hipEventRecord(e0, st), "hipEventRecord");
-            for (int r = 0; r < reps; r++) run_ntt_plan(true, data, data, 1ull << k, nullptr, nullptr, 1, t, st, cands[ci]);
+            for (int r = 0; r < reps; r++)
+                run_ntt_plan(true, data, data, 1ull << k, nullptr, nullptr, 1, t, st, cands[ci], true);
             ok = ok && check(hipEventRecord(e1, st), "hipEventRecord") && check(hipEventSynchronize(e1), "sync");
             float ms = 0.f;
             ok = ok && check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime") && peek_error() == 0;

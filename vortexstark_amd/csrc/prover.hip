@@ -269,11 +269,13 @@ struct vx_prover {
     uint64_t* d_call = nullptr;
     uint64_t* h_call = nullptr;  // pinned staging
     std::vector<uint64_t> last_call;
+    bool traces_aligned = false;  // every trace pointer of the current call is 16-byte aligned
     // graph replay of the launch sequence: one instantiated graph per batch
     // count, the most recent GRAPH_CACHE counts kept
     static constexpr size_t GRAPH_CACHE = 8;
     struct Captured {
         uint32_t count;
+        bool aligned;  // the captured launches move trace tiles as uint4
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
@@ -400,9 +402,9 @@ void enqueue_prove(vx_prover* p, uint32_t count) {
     auto mk = [&](const char* what) { p->mark(phase + ":" + what); };
     // LDE: interpolate on half_coset(log_n) into the first N words of the LDE
     // buffer, then evaluate on half_coset(log_n + 2) in place (zero pad implicit)
-    ntt_inverse_batch(p->d_eval_ptrs, count, p->trace_tab, s, (const uint32_t* const*)p->d_call);
+    ntt_inverse_batch(p->d_eval_ptrs, count, p->trace_tab, s, (const uint32_t* const*)p->d_call, p->traces_aligned);
     p->mark("lde:interpolate");
-    ntt_forward_batch(p->d_eval_ptrs, count, p->eval_tab, s, N);
+    ntt_forward_batch(p->d_eval_ptrs, count, p->eval_tab, s, N, true);
     p->mark("lde:evaluate");
     Scratch sc = {p->at(p->o_scr_a), p->at(p->o_scr_b)};
     uint32_t* pr = p->proof();
@@ -442,8 +444,10 @@ void enqueue_prove(vx_prover* p, uint32_t count) {
 // the pinned staging buffer must change (an earlier copy may still read it).
 bool upload_call(vx_prover* p, const uint32_t* const* traces, const uint32_t* ab, uint32_t count) {
     std::vector<uint64_t> want(2ull * p->capacity, 0);
+    p->traces_aligned = true;
     for (uint32_t i = 0; i < count; i++) {
         want[i] = (uint64_t)(uintptr_t)traces[i];
+        if (want[i] & 15) p->traces_aligned = false;
         ((uint32_t*)(want.data() + p->capacity))[2 * i] = ab[2 * i] % P;
         ((uint32_t*)(want.data() + p->capacity))[2 * i + 1] = ab[2 * i + 1] % P;
     }
@@ -463,7 +467,9 @@ int32_t run_batch(vx_prover* p, uint32_t count) {
         return peek_error() == 0 ? 0 : -1;
     }
     auto& G = p->graphs;
-    auto it = std::find_if(G.begin(), G.end(), [&](const vx_prover::Captured& c) { return c.count == count; });
+    auto it = std::find_if(G.begin(), G.end(), [&](const vx_prover::Captured& c) {
+        return c.count == count && c.aligned == p->traces_aligned;
+    });
     if (it != G.end()) {
         std::rotate(it, it + 1, G.end());  // most recently used last
     } else {
@@ -476,7 +482,7 @@ int32_t run_batch(vx_prover* p, uint32_t count) {
         if (!check(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
             return -1;
         enqueue_prove(p, count);
-        vx_prover::Captured c = {count, nullptr, nullptr};
+        vx_prover::Captured c = {count, p->traces_aligned, nullptr, nullptr};
         const bool ok = check(hipStreamEndCapture(p->stream, &c.graph), "hipStreamEndCapture");
         p->tl.on = timing;
         if (!ok || peek_error() || !check(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0),

@@ -191,6 +191,7 @@ struct vx_wide {
     const uint32_t** d_src_ptrs = nullptr;
     const uint32_t** h_src_ptrs = nullptr;  // pinned staging of the caller's column pointers
     std::vector<const uint32_t*> last_src;
+    bool src_aligned = false;  // every caller column pointer is 16-byte aligned
     LiftCol* d_lift = nullptr;
     std::vector<uint32_t*> layer;  // J layers, 4 columns each contiguous
     Scratch scratch = {nullptr, nullptr};
@@ -401,7 +402,11 @@ int32_t vx_wide_commit_device(vx_wide* w, const uint32_t* const* h_cols) {
     if (w->last_src.size() != w->n_cols || !std::equal(w->last_src.begin(), w->last_src.end(), h_cols)) {
         // the staging buffer may still feed an earlier copy: drain first
         if (!check(hipStreamSynchronize(s), "vx_wide_commit_device")) return -1;
-        for (uint32_t c = 0; c < w->n_cols; c++) w->h_src_ptrs[c] = h_cols[c];
+        w->src_aligned = true;
+        for (uint32_t c = 0; c < w->n_cols; c++) {
+            w->h_src_ptrs[c] = h_cols[c];
+            if ((uintptr_t)h_cols[c] & 15) w->src_aligned = false;
+        }
         if (!check(hipMemcpyAsync(w->d_src_ptrs, w->h_src_ptrs, 8ull * w->n_cols, hipMemcpyHostToDevice, s), "H2D"))
             return -1;
         w->last_src.assign(h_cols, h_cols + w->n_cols);
@@ -410,12 +415,12 @@ int32_t vx_wide_commit_device(vx_wide* w, const uint32_t* const* h_cols) {
     // LDE: a suffix of the 2^S twiddle buffer is the buffer of every smaller domain
     for (const auto& g : w->groups) {
         const NttTables ti = stwo_tables(w->tw_inv + (tc - (1ull << (g.log - 1))), g.log);
-        ntt_inverse_batch(w->d_lde_ptrs + g.first, g.count, ti, s, w->d_src_ptrs + g.first);
+        ntt_inverse_batch(w->d_lde_ptrs + g.first, g.count, ti, s, w->d_src_ptrs + g.first, w->src_aligned);
     }
     w->mark("lde:interpolate");
     for (const auto& g : w->groups) {
         const NttTables te = stwo_tables(w->tw_fwd + (tc - (1ull << (g.log + B - 1))), g.log + B);
-        ntt_forward_batch(w->d_lde_ptrs + g.first, g.count, te, s, 1ull << g.log);
+        ntt_forward_batch(w->d_lde_ptrs + g.first, g.count, te, s, 1ull << g.log, true);
     }
     w->mark("lde:evaluate");
     std::string phase = "lifted_commit";
