@@ -40,8 +40,9 @@ METRIC = "LDE+commit+FRI Melem/s (frac of HBM roofline); batched proofs/s, p50/p
 HBM_PEAK_GBS = 8000.0
 # int32 VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (lane-ops/s)
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
-# VALU ops of one Blake2s compression: 10 rounds x 8 G x 12 ops + 16 feed-forward xors
-OPS_PER_COMPRESSION = 10 * 8 * 12 + 16
+# VALU ops of one Blake2s compression on gfx950: 10 rounds x 8 G x 12 ops (v_add3,
+# v_add, v_xor, v_alignbit) + 8 three-input xors of the feed-forward (v_bitop3)
+OPS_PER_COMPRESSION = 10 * 8 * 12 + 8
 
 
 def load_traffic() -> dict:

@@ -2,8 +2,8 @@
 // helper is src/channel.rs:222-298 and its GPU copy cuda/include/blake2s.cuh).
 //
 // All state lives in VGPRs. Rotations lower to v_alignbit_b32 and the
-// three-input adds to v_add3_u32, so one G costs 12 VALU ops and a compression
-// about 960 + 16. The round schedule is written out with compile-time message
+// three-input adds to v_add3_u32, so one G costs 12 VALU ops; the feed-forward
+// is one v_bitop3_b32 per word: a compression is 960 + 8 VALU ops. The round schedule is written out with compile-time message
 // indices so no message word is ever addressed dynamically.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -21,6 +21,8 @@ struct Digest {
 };
 
 #define VX_ROTR(x, n) __builtin_rotateright32((x), (n))
+// x ^ y ^ z in one gfx950 instruction (v_bitop3_b32, truth table 0x96)
+#define VX_XOR3(x, y, z) __builtin_amdgcn_bitop3_b32((x), (y), (z), 0x96)
 #define VX_G(a, b, c, d, x, y)      \
     do {                            \
         a = a + b + (x);            \
@@ -59,14 +61,14 @@ __device__ __forceinline__ void b2s_compress(uint32_t h[8], const uint32_t m[16]
     VX_ROUND(m, 13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10);
     VX_ROUND(m, 6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5);
     VX_ROUND(m, 10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0);
-    h[0] ^= v0 ^ v8;
-    h[1] ^= v1 ^ v9;
-    h[2] ^= v2 ^ v10;
-    h[3] ^= v3 ^ v11;
-    h[4] ^= v4 ^ v12;
-    h[5] ^= v5 ^ v13;
-    h[6] ^= v6 ^ v14;
-    h[7] ^= v7 ^ v15;
+    h[0] = VX_XOR3(h[0], v0, v8);
+    h[1] = VX_XOR3(h[1], v1, v9);
+    h[2] = VX_XOR3(h[2], v2, v10);
+    h[3] = VX_XOR3(h[3], v3, v11);
+    h[4] = VX_XOR3(h[4], v4, v12);
+    h[5] = VX_XOR3(h[5], v5, v13);
+    h[6] = VX_XOR3(h[6], v6, v14);
+    h[7] = VX_XOR3(h[7], v7, v15);
 }
 
 __device__ __forceinline__ void b2s_init(uint32_t h[8]) {
@@ -173,8 +175,8 @@ __device__ __forceinline__ void b2s_compress_quad(const uint32_t* msg, uint32_t 
         c = quad_rot2(c);
         d = quad_rot1(d);
     }
-    hl ^= a ^ c;
-    hh ^= b ^ d;
+    hl = VX_XOR3(hl, a, c);
+    hh = VX_XOR3(hh, b, d);
 }
 
 // Initial chaining words of lane l (IV with the parameter block folded in).

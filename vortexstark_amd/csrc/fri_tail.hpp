@@ -1,0 +1,132 @@
+// The small FRI layers of a proof in one launch.
+//
+// Below 1024 values a layer's fold, leaf hashes, tree and transcript step fit
+// one workgroup, and launching a kernel per layer costs more than the layer
+// itself. fri_tail_kernel runs every remaining line fold of a proof in one
+// workgroup: fold (cuda/fri.cu:10-39) into the next layer, hash it as an SoA4
+// tree (level-store nodes written as by the stage/tail kernels), reduce to the
+// root, mix the root into the transcript and draw the next challenge, which
+// stays in LDS for the next layer (and is also written to device memory).
+// blockIdx.y selects the proof of a batch (buffers bs words apart).
+#pragma once
+#include "merkle_engine.hpp"
+
+namespace vx {
+
+constexpr uint32_t FRI_TAIL_MAX_LAYERS = 12;
+constexpr uint32_t FRI_TAIL_MAX_ITEMS = 1024;  // values of the largest layer handled here
+constexpr uint32_t FRI_TAIL_THREADS = 512;
+
+struct FriTailLayer {
+    const uint32_t* in[4];   // previous layer, 2n values per column
+    uint32_t* out[4];        // this layer, n values per column
+    const uint32_t* tw;      // n fold twiddles (shared by a batch)
+    uint32_t* root_dst;      // 8 words
+    uint32_t* alpha_dst;     // next layer's challenge (null after the last layer)
+    LevelStore st;
+    uint32_t n;              // values of this layer: a power of two in [2, 1024]
+};
+struct FriTailArgs {
+    FriTailLayer L[FRI_TAIL_MAX_LAYERS];
+    uint32_t n_layers;
+    const uint32_t* alpha0;  // challenge of the first layer
+    DevChannel* channel;
+};
+
+static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTailArgs args, uint64_t bs) {
+    constexpr uint32_t W = FRI_TAIL_MAX_ITEMS / 2;
+    __shared__ uint32_t nodes[8 * W];  // word-major: word q of node j at q * W + j
+    __shared__ uint32_t na[8 * 64], nb[8 * 64];
+    __shared__ uint32_t msg[16];
+    __shared__ uint32_t al[4];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t boff = (uint64_t)blockIdx.y * bs;
+    DevChannel* ch = (DevChannel*)((uint32_t*)args.channel + boff);
+    if (tid < 4) al[tid] = args.alpha0[boff + tid];
+    __syncthreads();
+    for (uint32_t li = 0; li < args.n_layers; li++) {
+        const FriTailLayer& L = args.L[li];
+        const LevelStore st = L.st.shifted(boff);
+        const QM31 alpha = {{al[0], al[1], al[2], al[3]}};
+        const uint32_t half = L.n / 2;
+        const uint32_t* i0 = L.in[0] + boff;
+        const uint32_t* i1 = L.in[1] + boff;
+        const uint32_t* i2 = L.in[2] + boff;
+        const uint32_t* i3 = L.in[3] + boff;
+        // level 1: parents of leaf pairs (2j, 2j+1), each leaf a fold of two values
+        for (uint32_t j = tid; j < half; j += FRI_TAIL_THREADS) {
+            const uint32_t k = 2 * j;
+            const uint32_t x = 2 * k;
+            const uint4 a = *(const uint4*)(i0 + x), b = *(const uint4*)(i1 + x);
+            const uint4 c = *(const uint4*)(i2 + x), d = *(const uint4*)(i3 + x);
+            const uint2 w = *(const uint2*)(L.tw + k);
+            const QM31 f0 = fold_pair({{a.x, b.x, c.x, d.x}}, {{a.y, b.y, c.y, d.y}}, alpha, w.x);
+            const QM31 f1 = fold_pair({{a.z, b.z, c.z, d.z}}, {{a.w, b.w, c.w, d.w}}, alpha, w.y);
+            *(uint2*)(L.out[0] + boff + k) = make_uint2(f0.v[0], f1.v[0]);
+            *(uint2*)(L.out[1] + boff + k) = make_uint2(f0.v[1], f1.v[1]);
+            *(uint2*)(L.out[2] + boff + k) = make_uint2(f0.v[2], f1.v[2]);
+            *(uint2*)(L.out[3] + boff + k) = make_uint2(f0.v[3], f1.v[3]);
+            uint32_t l[8], r[8], h[8];
+            b2s_leaf4(f0.v[0], f0.v[1], f0.v[2], f0.v[3], l);
+            b2s_leaf4(f1.v[0], f1.v[1], f1.v[2], f1.v[3], r);
+            b2s_node(l, r, h);
+            if (st.want(1)) store8(st.at(1, j), h);
+#pragma unroll
+            for (int q = 0; q < 8; q++) nodes[q * W + j] = h[q];
+        }
+        __syncthreads();
+        uint32_t m = half, lvl = 1;
+        while (m > 64) {
+            const uint32_t hm = m / 2;
+            uint32_t l[8], r[8];
+            if (tid < hm) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    l[q] = nodes[q * W + 2 * tid];
+                    r[q] = nodes[q * W + 2 * tid + 1];
+                }
+            }
+            __syncthreads();
+            lvl++;
+            if (tid < hm) {
+                uint32_t h[8];
+                b2s_node(l, r, h);
+#pragma unroll
+                for (int q = 0; q < 8; q++) nodes[q * W + tid] = h[q];
+                if (st.want(lvl)) store8(st.at(lvl, tid), h);
+            }
+            __syncthreads();
+            m = hm;
+        }
+        if (tid < 64) {  // one wave: narrow levels, root, transcript
+            if (tid < m)
+                for (int q = 0; q < 8; q++) na[tid * 8 + q] = nodes[q * W + tid];
+            VX_WAVE_SYNC();
+            const uint32_t* root = narrow_reduce(na, nb, m, lvl, 0, st);
+            if (tid < 8) L.root_dst[boff + tid] = root[tid];
+            uint32_t* adst = L.alpha_dst ? L.alpha_dst + boff : nullptr;
+            quad_transcript(ch, root, msg, false, adst);
+            if (adst && tid < 4) al[tid] = adst[tid];  // written by this wave
+        }
+        __syncthreads();
+    }
+}
+
+// Launches fri_tail_kernel over layers [j0, J) (in chunks of FRI_TAIL_MAX_LAYERS);
+// layer(j) fills the descriptor of layer j, alpha(j) is the device address of
+// its challenge (alpha(j0) already drawn).
+template <class LayerFn, class AlphaFn>
+void launch_fri_tail(uint32_t j0, uint32_t J, LayerFn&& layer, AlphaFn&& alpha, DevChannel* ch, uint32_t batch,
+                     uint64_t bs, hipStream_t s) {
+    for (uint32_t j = j0; j < J; j += FRI_TAIL_MAX_LAYERS) {
+        FriTailArgs a = {};
+        a.n_layers = J - j < FRI_TAIL_MAX_LAYERS ? J - j : FRI_TAIL_MAX_LAYERS;
+        for (uint32_t i = 0; i < a.n_layers; i++) a.L[i] = layer(j + i);
+        a.alpha0 = alpha(j);
+        a.channel = ch;
+        fri_tail_kernel<<<dim3(1, batch), FRI_TAIL_THREADS, 0, s>>>(a, bs);
+        check_launch("fri_tail_kernel");
+    }
+}
+
+}  // namespace vx

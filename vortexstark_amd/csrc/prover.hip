@@ -23,6 +23,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "commit.hpp"
+#include "fri_tail.hpp"
 #include "merkle_engine.hpp"
 
 namespace vx {
@@ -421,13 +422,34 @@ void enqueue_prove(vx_prover* p, uint32_t count) {
         uint32_t* nxt = p->J > 1 ? p->alpha_f(1) : nullptr;
         commit_tree(src, E / 2, p->stores[2], TailOut{pr + p->off_fri_roots, p->ch(), nxt, 0}, sc, s, mk, count, R);
     }
-    for (uint32_t j = 1; j < p->J; j++) {
+    // layers of at most FRI_TAIL_MAX_ITEMS values all go to one fri_tail_kernel launch
+    uint32_t js = 1;
+    while (js < p->J && (E >> (js + 1)) > FRI_TAIL_MAX_ITEMS) js++;
+    for (uint32_t j = 1; j < js; j++) {
         phase = "fri" + std::to_string(j) + "_line_fold_commit";
         SrcLineFold src = {p->col(j - 1, 0), p->col(j - 1, 1), p->col(j - 1, 2), p->col(j - 1, 3), p->alpha_f(j),
                            p->fri_tw[j],     p->col(j, 0),     p->col(j, 1),     p->col(j, 2),     p->col(j, 3)};
         uint32_t* nxt = j + 1 < p->J ? p->alpha_f(j + 1) : nullptr;
         commit_tree(src, E >> (j + 1), p->stores[2 + j], TailOut{pr + p->off_fri_roots + 8ull * j, p->ch(), nxt, 0},
                     sc, s, mk, count, R);
+    }
+    if (js < p->J) {
+        auto layer = [&](uint32_t j) {
+            FriTailLayer L = {};
+            for (int c = 0; c < 4; c++) {
+                L.in[c] = p->col(j - 1, c);
+                L.out[c] = p->col(j, c);
+            }
+            L.tw = p->fri_tw[j];
+            L.root_dst = pr + p->off_fri_roots + 8ull * j;
+            L.alpha_dst = j + 1 < p->J ? p->alpha_f(j + 1) : nullptr;
+            L.st = p->stores[2 + j];
+            L.n = (uint32_t)(E >> (j + 1));
+            return L;
+        };
+        launch_fri_tail(js, p->J, layer, [&](uint32_t j) { return (const uint32_t*)p->alpha_f(j); }, p->ch(), count,
+                        R, s);
+        p->mark("fri_small_layers");
     }
     const uint32_t jl = p->J - 1;
     queries_kernel<<<count, 256, 0, s>>>(p->ch(), p->col(jl, 0), p->col(jl, 1), p->col(jl, 2), p->col(jl, 3),

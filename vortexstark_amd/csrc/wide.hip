@@ -25,6 +25,7 @@
 
 #include "commit.hpp"
 #include "common.hpp"
+#include "fri_tail.hpp"
 #include "kernels.hpp"
 
 namespace vx {
@@ -439,12 +440,32 @@ int32_t vx_wide_commit_device(vx_wide* w, const uint32_t* const* h_cols) {
         uint32_t* nxt = w->J > 1 ? w->chal + 8 : nullptr;
         commit_tree(src, w->layer_size(0), none, TailOut{w->roots + 8, w->ch, nxt, 0}, w->scratch, s, mk);
     }
-    for (uint32_t j = 1; j < w->J; j++) {
+    uint32_t js = 1;
+    while (js < w->J && w->layer_size(js) > FRI_TAIL_MAX_ITEMS) js++;
+    for (uint32_t j = 1; j < js; j++) {
         phase = "fri" + std::to_string(j) + "_line_fold_commit";
         SrcLineFold src = {w->col(j - 1, 0), w->col(j - 1, 1), w->col(j - 1, 2), w->col(j - 1, 3), w->chal + 4 * (j + 1),
                            w->tw_line[j],    w->col(j, 0),     w->col(j, 1),     w->col(j, 2),     w->col(j, 3)};
         uint32_t* nxt = j + 1 < w->J ? w->chal + 4 * (j + 2) : nullptr;
         commit_tree(src, w->layer_size(j), none, TailOut{w->roots + 8 * (j + 1), w->ch, nxt, 0}, w->scratch, s, mk);
+    }
+    if (js < w->J) {
+        auto layer = [&](uint32_t j) {
+            FriTailLayer L = {};
+            for (int c = 0; c < 4; c++) {
+                L.in[c] = w->col(j - 1, c);
+                L.out[c] = w->col(j, c);
+            }
+            L.tw = w->tw_line[j];
+            L.root_dst = w->roots + 8 * (j + 1);
+            L.alpha_dst = j + 1 < w->J ? w->chal + 4 * (j + 2) : nullptr;
+            L.st = none;
+            L.n = (uint32_t)w->layer_size(j);
+            return L;
+        };
+        launch_fri_tail(js, w->J, layer, [&](uint32_t j) { return (const uint32_t*)(w->chal + 4 * (j + 1)); }, w->ch,
+                        1, 0, s);
+        w->mark("fri_small_layers");
     }
     return peek_error() == 0 ? 0 : -1;
 }
