@@ -319,6 +319,18 @@ struct SrcLifted {
     }
 };
 
+// x < 2^64 -> a value < 2^33 congruent to it mod p: bits 0..30, plus bits
+// 31..62 (times 2^31 = 1), plus bit 63 (times 2^63 = 2 mod p).
+__device__ __forceinline__ uint64_t fold62(uint64_t x) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    return (uint64_t)(lo & P) + __builtin_amdgcn_alignbit(hi, lo, 31) + 2u * (hi >> 31);
+}
+// x < 2^64 -> x mod p
+__device__ __forceinline__ uint32_t reduce64(uint64_t x) {
+    x = fold62(fold62(x));  // < 2^31 + 4
+    return canon((uint32_t)x);
+}
+
 // First FRI layer of the wide commitment: f = sum_c beta^c * lifted_c is formed
 // on the fly from the LDE columns, circle-folded pairwise (2j, 2j + 1) with
 // alpha and tw[j], stored to out and hashed as an SoA4 leaf.
@@ -333,35 +345,82 @@ struct SrcWideFold {
     uint32_t* o2;
     uint32_t* o3;
     __device__ __forceinline__ SrcWideFold at(uint64_t) const { return *this; }
+    // Column c's values at f indices j .. j + 3 (j a multiple of 4): a full-size
+    // column gives four rows, a lifted one rows r, r + 1 (repeated for j + 2, j + 3).
+    __device__ __forceinline__ u32x4 load_col(ConstLiftCol* tab, uint32_t c, uint64_t j) const {
+        const uint32_t* p = tab[c].p;
+        const uint32_t sh = tab[c].sh;
+        if (sh == 0) return *(GlobalV4*)(p + j);
+        const u32x2 v = *(GlobalV2*)(p + ((j >> sh) << 1));
+        return u32x4{v.x, v.y, 0u, 0u};
+    }
     __device__ __forceinline__ void pair(uint64_t i, uint32_t h[8]) const {
         const uint64_t j = 2 * i;  // f index of the first of four inputs (a multiple of 4)
-        QM31 f[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) f[k] = {{0, 0, 0, 0}};
         ConstLiftCol* tab = const_table(cols);
         typedef const __attribute__((address_space(4))) uint32_t ConstU32;
         ConstU32* bp = (ConstU32*)beta_pows;
-#pragma unroll 4
-        for (uint32_t c = 0; c < n_cols; c++) {
-            const uint32_t* p = tab[c].p;
-            const uint32_t sh = tab[c].sh;
-            const QM31 w = {{bp[4 * c], bp[4 * c + 1], bp[4 * c + 2], bp[4 * c + 3]}};
-            if (sh == 0) {
-                const u32x4 v = *(GlobalV4*)(p + j);
-                f[0] = qadd(f[0], qmul_m31(w, v.x));
-                f[1] = qadd(f[1], qmul_m31(w, v.y));
-                f[2] = qadd(f[2], qmul_m31(w, v.z));
-                f[3] = qadd(f[3], qmul_m31(w, v.w));
-            } else {
-                // rows of j .. j + 3 are r, r + 1, r, r + 1 (sh >= 2)
-                const u32x2 v = *(GlobalV2*)(p + ((j >> sh) << 1));
-                const QM31 a = qmul_m31(w, v.x), b = qmul_m31(w, v.y);
-                f[0] = qadd(f[0], a);
-                f[1] = qadd(f[1], b);
-                f[2] = qadd(f[2], a);
-                f[3] = qadd(f[3], b);
-            }
+        // sum_c beta^c * v_c with 64-bit products accumulated unreduced: three
+        // products (< 2^62 each) fit on top of a folded (< 2^33) accumulator,
+        // so accumulators are folded after every group of three columns.
+        // Full-size columns feed af (four points), lifted ones ag (two).
+        uint64_t af[4][4], ag[2][4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            for (int k = 0; k < 4; k++) af[k][m] = 0;
+            ag[0][m] = ag[1][m] = 0;
         }
+        constexpr uint32_t G = 3;
+        u32x4 cur[G], nxt[G];
+#pragma unroll
+        for (uint32_t k = 0; k < G; k++)
+            if (k < n_cols) cur[k] = load_col(tab, k, j);
+        for (uint32_t c0 = 0; c0 < n_cols; c0 += G) {
+            // the next group's loads are in flight while this group accumulates
+#pragma unroll
+            for (uint32_t k = 0; k < G; k++)
+                if (c0 + G + k < n_cols) nxt[k] = load_col(tab, c0 + G + k, j);
+            bool tf = false, tl = false;
+#pragma unroll
+            for (uint32_t k = 0; k < G; k++) {
+                const uint32_t c = c0 + k;
+                if (c >= n_cols) break;
+                const uint32_t w[4] = {bp[4 * c], bp[4 * c + 1], bp[4 * c + 2], bp[4 * c + 3]};
+                const u32x4 v = cur[k];
+                if (tab[c].sh == 0) {
+                    tf = true;
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        af[0][m] += (uint64_t)w[m] * v.x;
+                        af[1][m] += (uint64_t)w[m] * v.y;
+                        af[2][m] += (uint64_t)w[m] * v.z;
+                        af[3][m] += (uint64_t)w[m] * v.w;
+                    }
+                } else {
+                    tl = true;
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        ag[0][m] += (uint64_t)w[m] * v.x;
+                        ag[1][m] += (uint64_t)w[m] * v.y;
+                    }
+                }
+            }
+            if (tf)
+#pragma unroll
+                for (int m = 0; m < 4; m++)
+                    for (int k = 0; k < 4; k++) af[k][m] = fold62(af[k][m]);
+            if (tl)
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    ag[0][m] = fold62(ag[0][m]);
+                    ag[1][m] = fold62(ag[1][m]);
+                }
+#pragma unroll
+            for (uint32_t k = 0; k < G; k++) cur[k] = nxt[k];
+        }
+        QM31 f[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            for (int m = 0; m < 4; m++) f[k].v[m] = reduce64(af[k][m] + ag[k & 1][m]);
         const QM31 al = load_alpha(alpha);
         const uint2 w = *(const uint2*)(tw + i);
         const QM31 g0 = fold_pair(f[0], f[1], al, w.x);
