@@ -7,16 +7,17 @@
 
 namespace vx {
 
-inline uint32_t pick_logc(uint64_t n) {
+inline uint32_t pick_logc(uint64_t n, uint32_t logc_max = MERKLE_LOGC_MAX) {
     const uint64_t MIN_WAVES = 1024;
-    for (int c = (int)MERKLE_LOGC_MAX; c > 0; c--)
+    for (int c = (int)logc_max; c > 0; c--)
         if ((n >> (7 + c)) >= MIN_WAVES) return (uint32_t)c;
     return 0;
 }
 
 // Trees of at most this many items go straight to the tail kernel; larger
-// ones are reduced by stage kernels until this many nodes remain.
-constexpr uint32_t PROVER_TAIL_ITEMS = 1024;
+// ones are reduced by stage kernels until this many nodes remain (one more
+// tail level is cheaper than one more stage launch).
+constexpr uint32_t PROVER_TAIL_ITEMS = MERKLE_TAIL_MAX;
 
 struct Scratch {
     uint32_t* a;
@@ -34,7 +35,7 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
         mark("tail");
         return;
     }
-    uint32_t logc = pick_logc(n_leaves);
+    uint32_t logc = pick_logc(n_leaves, StageLogcMax<Src>::value);
     uint64_t waves = n_leaves >> (7 + logc);
     merkle_stage_kernel<Src><<<dim3(ceil_div(waves, MERKLE_STAGE_WAVES), batch), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
         src, 0, n_leaves, logc, sc.a, st, bs);
@@ -62,11 +63,15 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
     mark("tail");
 }
 
-inline uint64_t max_stage_nodes(uint64_t n_leaves) {
+// Largest stage output of a tree of n_leaves items whose leaf stage uses at most
+// leaf_logc_max (node stages use the default).
+inline uint64_t max_stage_nodes(uint64_t n_leaves, uint32_t leaf_logc_max = MERKLE_LOGC_MAX) {
     if (n_leaves <= PROVER_TAIL_ITEMS) return 0;
     uint64_t mx = 0, n = n_leaves;
+    bool leaf = true;
     while (n > PROVER_TAIL_ITEMS) {
-        n >>= 7 + pick_logc(n);
+        n >>= 7 + pick_logc(n, leaf ? leaf_logc_max : MERKLE_LOGC_MAX);
+        leaf = false;
         if (n > mx) mx = n;
     }
     return mx;

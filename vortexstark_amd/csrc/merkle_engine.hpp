@@ -23,6 +23,8 @@
 //    store; decommitment recomputes the 2^store_from-leaf subtree of a query
 //    and reads the rest, so full trees are never written to HBM.
 #pragma once
+#include <type_traits>
+
 #include "blake2s.hpp"
 #include "field.hpp"
 #include "transcript.hpp"
@@ -271,6 +273,7 @@ typedef const __attribute__((address_space(1))) u32x4 GlobalV4;
 __device__ __forceinline__ ConstLiftCol* const_table(const LiftCol* t) { return (ConstLiftCol*)t; }
 
 struct SrcLifted {
+    static constexpr uint32_t STAGE_LOGC_MAX = 4;
     const LiftCol* cols;  // device, hashing order
     uint32_t n_cols;
     __device__ __forceinline__ SrcLifted at(uint64_t) const { return *this; }
@@ -335,6 +338,7 @@ __device__ __forceinline__ uint32_t reduce64(uint64_t x) {
 // on the fly from the LDE columns, circle-folded pairwise (2j, 2j + 1) with
 // alpha and tw[j], stored to out and hashed as an SoA4 leaf.
 struct SrcWideFold {
+    static constexpr uint32_t STAGE_LOGC_MAX = 4;
     const LiftCol* cols;
     uint32_t n_cols;
     const uint32_t* beta_pows;  // device, 4 words per column
@@ -522,10 +526,22 @@ __device__ __forceinline__ void quad_transcript(DevChannel* ch, const uint32_t* 
 }
 
 // ---------------------------------------------------------------- stage kernel
-// LDS per wave: MERKLE_LOGC_MAX pending chunks + 1 exchange chunk, each 64
+// LDS per wave: up to logc_max pending chunks + 1 exchange chunk, each 64
 // nodes stored word-major ([word][lane]) so chunk writes are conflict-free.
-constexpr uint32_t STAGE_SLOTS = MERKLE_LOGC_MAX + 1;
-constexpr uint32_t STAGE_WAVE_LDS = STAGE_SLOTS * 8 * 64;
+// A source may cap logc_max (STAGE_LOGC_MAX) to trade tree depth per wave for
+// occupancy: memory-heavy leaf sources need more waves in flight.
+template <class S, class = void>
+struct StageLogcMax {
+    static constexpr uint32_t value = MERKLE_LOGC_MAX;
+};
+template <class S>
+struct StageLogcMax<S, std::void_t<decltype(S::STAGE_LOGC_MAX)>> {
+    static constexpr uint32_t value = S::STAGE_LOGC_MAX;
+};
+template <class S>
+constexpr uint32_t stage_wave_lds() {
+    return (StageLogcMax<S>::value + 1) * 8 * 64;
+}
 
 // blockIdx.y selects the proof of a batch: every per-proof buffer (items,
 // out_nodes, level store) lies blockIdx.y * bs words past proof 0's.
@@ -534,14 +550,15 @@ __global__ __launch_bounds__(64 * MERKLE_STAGE_WAVES) void merkle_stage_kernel(S
                                                                              uint64_t n_items, uint32_t logc,
                                                                              uint32_t* out_nodes, LevelStore st0,
                                                                              uint64_t bs) {
-    __shared__ uint32_t lds[MERKLE_STAGE_WAVES * STAGE_WAVE_LDS];
+    constexpr uint32_t WAVE_LDS = stage_wave_lds<Src>();
+    __shared__ uint32_t lds[MERKLE_STAGE_WAVES * WAVE_LDS];
     const uint64_t boff = (uint64_t)blockIdx.y * bs;
     const Src src = src0.at(boff);
     const LevelStore st = st0.shifted(boff);
     out_nodes += boff;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t* wl = lds + w * STAGE_WAVE_LDS;
-    uint32_t* xchg = wl + MERKLE_LOGC_MAX * 8 * 64;
+    uint32_t* wl = lds + w * WAVE_LDS;
+    uint32_t* xchg = wl + StageLogcMax<Src>::value * 8 * 64;
     const uint64_t wave_id = (uint64_t)blockIdx.x * MERKLE_STAGE_WAVES + w;
     const uint64_t base = wave_id << (7 + logc);
     if (base >= n_items) return;  // wave-uniform
