@@ -427,8 +427,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(independent proofs shard by job; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path here "
+              "(independent proofs would shard by job). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import vortexstark_amd as vx
