@@ -22,6 +22,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "ntt_plan.hpp"
 
 namespace vx {
 
@@ -234,60 +235,10 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
     }
 }
 
-// Pass shapes. The contiguous pass (c = 0) holds up to 2^14 words in LDS;
-// strided passes move 2^c contiguous words per row (c = 4: 64-byte, c = 5:
-// 128-byte segments) of up to 2^14-word tiles. Which split a size uses comes
-// from the plan table: the default below, or what the autotuner measured.
-constexpr uint32_t NTT_LOW_MAX = 14;   // most layers of the contiguous pass
-constexpr uint32_t NTT_HIGH_MAX5 = 9;  // most layers of a strided pass, c = 5
-constexpr uint32_t NTT_HIGH_MAX4 = 10; // most layers of a strided pass, c = 4
-constexpr uint32_t NTT_LOW = 12;       // default split: contiguous pass
-constexpr uint32_t NTT_HIGH = 7;       // default split: most layers per strided pass
-constexpr uint32_t NTT_COLS = 5;       // default split: strided row segment
-
-struct Pass {
-    uint32_t s, m, c;
-};
-
 namespace {
 std::mutex g_plan_mu;
 std::vector<Pass> g_plan[32];  // tuned or set plans; empty: default split
 }  // namespace
-
-static std::vector<Pass> default_plan(uint32_t k) {
-    std::vector<Pass> ps;
-    if (k == 0) return ps;
-    uint32_t m0 = k < NTT_LOW ? k : NTT_LOW;
-    ps.push_back({0, m0, 0});
-    uint32_t R = k - m0;
-    if (R) {
-        uint32_t np = (R + NTT_HIGH - 1) / NTT_HIGH, s = m0;
-        for (uint32_t i = 0; i < np; i++) {
-            uint32_t mi = R / np + (i < R % np ? 1 : 0);
-            ps.push_back({s, mi, NTT_COLS});
-            s += mi;
-        }
-    }
-    return ps;
-}
-
-// A plan is valid when its passes cover layers 0 .. k-1 in order, the first is
-// contiguous, every strided pass starts at a stride of at least its segment
-// and every shape has a compiled kernel.
-static bool plan_valid(uint32_t k, const std::vector<Pass>& ps) {
-    if (k == 0) return ps.empty();
-    if (ps.empty() || ps.size() > 8 || ps[0].c != 0 || ps[0].s != 0) return false;
-    uint32_t s = 0;
-    for (const Pass& p : ps) {
-        if (p.s != s || p.m == 0) return false;
-        if (p.c == 0 && (p.s != 0 || p.m > NTT_LOW_MAX)) return false;
-        if (p.c == 5 && (p.m > NTT_HIGH_MAX5 || p.s < 5)) return false;
-        if (p.c == 4 && (p.m > NTT_HIGH_MAX4 || p.s < 4)) return false;
-        if (p.c != 0 && p.c != 4 && p.c != 5) return false;
-        s += p.m;
-    }
-    return s == k;
-}
 
 static std::vector<Pass> plan_passes(uint32_t k) {
     {
@@ -568,46 +519,6 @@ __global__ void bit_reverse_kernel(uint32_t* d, uint32_t log_n, uint32_t n) {
         d[i] = d[j];
         d[j] = t;
     }
-}
-
-// Candidate splits of a size-2^k transform for the autotuner: every contiguous
-// pass depth that leaves at most three strided passes, with 64- or 128-byte
-// row segments and the fewest (or one more) balanced strided passes.
-static std::vector<std::vector<Pass>> plan_candidates(uint32_t k) {
-    std::vector<std::vector<Pass>> out;
-    auto add = [&](const std::vector<Pass>& ps) {
-        if (!plan_valid(k, ps)) return;
-        for (const auto& q : out)
-            if (q.size() == ps.size() &&
-                std::equal(q.begin(), q.end(), ps.begin(),
-                           [](const Pass& x, const Pass& y) { return x.m == y.m && x.c == y.c; }))
-                return;
-        out.push_back(ps);
-    };
-    const uint32_t lo = std::max<uint32_t>(k >= 16 ? 8 : 1, k > 3 * NTT_HIGH_MAX4 ? k - 3 * NTT_HIGH_MAX4 : 1);
-    for (uint32_t m0 = lo; m0 <= std::min(k, NTT_LOW_MAX); m0++) {
-        const uint32_t R = k - m0;
-        if (R == 0) {
-            add({{0, m0, 0}});
-            continue;
-        }
-        for (uint32_t c : {4u, 5u}) {
-            const uint32_t hmax = c == 4 ? NTT_HIGH_MAX4 : NTT_HIGH_MAX5;
-            const uint32_t npmin = (R + hmax - 1) / hmax;
-            for (uint32_t np = npmin; np <= npmin + 1 && np <= 7 && np <= R; np++) {
-                std::vector<Pass> ps = {{0, m0, 0}};
-                uint32_t s0 = m0;
-                for (uint32_t i = 0; i < np; i++) {
-                    const uint32_t mi = R / np + (i < R % np ? 1 : 0);
-                    ps.push_back({s0, mi, c});
-                    s0 += mi;
-                }
-                add(ps);
-            }
-        }
-    }
-    add(default_plan(k));
-    return out;
 }
 
 static bool tables_from_abi(const uint32_t* tw, const uint32_t* circ, const uint32_t* h_offs, uint32_t n_line,

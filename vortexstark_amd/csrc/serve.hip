@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "result_ring.hpp"
 
 namespace {
 
@@ -46,53 +47,6 @@ uint32_t class_capacity(uint32_t log_n) {
     if (log_n >= 21) return 2;
     return std::min<uint32_t>(256, std::max<uint32_t>(2, 1u << (22 - log_n)));
 }
-
-// Pinned host memory for finished proofs, allocated in batch-sized regions in
-// ring order; a region is reused once every proof in it has been released.
-struct ResultRing {
-    uint8_t* base = nullptr;
-    uint64_t size = 0;
-    struct Region {
-        uint64_t off, bytes;
-        uint32_t live;
-    };
-    std::deque<Region> regions;  // allocation order
-    uint64_t first_id = 0;       // id of regions.front()
-
-    // returns the region id, or -1 when there is no contiguous room now
-    int64_t alloc(uint64_t bytes, uint32_t live) {
-        if (!bytes || bytes > size) return -1;
-        uint64_t off = 0;
-        if (!regions.empty()) {
-            const uint64_t tail = regions.front().off, head = regions.back().off + regions.back().bytes;
-            if (head > tail) {  // live data in [tail, head)
-                if (head + bytes <= size)
-                    off = head;
-                else if (bytes <= tail)
-                    off = 0;
-                else
-                    return -1;
-            } else {  // wrapped: live data in [tail, size) and [0, head)
-                if (head + bytes <= tail)
-                    off = head;
-                else
-                    return -1;
-            }
-        }
-        regions.push_back({off, bytes, live});
-        return (int64_t)(first_id + regions.size() - 1);
-    }
-    uint8_t* at(int64_t id) const { return base + regions[id - first_id].off; }
-    void release(int64_t id) {
-        if (id < (int64_t)first_id || id >= (int64_t)(first_id + regions.size())) return;
-        Region& r = regions[id - first_id];
-        if (r.live) r.live--;
-        while (!regions.empty() && regions.front().live == 0) {
-            regions.pop_front();
-            first_id++;
-        }
-    }
-};
 
 struct Job {
     uint32_t log_n = 0, n_cols = 0, remaining = 0;
@@ -134,7 +88,7 @@ struct vx_server {
     uint64_t budget = 0, used = 0;
     uint32_t n_q = 80, log_last = 3;
     bool graphs = false;  // batches replay captured graphs
-    ResultRing ring;
+    vx::ResultRing ring;
     std::map<uint32_t, Class> classes;
     std::vector<Job> jobs;
     uint64_t completed = 0, batches = 0, proofs_done = 0, peak_used = 0;
