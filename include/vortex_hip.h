@@ -203,7 +203,12 @@ vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_las
  * equal-size arena of device memory; vx_prover_device_bytes reports the total). */
 vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer, uint32_t capacity);
 uint32_t vx_prover_capacity(const vx_prover* p);
+/* Device bytes the prover allocated (arenas, twiddles, descriptors). */
 uint64_t vx_prover_device_bytes(const vx_prover* p);
+/* Device bytes vx_prover_create_batch would allocate for these parameters (0 if
+ * unsupported); host arithmetic only, no device call. For callers admitting
+ * work against a memory budget; the reference has no counterpart. */
+uint64_t vx_prover_estimate_bytes(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer, uint32_t capacity);
 void vx_prover_destroy(vx_prover* p);
 /* Stream the prover's launches go to (default: its own stream). */
 void* vx_prover_stream(vx_prover* p);
@@ -222,8 +227,9 @@ int32_t vx_prover_prove_batch_device(vx_prover* p, const uint32_t* const* h_trac
                                      uint32_t count);
 /* Batched Fibonacci proofs with the traces generated on the device from (a, b). */
 int32_t vx_prove_fibonacci_batch(vx_prover* p, const uint32_t* h_ab, uint32_t count);
-/* Replay the launch sequence of a batch as one HIP graph (captured on first use
- * per batch count) instead of individual launches. */
+/* Replay the launch sequence of a full batch (count = capacity) as one HIP graph
+ * (captured on first use) instead of individual launches; partial batches are
+ * launched directly. */
 int32_t vx_prover_set_graph(vx_prover* p, int32_t enable);
 /* Device address of the KRAK v1 bytes of the last proof (valid after the stream completes). */
 const void* vx_prover_proof_device(const vx_prover* p);

@@ -69,3 +69,13 @@ def test_graph_replays_match_eager(vx):
     bp.set_graph(False)
     assert bp.prove_batch(ab[:2]) == [O.prove(a, b, log_n, 80, 3) for a, b in ab[:2]]
     bp.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("log_n,nq,last,cap", [(10, 80, 3, 4), (14, 64, 4, 2), (18, 100, 4, 1), (20, 80, 3, 3)])
+def test_device_bytes_equal_estimate(vx, log_n, nq, last, cap):
+    p = vx.Prover(log_n, nq, last, capacity=cap)
+    try:
+        assert p.device_bytes == vx.lib.vx_prover_estimate_bytes(log_n, nq, last, cap)
+    finally:
+        p.close()

@@ -39,3 +39,24 @@ def test_no_device_needed_to_load():
     import vortexstark_amd
 
     assert vortexstark_amd.lib.vx_device_count() >= 0
+
+
+def test_prover_estimate_bytes_host_only():
+    """vx_prover_estimate_bytes is host arithmetic: unsupported parameters give
+    0, the reference defaults apply to zero arguments, and a batch costs one
+    arena plus one 32-byte call slot more per proof."""
+    import vortexstark_amd
+
+    est = vortexstark_amd.lib.vx_prover_estimate_bytes
+    assert est(1, 80, 3, 1) == 0 and est(29, 80, 3, 1) == 0 and est(12, 80, 3, 0) == 0
+    assert est(12, 80, 11, 1) == 0 and est(12, 5000, 3, 1) == 0
+    assert est(12, 0, 0, 1) == est(12, 80, 3, 1)
+    one, two, nine = est(12, 80, 3, 1), est(12, 80, 3, 2), est(12, 80, 3, 9)
+    arena = two - one
+    assert arena > 0 and nine - one == 8 * arena
+    for log_n in range(4, 27):
+        assert est(log_n + 1, 80, 3, 1) > est(log_n, 80, 3, 1)
+    # at least the trace, its LDE and FRI layer 0 of every proof
+    for log_n in (10, 16, 20):
+        E = 4 << log_n
+        assert est(log_n, 80, 3, 2) - est(log_n, 80, 3, 1) >= 4 * (E // 4) + 4 * E + 16 * (E // 2)

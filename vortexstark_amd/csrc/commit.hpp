@@ -7,10 +7,16 @@
 
 namespace vx {
 
-inline uint32_t pick_logc(uint64_t n, uint32_t logc_max = MERKLE_LOGC_MAX) {
+// Chunks per wave (2^c, 128 items each) of a stage over n items per proof for
+// `batch` proofs: the largest c that still leaves 1024 waves in the launch (and
+// at least two per proof). A wave ends with a 64 -> 1 reduction on narrow
+// levels, so larger chunks keep more of its work on full-width levels.
+inline uint32_t pick_logc(uint64_t n, uint32_t logc_max = MERKLE_LOGC_MAX, uint32_t batch = 1) {
     const uint64_t MIN_WAVES = 1024;
-    for (int c = (int)logc_max; c > 0; c--)
-        if ((n >> (7 + c)) >= MIN_WAVES) return (uint32_t)c;
+    for (int c = (int)logc_max; c > 0; c--) {
+        const uint64_t w = n >> (7 + c);
+        if (w >= 2 && w * batch >= MIN_WAVES) return (uint32_t)c;
+    }
     return 0;
 }
 
@@ -35,7 +41,7 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
         mark("tail");
         return;
     }
-    uint32_t logc = pick_logc(n_leaves, StageLogcMax<Src>::value);
+    uint32_t logc = pick_logc(n_leaves, StageLogcMax<Src>::value, batch);
     uint64_t waves = n_leaves >> (7 + logc);
     merkle_stage_kernel<Src><<<dim3(ceil_div(waves, MERKLE_STAGE_WAVES), batch), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
         src, 0, n_leaves, logc, sc.a, st, bs);
@@ -46,7 +52,7 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
     uint32_t* cur = sc.a;
     uint32_t* nxt = sc.b;
     while (n > PROVER_TAIL_ITEMS) {
-        logc = pick_logc(n);
+        logc = pick_logc(n, MERKLE_LOGC_MAX, batch);
         waves = n >> (7 + logc);
         merkle_stage_kernel<SrcNodes><<<dim3(ceil_div(waves, MERKLE_STAGE_WAVES), batch), 64 * MERKLE_STAGE_WAVES,
                                         0, s>>>(SrcNodes{cur}, lvl, n, logc, nxt, st, bs);
@@ -64,7 +70,8 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
 }
 
 // Largest stage output of a tree of n_leaves items whose leaf stage uses at most
-// leaf_logc_max (node stages use the default).
+// leaf_logc_max (node stages use the default). A batch picks chunks at least
+// this large, so its stage outputs are no larger.
 inline uint64_t max_stage_nodes(uint64_t n_leaves, uint32_t leaf_logc_max = MERKLE_LOGC_MAX) {
     if (n_leaves <= PROVER_TAIL_ITEMS) return 0;
     uint64_t mx = 0, n = n_leaves;

@@ -193,19 +193,27 @@ struct SrcCircleFold {
     __device__ __forceinline__ SrcCircleFold at(uint64_t off) const {
         return {t + off, mask, alpha_q + off, alpha_f + off, tw, o0 + off, o1 + off, o2 + off, o3 + off};
     }
+    // With q(i) = aq * d(i), d(i) = t[i+2] - t[i+1] - t[i] an M31 value, the fold
+    // (q0 + q1) + af * ((q0 - q1) * tw) equals aq * (d0 + d1) + (af * aq) * ((d0 - d1) * tw):
+    // nine M31 products per output instead of twenty-one.
+    __device__ __forceinline__ static QM31 fold_d(uint32_t d0, uint32_t d1, QM31 aq, QM31 g, uint32_t tw) {
+        const uint32_t s = madd(d0, d1), u = mmul(msub(d0, d1), tw);
+        QM31 r;
+#pragma unroll
+        for (int k = 0; k < 4; k++) r.v[k] = madd(mmul(aq.v[k], s), mmul(g.v[k], u));
+        return r;
+    }
     __device__ __forceinline__ void pair(uint64_t i, uint32_t h[8]) const {
-        QM31 aq = load_alpha(alpha_q), af = load_alpha(alpha_f);
+        const QM31 aq = load_alpha(alpha_q), g = qmul(load_alpha(alpha_f), aq);
         uint64_t j = 2 * i;  // quotient index of the first of four inputs
         uint2 a = *(const uint2*)(t + j);
         uint2 b = *(const uint2*)(t + ((j + 2) & mask));
         uint2 c = *(const uint2*)(t + ((j + 4) & mask));
-        QM31 q0 = quotient_at(a.x, a.y, b.x, aq);
-        QM31 q1 = quotient_at(a.y, b.x, b.y, aq);
-        QM31 q2 = quotient_at(b.x, b.y, c.x, aq);
-        QM31 q3 = quotient_at(b.y, c.x, c.y, aq);
+        const uint32_t d0 = msub(b.x, madd(a.y, a.x)), d1 = msub(b.y, madd(b.x, a.y));
+        const uint32_t d2 = msub(c.x, madd(b.y, b.x)), d3 = msub(c.y, madd(c.x, b.y));
         uint2 w = *(const uint2*)(tw + i);
-        QM31 f0 = fold_pair(q0, q1, af, w.x);
-        QM31 f1 = fold_pair(q2, q3, af, w.y);
+        QM31 f0 = fold_d(d0, d1, aq, g, w.x);
+        QM31 f1 = fold_d(d2, d3, aq, g, w.y);
         *(uint2*)(o0 + i) = make_uint2(f0.v[0], f1.v[0]);
         *(uint2*)(o1 + i) = make_uint2(f0.v[1], f1.v[1]);
         *(uint2*)(o2 + i) = make_uint2(f0.v[2], f1.v[2]);

@@ -368,6 +368,38 @@ __global__ void coset_points_kernel(CPt initial, CPt step, uint32_t* ox, uint32_
     ox[i] = p.x;
     oy[i] = p.y;
 }
+// Points of a geometric progression: thread t of T visits i = t + k * T for
+// k < PROG_K (i < n), one point product per step after a single power.
+constexpr uint32_t PROG_K = 16;
+template <class F>
+__device__ __forceinline__ void progression(CPt initial, CPt step, uint64_t n, F&& f) {
+    const uint64_t T = (n + PROG_K - 1) / PROG_K;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const CPt stride = cp_pow_mul(CPt{1, 0}, step, T);
+    CPt p = cp_pow_mul(initial, step, t);
+#pragma unroll 4
+    for (uint32_t k = 0; k < PROG_K; k++) {
+        const uint64_t i = t + (uint64_t)k * T;
+        if (i >= n) break;
+        f(i, p);
+        p = cpmul(p, stride);
+    }
+}
+inline uint32_t progression_blocks(uint64_t n, uint32_t threads) {
+    return (uint32_t)ceil_div((n + PROG_K - 1) / PROG_K, threads);
+}
+
+// Fold-twiddle sources of a domain of n = 2^log_n points: out[i] = coordinate
+// of initial * step^(bitrev(2i)). The exponents bitrev(2i) are exactly the
+// j < n/2, so the points are made in natural order and stored to
+// i = bitrev(j) / 2.
+__global__ void fold_sources_pow2_kernel(CPt initial, CPt step, uint32_t* out, uint32_t half_n, uint32_t log_n,
+                                         int use_y) {
+    progression(initial, step, half_n, [&](uint64_t j, CPt p) {
+        out[brev_bits((uint32_t)j, log_n) >> 1] = use_y ? p.y : p.x;
+    });
+}
 __global__ void fold_sources_kernel(CPt initial, CPt step, uint32_t* out, uint32_t half_n, uint32_t log_n,
                                     int use_y) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -388,53 +420,54 @@ __global__ void squash_kernel(const uint32_t* in, uint32_t* tw_out, uint32_t* sq
 // x(initial^(2^l) * step^(i * 2^(2l+1))), i.e. the x-coordinate that the
 // extract-and-squash chain of circle.rs:394-425 produces for that slot
 // (each squash doubles the point and keeps every second one).
-__global__ void line_twiddles_kernel(const CPt* layer_base, const CPt* layer_step, const uint32_t* offs,
-                                     uint32_t n_layers, uint32_t* line, uint64_t total) {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    // find layer: offs ascending
-    uint32_t l = 0;
-    while (l + 1 < n_layers && offs[l + 1] <= g) l++;
-    uint32_t i = (uint32_t)(g - offs[l]);
-    CPt p = cp_pow_mul(layer_base[l], layer_step[l], i);
-    line[g] = p.x;
+// Per-layer starting point, step and word offset of the line twiddles, passed
+// by value (at most 31 layers).
+struct LineTwiddleMeta {
+    CPt base[32];
+    CPt step[32];
+    uint32_t offs[32];
+};
+__global__ void line_twiddles_kernel(LineTwiddleMeta m, uint32_t* line, uint64_t n) {
+    const uint32_t l = blockIdx.y;  // layer l holds n / 2^(l+1) words
+    const uint64_t len = n >> (l + 1);
+    uint32_t* dst = line + m.offs[l];
+    progression(m.base[l], m.step[l], len, [&](uint64_t i, CPt p) { dst[i] = p.x; });
 }
 __global__ void circle_twiddles_kernel(CPt initial, CPt step, uint32_t* circle, uint32_t half_n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= half_n) return;
-    circle[i] = cp_pow_mul(initial, step, i).y;
+    progression(initial, step, half_n, [&](uint64_t i, CPt p) { circle[i] = p.y; });
 }
 
-// Montgomery batch inverse over 64-word chunks, one chunk per thread, with the
-// zero conventions of cuda/batch_inverse.cu:8-41 (zeros map to zero).
-__global__ void batch_inverse_kernel(const uint32_t* in, uint32_t* out, uint64_t n) {
-    const uint32_t CH = 64;
-    uint64_t start = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * CH;
-    if (start >= n) return;
-    uint32_t count = (uint32_t)(n - start < CH ? n - start : CH);
-    uint32_t prefix[64];
-    prefix[0] = in[start];
-    for (uint32_t i = 1; i < count; i++) {
-        uint32_t v = in[start + i];
-        prefix[i] = v == 0 ? prefix[i - 1] : mmul(prefix[i - 1], v);
+// Montgomery batch inverse with the zero conventions of
+// cuda/batch_inverse.cu:8-41 (zeros map to zero, and are skipped in the
+// running product). Thread t of T inverts the BI_K words t + k * T, so each
+// load and store of a wave is contiguous and the prefix products stay in
+// registers; the grouping does not change any output.
+constexpr uint32_t BI_K = 32;
+__global__ __launch_bounds__(128) void batch_inverse_kernel(const uint32_t* in, uint32_t* out, uint64_t n) {
+    const uint64_t T = (n + BI_K - 1) / BI_K;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    uint32_t v[BI_K], pre[BI_K];
+    uint32_t acc = 1;
+#pragma unroll
+    for (uint32_t k = 0; k < BI_K; k++) {
+        const uint64_t i = t + (uint64_t)k * T;
+        v[k] = i < n ? in[i] : 0;
+        pre[k] = acc;  // product of the non-zero v[0 .. k-1]
+        if (v[k]) acc = mmul(acc, v[k]);
     }
-    uint32_t iv = minv(prefix[count - 1]);
-    for (uint32_t i = count - 1; i > 0; i--) {
-        uint32_t v = in[start + i];
-        if (v == 0) {
-            out[start + i] = 0;
-        } else {
-            out[start + i] = mmul(iv, prefix[i - 1]);
-            iv = mmul(iv, v);
-        }
+    uint32_t iv = minv(acc);  // 1 / product of all non-zero v
+#pragma unroll
+    for (int k = BI_K - 1; k >= 0; k--) {
+        const uint64_t i = t + (uint64_t)k * T;
+        if (i < n) out[i] = v[k] ? mmul(iv, pre[k]) : 0;
+        if (v[k]) iv = mmul(iv, v[k]);
     }
-    out[start] = in[start] == 0 ? 0 : iv;
 }
 
 void batch_inverse(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t s) {
     if (!n) return;
-    uint64_t chunks = (n + 63) / 64;
-    batch_inverse_kernel<<<ceil_div(chunks, 128), 128, 0, s>>>(in, out, n);
+    batch_inverse_kernel<<<ceil_div((n + BI_K - 1) / BI_K, 128), 128, 0, s>>>(in, out, n);
     check_launch("batch_inverse_kernel");
 }
 
@@ -444,41 +477,29 @@ void build_ntt_tables(CPt initial, CPt step, uint32_t log_n, uint32_t* line, uin
     uint32_t n_line = log_n > 0 ? log_n - 1 : 0;
     for (int i = 0; i < 32; i++) offs[i] = 0;
     uint64_t off = 0, cur = n;
-    std::vector<CPt> bases(n_line ? n_line : 1), steps(n_line ? n_line : 1);
-    std::vector<uint32_t> hoffs(n_line ? n_line : 1, 0);
+    LineTwiddleMeta meta = {};
     HostPt hi = {initial.x, initial.y}, hs = {step.x, step.y};
     for (uint32_t l = 0; l < n_line; l++) {
         offs[l] = (uint32_t)off;
-        hoffs[l] = (uint32_t)off;
+        meta.offs[l] = (uint32_t)off;
         HostPt b = host_repeated_double(hi, l);                 // initial^(2^l)
         HostPt st = host_pow(hs, 1ull << (2 * l + 1));          // step^(2^(2l+1))
-        bases[l] = {b.x, b.y};
-        steps[l] = {st.x, st.y};
+        meta.base[l] = {b.x, b.y};
+        meta.step[l] = {st.x, st.y};
         off += cur / 2;
         cur /= 2;
     }
     if (n_line) {
         // n - 1 line words in total; the last entry of each layer is unused by
         // the transform but is part of the reference layout.
-        CPt* d_meta = nullptr;
-        uint32_t* d_offs = nullptr;
-        check(hipMalloc((void**)&d_meta, sizeof(CPt) * 2 * n_line), "hipMalloc");
-        check(hipMalloc((void**)&d_offs, 4 * n_line), "hipMalloc");
-        std::vector<CPt> meta(bases);
-        meta.insert(meta.end(), steps.begin(), steps.end());
-        check(hipMemcpyAsync(d_meta, meta.data(), sizeof(CPt) * 2 * n_line, hipMemcpyHostToDevice, s), "H2D");
-        check(hipMemcpyAsync(d_offs, hoffs.data(), 4 * n_line, hipMemcpyHostToDevice, s), "H2D");
-        line_twiddles_kernel<<<ceil_div(off, 256), 256, 0, s>>>(d_meta, d_meta + n_line, d_offs, n_line, line, off);
+        line_twiddles_kernel<<<dim3(progression_blocks(n / 2, 256), n_line), 256, 0, s>>>(meta, line, n);
         check_launch("line_twiddles_kernel");
         // Word n-2 of the n-1 word layout belongs to no layer; keep it defined (zero)
         // so the batch inverse below never reads an undefined value.
         check(hipMemsetAsync(line + off, 0, 4 * (n - 1 - off), s), "hipMemsetAsync");
-        check(hipStreamSynchronize(s), "sync");
-        check(hipFree(d_meta), "hipFree");
-        check(hipFree(d_offs), "hipFree");
     }
     if (n >= 2) {
-        circle_twiddles_kernel<<<ceil_div(n / 2, 256), 256, 0, s>>>(initial, step, circle, (uint32_t)(n / 2));
+        circle_twiddles_kernel<<<progression_blocks(n / 2, 256), 256, 0, s>>>(initial, step, circle, (uint32_t)(n / 2));
         check_launch("circle_twiddles_kernel");
     }
     if (iline && n_line) batch_inverse(line, iline, n - 1, s);
@@ -489,8 +510,9 @@ void build_fold_twiddles(CPt initial, CPt step, uint32_t log_n, bool use_y, uint
                          hipStream_t s) {
     uint32_t half = (uint32_t)((1ull << log_n) / 2);
     if (!half) return;
-    fold_sources_kernel<<<ceil_div(half, 256), 256, 0, s>>>(initial, step, scratch, half, log_n, use_y ? 1 : 0);
-    check_launch("fold_sources_kernel");
+    fold_sources_pow2_kernel<<<progression_blocks(half, 256), 256, 0, s>>>(initial, step, scratch, half, log_n,
+                                                                           use_y ? 1 : 0);
+    check_launch("fold_sources_pow2_kernel");
     batch_inverse(scratch, out, half, s);
 }
 

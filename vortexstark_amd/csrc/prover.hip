@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void queries_kernel(DevChannel* ch0, const uin
 // ---------------------------------------------------------------- decommitment
 // prover.rs:1184-1247 and 2171-2234: value at the query index and at its
 // sibling (index ^ 1), with both authentication paths (siblings from the leaf
-// level up, root excluded). One wave per (query, tree, proof); tree
+// level up, root excluded). Grid: (query group, tree, proof); tree
 // descriptors address proof 0 and move by blockIdx.z * bs words.
 struct TreeDesc {
     uint32_t kind;   // 0: one M31 column, 1: Fibonacci quotient of the LDE, 2: SoA4
@@ -149,74 +149,119 @@ __device__ void tree_leaf(const TreeDesc& t, uint64_t off, uint64_t i, uint32_t 
     }
 }
 
-constexpr uint32_t DECOMMIT_SUB = 5;  // recomputed subtree: 32 leaves
-__global__ __launch_bounds__(64) void decommit_kernel(const TreeDesc* trees, const uint32_t* queries0,
-                                                      uint32_t* proof0, uint64_t bs) {
-    __shared__ uint32_t lv[8 * 32];    // current level of the subtree, word-major
-    __shared__ uint32_t path[8 * 32];  // sibling of the query's ancestor per level
-    __shared__ uint32_t leaf_self[8];  // leaf hash of the query index itself
+constexpr uint32_t DECOMMIT_SUB = 5;                                    // recomputed subtree: up to 32 leaves
+constexpr uint32_t DECOMMIT_GROUP = 8;                                  // queries of one tree per workgroup
+constexpr uint32_t DECOMMIT_THREADS = DECOMMIT_GROUP << DECOMMIT_SUB;  // one thread per subtree leaf
+constexpr uint32_t DECOMMIT_MAX_DEPTH = 32;
+
+// Lowest level a tree of depth d keeps in its level store. Trees of up to 2^16
+// leaves keep every level from 1 up (a query's path is read, only its two
+// leaves are hashed); deeper trees keep 2^16 nodes and more until the lowest
+// stored level reaches DECOMMIT_SUB, where a query rebuilds a 32-leaf subtree.
+constexpr uint32_t STORE_FULL_DEPTH = 16;
+inline uint32_t store_from(uint32_t d) {
+    const uint32_t f = d > STORE_FULL_DEPTH ? d - STORE_FULL_DEPTH + 1 : 1;
+    return std::min(std::min(f, DECOMMIT_SUB), d);
+}
+
+// Workgroup = DECOMMIT_GROUP queries of one tree of one proof. The subtrees
+// below the level store are rebuilt together: every level's nodes of all the
+// group's queries are packed into the lowest threads, so the waves that hash
+// are full and the others wait at the barrier (a single query per wave would
+// leave 31 of 63 compressions on partly idle waves).
+__global__ __launch_bounds__(DECOMMIT_THREADS) void decommit_kernel(const TreeDesc* trees, const uint32_t* queries0,
+                                                                     uint32_t* proof0, uint32_t n_q, uint64_t bs) {
+    constexpr uint32_t S = 1u << DECOMMIT_SUB;
+    constexpr uint32_t NW = DECOMMIT_THREADS;
+    __shared__ uint32_t lv[8 * NW];  // word w of node k of query g's subtree level at w * NW + g * S + k
+    __shared__ uint32_t path[DECOMMIT_GROUP][8 * DECOMMIT_MAX_DEPTH];  // sibling of the ancestor per level
+    __shared__ uint32_t leaf_self[DECOMMIT_GROUP][8];                   // leaf hash of the query index itself
+    __shared__ uint64_t li_s[DECOMMIT_GROUP];
     const TreeDesc t = trees[blockIdx.y];
     const uint64_t off = (uint64_t)blockIdx.z * bs;
     const uint32_t* queries = queries0 + off;
     uint32_t* proof = proof0 + off;
-    const uint32_t q = blockIdx.x, lane = threadIdx.x;
-    const uint64_t li = (uint64_t)queries[q] >> t.shift;
+    const uint32_t tid = threadIdx.x;
     const uint32_t D = t.depth;
     const uint32_t lam = t.from < D ? t.from : D;
+    const uint32_t lmask = (1u << lam) - 1;
     const uint32_t width = t.kind == 0 ? 1 : 4;
-    const uint64_t sub_base = li & ~((1ull << lam) - 1);
+    const uint32_t q0 = blockIdx.x * DECOMMIT_GROUP;
+    const uint32_t nq = n_q - q0 < DECOMMIT_GROUP ? n_q - q0 : DECOMMIT_GROUP;
 
-    if (lane == 0) {
-        uint32_t v[4], s[4];
+    if (tid < nq) {
+        const uint32_t q = q0 + tid;
+        const uint64_t li = (uint64_t)queries[q] >> t.shift;
+        li_s[tid] = li;
+        uint32_t v[4], sv[4];
         tree_values(t, off, li, v);
-        tree_values(t, off, li ^ 1, s);
+        tree_values(t, off, li ^ 1, sv);
         for (uint32_t k = 0; k < width; k++) {
             proof[t.val_off + (uint64_t)q * width + k] = v[k];
-            proof[t.sib_off + (uint64_t)q * width + k] = s[k];
+            proof[t.sib_off + (uint64_t)q * width + k] = sv[k];
         }
-    }
-    uint32_t cnt = 1u << lam;
-    if (lane < cnt) {
-        uint32_t h[8];
-        tree_leaf(t, off, sub_base + lane, h);
-        for (int w = 0; w < 8; w++) lv[w * 32 + lane] = h[w];
     }
     __syncthreads();
-    uint32_t pos = (uint32_t)(li - sub_base);
-    if (lane < 8) leaf_self[lane] = lv[lane * 32 + pos];
-    for (uint32_t L = 0; L < lam; L++) {
-        if (lane < 8) path[L * 8 + lane] = lv[lane * 32 + (pos ^ 1)];
-        uint32_t half = cnt / 2, l[8], r[8];
-        if (lane < half)
-            for (int w = 0; w < 8; w++) {
-                l[w] = lv[w * 32 + 2 * lane];
-                r[w] = lv[w * 32 + 2 * lane + 1];
-            }
-        __syncthreads();
-        if (lane < half) {
+    {
+        const uint32_t g = tid >> lam, k = tid & lmask;
+        if (g < nq) {
             uint32_t h[8];
-            b2s_node(l, r, h);
-            for (int w = 0; w < 8; w++) lv[w * 32 + lane] = h[w];
+            tree_leaf(t, off, (li_s[g] & ~(uint64_t)lmask) + k, h);
+#pragma unroll
+            for (int w = 0; w < 8; w++) lv[w * NW + g * S + k] = h[w];
+        }
+    }
+    __syncthreads();
+    if (tid < 8 * nq) {
+        const uint32_t g = tid >> 3, w = tid & 7;
+        leaf_self[g][w] = lv[w * NW + g * S + ((uint32_t)li_s[g] & lmask)];
+    }
+    for (uint32_t L = 0; L < lam; L++) {
+        if (tid < 8 * nq) {
+            const uint32_t g = tid >> 3, w = tid & 7;
+            const uint32_t pos = ((uint32_t)li_s[g] & lmask) >> L;
+            path[g][L * 8 + w] = lv[w * NW + g * S + (pos ^ 1)];
+        }
+        if (L + 1 == lam) break;  // level lam comes from the store
+        const uint32_t hl = lam - L - 1;  // log2 of each query's nodes on level L + 1
+        const uint32_t g = tid >> hl, k = tid & ((1u << hl) - 1);
+        const bool act = g < nq;
+        uint32_t l[8], r[8];
+        if (act) {
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                l[w] = lv[w * NW + g * S + 2 * k];
+                r[w] = lv[w * NW + g * S + 2 * k + 1];
+            }
         }
         __syncthreads();
-        cnt = half;
-        pos >>= 1;
+        if (act) {
+            uint32_t h[8];
+            b2s_node(l, r, h);
+#pragma unroll
+            for (int w = 0; w < 8; w++) lv[w * NW + g * S + k] = h[w];
+        }
+        __syncthreads();
     }
     const LevelStore st = LevelStore{t.store, D, lam}.shifted(off);
-    for (uint32_t L = lam; L < D; L++) {
-        if (lane < 8) path[L * 8 + lane] = st.at(L, ((li >> L) ^ 1))[lane];
+    const uint32_t up = D - lam;
+    for (uint32_t idx = tid; idx < nq * up * 8; idx += DECOMMIT_THREADS) {
+        const uint32_t w = idx & 7, L = lam + (idx >> 3) % up, g = (idx >> 3) / up;
+        path[g][L * 8 + w] = st.at(L, (li_s[g] >> L) ^ 1)[w];
     }
     __syncthreads();
     const uint64_t rec = 1 + 8ull * D;
-    uint32_t* p0 = proof + t.path_off + (uint64_t)q * rec;
-    uint32_t* p1 = proof + t.sib_path_off + (uint64_t)q * rec;
-    if (lane == 0) {
-        p0[0] = D;
-        p1[0] = D;
-    }
-    for (uint32_t k = lane; k < 8 * D; k += 64) {
-        p0[1 + k] = path[k];
-        p1[1 + k] = k < 8 ? leaf_self[k] : path[k];
+    for (uint32_t idx = tid; idx < nq * (1 + 8 * D); idx += DECOMMIT_THREADS) {
+        const uint32_t g = idx / (1 + 8 * D), k = idx % (1 + 8 * D);
+        uint32_t* p0 = proof + t.path_off + (uint64_t)(q0 + g) * rec;
+        uint32_t* p1 = proof + t.sib_path_off + (uint64_t)(q0 + g) * rec;
+        if (k == 0) {
+            p0[0] = D;
+            p1[0] = D;
+        } else {
+            p0[k] = path[g][k - 1];
+            p1[k] = k - 1 < 8 ? leaf_self[g][k - 1] : path[g][k - 1];
+        }
     }
 }
 
@@ -238,6 +283,27 @@ __global__ void fib_trace_batch_kernel(const uint32_t* ab, uint64_t n, uint32_t 
     }
 }
 
+// Copies words of src to count - 1 places stride words apart (dst0 = src + stride).
+__global__ void replicate_kernel(const uint32_t* src, uint64_t words, uint64_t stride) {
+    uint32_t* d = const_cast<uint32_t*>(src) + (uint64_t)(blockIdx.y + 1) * stride;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        d[i] = src[i];
+}
+
+// Proofs 0 .. gridDim.y-1 (words each, R words apart) and their transcript
+// failure flags, written by the device straight into pinned host memory: the
+// copy stays on the prover's compute queue, with no copy-engine hand-off.
+__global__ void proofs_to_host_kernel(const uint32_t* src, uint64_t R, uint64_t words, uint32_t* dst,
+                                      const uint32_t* failed_src, uint32_t* failed_dst) {
+    const uint32_t y = blockIdx.y;
+    const uint32_t* s = src + (uint64_t)y * R;
+    uint32_t* d = dst + (uint64_t)y * words;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+    if (failed_dst && blockIdx.x == 0 && threadIdx.x == 0) failed_dst[y] = failed_src[(uint64_t)y * R];
+    __threadfence_system();
+}
+
 }  // namespace vx
 
 // ---------------------------------------------------------------- prover object
@@ -253,6 +319,7 @@ struct vx_prover {
     uint64_t N = 0, E = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
+    uint64_t alloc_bytes = 0;  // device bytes of allocs
     NttTables trace_tab, eval_tab;
     std::vector<uint32_t*> fri_tw;
     // arena layout (word offsets inside one proof's arena)
@@ -269,11 +336,12 @@ struct vx_prover {
     // call parameters: trace pointers (capacity) then (a, b) per proof
     uint64_t* d_call = nullptr;
     uint64_t* h_call = nullptr;  // pinned staging
+    uint8_t* h_setup = nullptr;  // pinned staging of the set-up copies (kept until destroy)
     std::vector<uint64_t> last_call;
     bool traces_aligned = false;  // every trace pointer of the current call is 16-byte aligned
-    // graph replay of the launch sequence: one instantiated graph per batch
-    // count, the most recent GRAPH_CACHE counts kept
-    static constexpr size_t GRAPH_CACHE = 8;
+    // graph replay of the launch sequence of a full batch: one instantiated
+    // graph per trace alignment
+    static constexpr size_t GRAPH_CACHE = 2;
     struct Captured {
         uint32_t count;
         bool aligned;  // the captured launches move trace tiles as uint4
@@ -296,6 +364,7 @@ struct vx_prover {
     bool alloc(void** p, uint64_t bytes) {
         if (!check(hipMalloc(p, bytes ? bytes : 4), "vx_prover: hipMalloc")) return false;
         allocs.push_back(*p);
+        alloc_bytes += bytes;
         return true;
     }
     static void destroy(Captured& c) {
@@ -311,6 +380,7 @@ struct vx_prover {
         drop_graphs();
         for (void* p : allocs) (void)hipFree(p);
         if (h_call) (void)hipHostFree(h_call);
+        if (h_setup) (void)hipHostFree(h_setup);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -380,8 +450,8 @@ void plan_arena(vx_prover* p) {
     p->stores.clear();
     auto depth_of = [&](uint32_t t) { return t < 2 ? p->log_e : p->log_e - 1 - (t - 2); };
     for (uint32_t t = 0; t < 2 + p->J; t++) {
-        uint32_t d = depth_of(t), from = DECOMMIT_SUB < d ? DECOMMIT_SUB : d;
-        p->o_store.push_back(take(level_store_words(d, from)));
+        const uint32_t d = depth_of(t);
+        p->o_store.push_back(take(level_store_words(d, store_from(d))));
     }
     const uint64_t mx = max_stage_nodes(p->E);
     p->o_scr_a = take(8 * (mx ? mx : 1));
@@ -457,7 +527,8 @@ void enqueue_prove(vx_prover* p, uint32_t count) {
                                          p->at(p->o_queries), p->ab_dev(), R);
     check_launch("queries_kernel");
     p->mark("queries");
-    decommit_kernel<<<dim3(p->n_q, 2 + p->J, count), 64, 0, s>>>(p->d_trees, p->at(p->o_queries), pr, R);
+    decommit_kernel<<<dim3((p->n_q + DECOMMIT_GROUP - 1) / DECOMMIT_GROUP, 2 + p->J, count), DECOMMIT_THREADS, 0, s>>>(
+        p->d_trees, p->at(p->o_queries), pr, p->n_q, R);
     check_launch("decommit_kernel");
     p->mark("decommit");
 }
@@ -483,7 +554,10 @@ bool upload_call(vx_prover* p, const uint32_t* const* traces, const uint32_t* ab
 }
 
 int32_t run_batch(vx_prover* p, uint32_t count) {
-    if (!p->use_graph) {
+    // Graphs replay full batches only: a partial batch (any other count) would
+    // need a graph of its own, and capturing and retiring graphs for the many
+    // counts a server sees costs more host time than launching directly.
+    if (!p->use_graph || count != p->capacity) {
         p->tl.n = 0;
         enqueue_prove(p, count);
         return peek_error() == 0 ? 0 : -1;
@@ -517,19 +591,16 @@ int32_t run_batch(vx_prover* p, uint32_t count) {
     return check(hipGraphLaunch(G.back().exec, p->stream), "hipGraphLaunch") ? 0 : -1;
 }
 
-}  // namespace
-
-extern "C" {
-
-vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer, uint32_t capacity) {
-    if (n_queries == 0) n_queries = 80;       // prover.rs:90
+// Applies the reference defaults (80 queries, last layer 2^3) for zero
+// arguments and checks the parameters a prover supports.
+bool prover_params(uint32_t log_n, uint32_t& n_queries, uint32_t& log_last_layer, uint32_t capacity) {
+    if (n_queries == 0) n_queries = 80;           // prover.rs:90
     if (log_last_layer == 0) log_last_layer = 3;  // fri.rs:265
-    if (log_n < 2 || log_n + 2 > 30 || log_last_layer > log_n + 1 || log_last_layer > 10 || n_queries > 4096 ||
-        capacity < 1 || capacity > 65535) {
-        set_error(-2, "vx_prover_create: unsupported parameters");
-        return nullptr;
-    }
-    vx_prover* p = new vx_prover();
+    return !(log_n < 2 || log_n + 2 > 30 || log_last_layer > log_n + 1 || log_last_layer > 10 || n_queries > 4096 ||
+             capacity < 1 || capacity > 65535);
+}
+
+void set_sizes(vx_prover* p, uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer, uint32_t capacity) {
     p->log_n = log_n;
     p->log_e = log_n + 2;  // BLOWUP_BITS = 2 (prover.rs:86)
     p->n_q = n_queries;
@@ -538,6 +609,41 @@ vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t l
     p->N = 1ull << log_n;
     p->E = 1ull << p->log_e;
     p->J = p->log_e - log_last_layer;
+}
+
+// Device bytes vx_prover_create_batch allocates besides the arenas: NTT
+// twiddles of the trace and evaluation domains, fold twiddles and their
+// scratch, tree descriptors and per-proof call slots.
+uint64_t shared_bytes(const vx_prover* p) {
+    const uint64_t N = p->N, E = p->E;
+    uint64_t b = 2 * 4 * (N - 1) + 2 * 4 * (N / 2) + 4 * (E - 1) + 4 * (E / 2) + 4 * (E / 2);
+    for (uint32_t j = 0; j < p->J; j++) b += 4 * ((1ull << (p->log_e - j)) / 2);
+    return b + sizeof(TreeDesc) * (2 + p->J) + (8 + 8 + 16) * (uint64_t)p->capacity;
+}
+
+// Device address of host memory the device may write directly (allocated by
+// hipHostMalloc), or null for any other memory.
+void* device_view_of_host(void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (a.type != hipMemoryTypeHost || !a.devicePointer || !a.hostPointer) return nullptr;
+    return (char*)a.devicePointer + ((char*)p - (char*)a.hostPointer);
+}
+
+}  // namespace
+
+extern "C" {
+
+vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer, uint32_t capacity) {
+    if (!prover_params(log_n, n_queries, log_last_layer, capacity)) {
+        set_error(-2, "vx_prover_create: unsupported parameters");
+        return nullptr;
+    }
+    vx_prover* p = new vx_prover();
+    set_sizes(p, log_n, n_queries, log_last_layer, capacity);
     const uint64_t N = p->N, E = p->E;
     bool ok = check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
     hipStream_t s = p->stream;
@@ -591,14 +697,9 @@ vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t l
     ok = ok && check(hipHostMalloc((void**)&p->h_call, 16ull * capacity), "hipHostMalloc");
     if (ok) {
         for (uint32_t t = 0; t < 2 + p->J; t++) {
-            uint32_t d = t < 2 ? p->log_e : p->log_e - 1 - (t - 2);
-            uint32_t from = DECOMMIT_SUB < d ? DECOMMIT_SUB : d;
-            p->stores.push_back(LevelStore{p->at(p->o_store[t]), d, from});
+            const uint32_t d = t < 2 ? p->log_e : p->log_e - 1 - (t - 2);
+            p->stores.push_back(LevelStore{p->at(p->o_store[t]), d, store_from(d)});
         }
-        // KRAK template into every proof slot
-        for (uint32_t i = 0; ok && i < capacity; i++)
-            ok = check(hipMemcpyAsync(p->proof(i), L.words.data(), 4 * p->proof_words, hipMemcpyHostToDevice, s),
-                       "H2D");
         std::vector<TreeDesc>& T = L.trees;
         uint32_t* ev = p->at(p->o_eval);
         T[0].kind = 0;
@@ -622,17 +723,33 @@ vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t l
             T[i].from = p->stores[i].from;
             T[i].store = p->stores[i].base;
         }
-        std::vector<uint64_t> evp(capacity), slots(capacity);
-        for (uint32_t i = 0; i < capacity; i++) {
-            evp[i] = (uint64_t)(uintptr_t)(p->arena + (uint64_t)i * p->R + p->o_eval);
-            slots[i] = (uint64_t)(uintptr_t)(p->arena + (uint64_t)i * p->R + p->o_trace);
+        // Set-up copies go through pinned staging owned by the prover, so they are
+        // asynchronous and nothing here waits for the device (a new prover must not
+        // stall the caller behind work of other streams); the prover's first
+        // launch is ordered after them on its stream.
+        const uint64_t tb = sizeof(TreeDesc) * T.size(), wb = 4 * p->proof_words, cb = 8ull * capacity;
+        ok = ok && check(hipHostMalloc((void**)&p->h_setup, tb + wb + 2 * cb), "hipHostMalloc");
+        if (ok) {
+            uint8_t* h = p->h_setup;
+            std::memcpy(h, T.data(), tb);
+            std::memcpy(h + tb, L.words.data(), wb);
+            uint64_t* evp = (uint64_t*)(h + tb + wb);
+            uint64_t* slots = evp + capacity;
+            for (uint32_t i = 0; i < capacity; i++) {
+                evp[i] = (uint64_t)(uintptr_t)(p->arena + (uint64_t)i * p->R + p->o_eval);
+                slots[i] = (uint64_t)(uintptr_t)(p->arena + (uint64_t)i * p->R + p->o_trace);
+            }
+            ok = check(hipMemcpyAsync(p->d_trees, h, tb, hipMemcpyHostToDevice, s), "H2D") &&
+                 check(hipMemcpyAsync(p->proof(0), h + tb, wb, hipMemcpyHostToDevice, s), "H2D") &&
+                 check(hipMemcpyAsync(p->d_eval_ptrs, evp, cb, hipMemcpyHostToDevice, s), "H2D") &&
+                 check(hipMemcpyAsync(p->d_slot_ptrs, slots, cb, hipMemcpyHostToDevice, s), "H2D");
         }
-        ok = ok && check(hipMemcpyAsync(p->d_trees, T.data(), sizeof(TreeDesc) * T.size(), hipMemcpyHostToDevice, s),
-                         "H2D");
-        ok = ok && check(hipMemcpyAsync(p->d_eval_ptrs, evp.data(), 8ull * capacity, hipMemcpyHostToDevice, s), "H2D");
-        ok = ok && check(hipMemcpyAsync(p->d_slot_ptrs, slots.data(), 8ull * capacity, hipMemcpyHostToDevice, s),
-                         "H2D");
-        ok = ok && check(hipStreamSynchronize(s), "vx_prover_create");
+        // KRAK template into every other proof slot
+        if (ok && capacity > 1) {
+            const uint32_t gx = (uint32_t)std::min<uint64_t>(64, (p->proof_words + 255) / 256);
+            replicate_kernel<<<dim3(gx, capacity - 1), 256, 0, s>>>(p->proof(0), p->proof_words, p->R);
+            ok = check_launch("replicate_kernel");
+        }
     }
     if (!ok) {
         delete p;
@@ -649,7 +766,17 @@ void vx_prover_destroy(vx_prover* p) { delete p; }
 void* vx_prover_stream(vx_prover* p) { return p ? (void*)p->stream : nullptr; }
 uint32_t vx_prover_capacity(const vx_prover* p) { return p ? p->capacity : 0; }
 uint64_t vx_prover_proof_size(const vx_prover* p) { return p ? 4 * p->proof_words : 0; }
-uint64_t vx_prover_device_bytes(const vx_prover* p) { return p ? 4 * p->R * p->capacity : 0; }
+uint64_t vx_prover_device_bytes(const vx_prover* p) { return p ? p->alloc_bytes : 0; }
+uint64_t vx_prover_estimate_bytes(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer, uint32_t capacity) {
+    if (!prover_params(log_n, n_queries, log_last_layer, capacity)) return 0;
+    vx_prover p;
+    set_sizes(&p, log_n, n_queries, log_last_layer, capacity);
+    Layout L;
+    build_layout(&p, L);
+    p.proof_words = L.words.size();
+    plan_arena(&p);
+    return 4 * p.R * capacity + shared_bytes(&p);
+}
 const void* vx_prover_proof_device(const vx_prover* p) { return p ? p->proof(0) : nullptr; }
 const void* vx_prover_proof_device_at(const vx_prover* p, uint32_t index) {
     return (p && index < p->capacity) ? p->proof(index) : nullptr;
@@ -729,7 +856,15 @@ int32_t vx_prover_enqueue_copy(vx_prover* p, uint32_t count, void* host_proofs, 
     }
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
     const uint64_t pb = 4 * p->proof_words, pitch = 4 * p->R;
-    // one strided copy per field: proof i sits i * R words after proof 0
+    uint32_t* dev_proofs = (uint32_t*)device_view_of_host(host_proofs);
+    uint32_t* dev_failed = host_failed ? (uint32_t*)device_view_of_host(host_failed) : nullptr;
+    if (dev_proofs && ((uintptr_t)dev_proofs & 3) == 0 && (!host_failed || (dev_failed && ((uintptr_t)dev_failed & 3) == 0))) {
+        const uint32_t gx = (uint32_t)std::min<uint64_t>(16, (p->proof_words + 255) / 256);
+        proofs_to_host_kernel<<<dim3(gx, count), 256, 0, s>>>(p->proof(0), p->R, p->proof_words, dev_proofs,
+                                                              &p->ch()->failed, dev_failed);
+        return check_launch("proofs_to_host_kernel") ? 0 : -1;
+    }
+    // other host memory: one strided DMA copy per field (proof i sits i * R words after proof 0)
     if (!check(hipMemcpy2DAsync(host_proofs, pb, p->proof(0), pitch, pb, count, hipMemcpyDeviceToHost, s), "D2H"))
         return -1;
     if (host_failed &&

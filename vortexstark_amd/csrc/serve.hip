@@ -41,11 +41,13 @@ double now_s() {
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// proofs per batch: about 2^24 LDE rows per launch (the stage kernels then run
-// thousands of waves), at least 2 and at most 256
+// proofs per batch: about 2^26 LDE rows per launch (a batch then spends most
+// of its time in stage kernels of thousands of waves, and the fixed cost of a
+// batch's narrow tree levels and FRI tail is shared by many proofs), at least
+// 2 and at most 1024
 uint32_t class_capacity(uint32_t log_n) {
-    if (log_n >= 21) return 2;
-    return std::min<uint32_t>(256, std::max<uint32_t>(2, 1u << (22 - log_n)));
+    if (log_n >= 23) return 2;
+    return std::min<uint32_t>(1024, std::max<uint32_t>(2, 1u << (24 - log_n)));
 }
 
 struct Job {
@@ -124,11 +126,9 @@ struct vx_server {
         if (!ok) release(s);
         return ok;
     }
-    // device bytes of a prover of this class, estimated before creating it
-    // (arena ~ 27 E + 4 N bytes per proof plus ~16 E of shared twiddles)
-    static uint64_t estimate_bytes(uint32_t log_n) {
-        const uint64_t E = 1ull << (log_n + 2);
-        return (uint64_t)class_capacity(log_n) * (28 * E + (4ull << log_n)) + (16 * E);
+    // device bytes of a prover of this class, known before creating it
+    uint64_t estimate_bytes(uint32_t log_n) const {
+        return vx_prover_estimate_bytes(log_n, n_q, log_last, class_capacity(log_n));
     }
     bool reserve_room(uint64_t need, uint32_t for_class) {
         while (used + need > budget) {
@@ -238,7 +238,9 @@ struct vx_server {
             }
         }
         bool ring_full = false;
-        for (auto& kv : classes) {
+        // largest proofs first: their batches are the longest, so they start first
+        for (auto it = classes.rbegin(); it != classes.rend(); ++it) {
+            auto& kv = *it;
             Class& c = kv.second;
             if (c.pending.empty()) continue;
             for (auto& s : c.slots)
