@@ -227,9 +227,9 @@ int32_t vx_prover_prove_batch_device(vx_prover* p, const uint32_t* const* h_trac
                                      uint32_t count);
 /* Batched Fibonacci proofs with the traces generated on the device from (a, b). */
 int32_t vx_prove_fibonacci_batch(vx_prover* p, const uint32_t* h_ab, uint32_t count);
-/* Replay the launch sequence of a full batch (count = capacity) as one HIP graph
- * (captured on first use) instead of individual launches; partial batches are
- * launched directly. */
+/* Replay the launch sequence of a batch as one HIP graph (captured on first use
+ * of each batch count, up to 32 counts per prover; further counts are launched
+ * directly) instead of individual launches. */
 int32_t vx_prover_set_graph(vx_prover* p, int32_t enable);
 /* Device address of the KRAK v1 bytes of the last proof (valid after the stream completes). */
 const void* vx_prover_proof_device(const vx_prover* p);

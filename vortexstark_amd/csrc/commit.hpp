@@ -8,11 +8,13 @@
 namespace vx {
 
 // Chunks per wave (2^c, 128 items each) of a stage over n items per proof for
-// `batch` proofs: the largest c that still leaves 1024 waves in the launch (and
-// at least two per proof). A wave ends with a 64 -> 1 reduction on narrow
-// levels, so larger chunks keep more of its work on full-width levels.
+// `batch` proofs: the largest c that still leaves MIN_WAVES waves in the launch
+// (and at least two per proof). A wave ends with a 64 -> 1 reduction on narrow
+// levels, so larger chunks keep more of its work on full-width levels; fewer
+// than three waves per SIMD (the most the stage kernel's LDS allows) leave
+// its VALU idle behind memory and LDS waits.
 inline uint32_t pick_logc(uint64_t n, uint32_t logc_max = MERKLE_LOGC_MAX, uint32_t batch = 1) {
-    const uint64_t MIN_WAVES = 1024;
+    const uint64_t MIN_WAVES = 3072;
     for (int c = (int)logc_max; c > 0; c--) {
         const uint64_t w = n >> (7 + c);
         if (w >= 2 && w * batch >= MIN_WAVES) return (uint32_t)c;

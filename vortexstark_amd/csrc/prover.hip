@@ -290,17 +290,23 @@ __global__ void replicate_kernel(const uint32_t* src, uint64_t words, uint64_t s
         d[i] = src[i];
 }
 
-// Proofs 0 .. gridDim.y-1 (words each, R words apart) and their transcript
-// failure flags, written by the device straight into pinned host memory: the
-// copy stays on the prover's compute queue, with no copy-engine hand-off.
-__global__ void proofs_to_host_kernel(const uint32_t* src, uint64_t R, uint64_t words, uint32_t* dst,
+// Proofs 0 .. count-1 (words each, R words apart) and their transcript failure
+// flags, written by the device straight into pinned host memory: the copy
+// stays on the prover's compute queue, with no copy-engine hand-off. The grid
+// is small (PCIe, not the CUs, sets the pace) so that the copy of one batch
+// leaves wave slots to the kernels of other batches.
+constexpr uint32_t P2H_BLOCKS = 64;
+__global__ void proofs_to_host_kernel(const uint32_t* src, uint64_t R, uint64_t words, uint32_t count, uint32_t* dst,
                                       const uint32_t* failed_src, uint32_t* failed_dst) {
-    const uint32_t y = blockIdx.y;
-    const uint32_t* s = src + (uint64_t)y * R;
-    uint32_t* d = dst + (uint64_t)y * words;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
-        d[i] = s[i];
-    if (failed_dst && blockIdx.x == 0 && threadIdx.x == 0) failed_dst[y] = failed_src[(uint64_t)y * R];
+    const uint64_t nth = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t y = 0; y < count; y++) {
+        const uint32_t* s = src + (uint64_t)y * R;
+        uint32_t* d = dst + (uint64_t)y * words;
+        for (uint64_t i = tid; i < words; i += nth) d[i] = s[i];
+    }
+    if (failed_dst)
+        for (uint64_t y = tid; y < count; y += nth) failed_dst[y] = failed_src[y * R];
     __threadfence_system();
 }
 
@@ -337,11 +343,12 @@ struct vx_prover {
     uint64_t* d_call = nullptr;
     uint64_t* h_call = nullptr;  // pinned staging
     uint8_t* h_setup = nullptr;  // pinned staging of the set-up copies (kept until destroy)
+    hipEvent_t call_copied = nullptr;  // recorded after the last copy out of h_call
     std::vector<uint64_t> last_call;
     bool traces_aligned = false;  // every trace pointer of the current call is 16-byte aligned
-    // graph replay of the launch sequence of a full batch: one instantiated
-    // graph per trace alignment
-    static constexpr size_t GRAPH_CACHE = 2;
+    // graph replay of the launch sequence: one instantiated graph per batch
+    // count and trace alignment, at most GRAPH_CACHE of them
+    static constexpr size_t GRAPH_CACHE = 32;
     struct Captured {
         uint32_t count;
         bool aligned;  // the captured launches move trace tiles as uint4
@@ -361,9 +368,19 @@ struct vx_prover {
     uint32_t* proof(uint32_t p = 0) const { return arena + (uint64_t)p * R + o_proof; }
     const uint32_t* ab_dev() const { return (const uint32_t*)(d_call + capacity); }
 
+    // Device memory is carved from one pool sized by the prover's exact
+    // estimate (one hipMalloc instead of dozens); regions are 256-byte aligned.
+    uint8_t* pool = nullptr;
+    uint64_t pool_size = 0, pool_used = 0;
     bool alloc(void** p, uint64_t bytes) {
-        if (!check(hipMalloc(p, bytes ? bytes : 4), "vx_prover: hipMalloc")) return false;
-        allocs.push_back(*p);
+        const uint64_t need = (bytes + 255) & ~255ull;
+        if (pool && pool_used + need <= pool_size) {
+            *p = pool + pool_used;
+            pool_used += need;
+        } else {
+            if (!check(hipMalloc(p, bytes ? bytes : 4), "vx_prover: hipMalloc")) return false;
+            allocs.push_back(*p);
+        }
         alloc_bytes += bytes;
         return true;
     }
@@ -379,6 +396,7 @@ struct vx_prover {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graphs();
         for (void* p : allocs) (void)hipFree(p);
+        if (call_copied) (void)hipEventDestroy(call_copied);
         if (h_call) (void)hipHostFree(h_call);
         if (h_setup) (void)hipHostFree(h_setup);
         if (stream) (void)hipStreamDestroy(stream);
@@ -545,34 +563,34 @@ bool upload_call(vx_prover* p, const uint32_t* const* traces, const uint32_t* ab
         ((uint32_t*)(want.data() + p->capacity))[2 * i + 1] = ab[2 * i + 1] % P;
     }
     if (want == p->last_call) return true;
-    if (!check(hipStreamSynchronize(p->stream), "vx_prover: upload")) return false;
+    // the previous copy out of the pinned staging must have been read first
+    if (p->call_copied && !check(hipEventSynchronize(p->call_copied), "vx_prover: upload")) return false;
+    if (!p->call_copied && !check(hipEventCreateWithFlags(&p->call_copied, hipEventDisableTiming), "hipEventCreate"))
+        return false;
     std::memcpy(p->h_call, want.data(), 8 * want.size());
-    if (!check(hipMemcpyAsync(p->d_call, p->h_call, 8 * want.size(), hipMemcpyHostToDevice, p->stream), "H2D"))
+    if (!check(hipMemcpyAsync(p->d_call, p->h_call, 8 * want.size(), hipMemcpyHostToDevice, p->stream), "H2D") ||
+        !check(hipEventRecord(p->call_copied, p->stream), "hipEventRecord"))
         return false;
     p->last_call = want;
     return true;
 }
 
 int32_t run_batch(vx_prover* p, uint32_t count) {
-    // Graphs replay full batches only: a partial batch (any other count) would
-    // need a graph of its own, and capturing and retiring graphs for the many
-    // counts a server sees costs more host time than launching directly.
-    if (!p->use_graph || count != p->capacity) {
-        p->tl.n = 0;
-        enqueue_prove(p, count);
-        return peek_error() == 0 ? 0 : -1;
-    }
     auto& G = p->graphs;
     auto it = std::find_if(G.begin(), G.end(), [&](const vx_prover::Captured& c) {
         return c.count == count && c.aligned == p->traces_aligned;
     });
+    // Graphs are kept until the prover is destroyed (destroying one waits for
+    // the whole device); once GRAPH_CACHE counts are cached, other counts are
+    // launched directly.
+    if (!p->use_graph || (it == G.end() && G.size() == vx_prover::GRAPH_CACHE)) {
+        p->tl.n = 0;
+        enqueue_prove(p, count);
+        return peek_error() == 0 ? 0 : -1;
+    }
     if (it != G.end()) {
         std::rotate(it, it + 1, G.end());  // most recently used last
     } else {
-        if (G.size() == vx_prover::GRAPH_CACHE) {
-            vx_prover::destroy(G.front());
-            G.erase(G.begin());
-        }
         const bool timing = p->tl.on;
         p->tl.on = false;  // no event records inside a captured graph
         if (!check(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
@@ -645,6 +663,12 @@ vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t l
     vx_prover* p = new vx_prover();
     set_sizes(p, log_n, n_queries, log_last_layer, capacity);
     const uint64_t N = p->N, E = p->E;
+    p->pool_size = vx_prover_estimate_bytes(log_n, n_queries, log_last_layer, capacity) + 256ull * (64 + p->J);
+    if (!check(hipMalloc((void**)&p->pool, p->pool_size), "vx_prover: hipMalloc")) {
+        delete p;
+        return nullptr;
+    }
+    p->allocs.push_back(p->pool);
     bool ok = check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
     hipStream_t s = p->stream;
 
@@ -859,9 +883,8 @@ int32_t vx_prover_enqueue_copy(vx_prover* p, uint32_t count, void* host_proofs, 
     uint32_t* dev_proofs = (uint32_t*)device_view_of_host(host_proofs);
     uint32_t* dev_failed = host_failed ? (uint32_t*)device_view_of_host(host_failed) : nullptr;
     if (dev_proofs && ((uintptr_t)dev_proofs & 3) == 0 && (!host_failed || (dev_failed && ((uintptr_t)dev_failed & 3) == 0))) {
-        const uint32_t gx = (uint32_t)std::min<uint64_t>(16, (p->proof_words + 255) / 256);
-        proofs_to_host_kernel<<<dim3(gx, count), 256, 0, s>>>(p->proof(0), p->R, p->proof_words, dev_proofs,
-                                                              &p->ch()->failed, dev_failed);
+        proofs_to_host_kernel<<<P2H_BLOCKS, 256, 0, s>>>(p->proof(0), p->R, p->proof_words, count, dev_proofs,
+                                                         &p->ch()->failed, dev_failed);
         return check_launch("proofs_to_host_kernel") ? 0 : -1;
     }
     // other host memory: one strided DMA copy per field (proof i sits i * R words after proof 0)
