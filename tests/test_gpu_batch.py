@@ -79,3 +79,44 @@ def test_device_bytes_equal_estimate(vx, log_n, nq, last, cap):
         assert p.device_bytes == vx.lib.vx_prover_estimate_bytes(log_n, nq, last, cap)
     finally:
         p.close()
+
+
+@pytest.mark.gpu
+def test_enqueue_copy_pinned_and_pageable(vx):
+    """vx_prover_enqueue_copy writes a batch's proofs and failure flags into
+    pinned memory with a device kernel and into other host memory with a DMA
+    copy; both give the bytes of proof_bytes."""
+    import ctypes as C
+
+    ab = pairs(11, 5)
+    p = vx.Prover(9, 80, 3, capacity=8)
+    try:
+        p.prove_batch_async(ab)
+        vx.synchronize_stream(p.stream)
+        want = [p.proof_bytes(i) for i in range(len(ab))]
+        size = p.proof_size * len(ab)
+        # pinned: device-written
+        hp, hf = C.c_void_p(), C.c_void_p()
+        assert vx.lib.vx_malloc_host(C.byref(hp), size) == 0 and vx.lib.vx_malloc_host(C.byref(hf), 4 * len(ab)) == 0
+        try:
+            C.memset(hp, 0xAB, size)
+            C.memset(hf, 0xFF, 4 * len(ab))
+            assert vx.lib.vx_prover_enqueue_copy(p.handle, len(ab), hp, hf, None) == 0
+            vx.synchronize_stream(p.stream)
+            got = C.string_at(hp, size)
+            flags = np.frombuffer(C.string_at(hf, 4 * len(ab)), dtype=np.uint32)
+        finally:
+            vx.lib.vx_free_host(hp)
+            vx.lib.vx_free_host(hf)
+        assert [got[i * p.proof_size:(i + 1) * p.proof_size] for i in range(len(ab))] == want
+        assert (flags == 0).all()
+        # pageable: DMA copy
+        buf = (C.c_uint8 * size)()
+        fl = (C.c_uint32 * len(ab))(*([7] * len(ab)))
+        assert vx.lib.vx_prover_enqueue_copy(p.handle, len(ab), buf, fl, None) == 0
+        vx.synchronize_stream(p.stream)
+        got2 = bytes(buf)
+        assert [got2[i * p.proof_size:(i + 1) * p.proof_size] for i in range(len(ab))] == want
+        assert list(fl) == [0] * len(ab)
+    finally:
+        p.close()
