@@ -210,6 +210,14 @@ def bench_plain(vx, args) -> dict:
                          f"last layer 2^{last}, {args.cpu_reps} reps after twiddle precompute, {secs:.2f}s; "
                          f"{cpu_model()}"}
         log(f"cpu baseline: {cpu_rate:.3f} Melem/s ({secs:.2f}s)")
+        # one thread, on a smaller trace so the run stays short
+        l1 = min(cl, 18)
+        L.orc_set_threads(1)
+        secs1 = L.orc_time_prove_fibonacci(a % vx.P, b % vx.P, l1, nq, last, 1)
+        cpu["single_thread"] = {"value": round((1 << l1) / secs1 / 1e6, 4), "unit": "Melem/s", "cores": 1,
+                                "sample": f"one oracle proof of a 2^{l1}-row column, {secs1:.2f}s"}
+        L.orc_set_threads(threads)
+        log(f"cpu baseline, 1 thread: {(1 << l1) / secs1 / 1e6:.3f} Melem/s at 2^{l1} rows ({secs1:.2f}s)")
     prover.close()
     return {"melem_s": melem_s, "ms_per_step": ms_per_step, "elapsed": elapsed, "proofs_per_s": args.steps / elapsed,
             "latency": quantiles(lat), "roofline": roofline, "roofline_hbm": roofline_hbm,
