@@ -450,6 +450,11 @@ def main() -> int:
         MEASURED["ntt_plans"] = {str(k): {"ms": round(ms, 4), "passes": plan}
                                  for k, (ms, plan) in vx.ntt_autotune(10, 28).items()}
         log(f"ntt autotune 2^10..2^28: {time.time() - t:.1f}s")
+        t = time.time()
+        MEASURED["merkle_stage_plans"] = {str(k): {"ms": round(ms, 4), "logc": c}
+                                          for k, (ms, c) in vx.merkle_autotune(12, 27).items()}
+        log(f"merkle stage autotune 2^12..2^27: {time.time() - t:.1f}s "
+            f"{ {k: v['logc'] for k, v in MEASURED['merkle_stage_plans'].items()} }")
     extra = {}
     if "plain" in skip:  # profiling runs of the other workloads only
         for name, fn in (("wide", bench_wide), ("batch", bench_batch), ("serve", bench_serve)):
@@ -491,6 +496,7 @@ def main() -> int:
         "parity_vs_oracle": plain["parity"],
         "cpu_baseline": plain["cpu"],
         "valu_peak_measured": round(MEASURED["valu_peak"], 3),
+        "merkle_stage_plans": MEASURED.get("merkle_stage_plans"),
         "ntt_plans": MEASURED.get("ntt_plans"),
     }
     out.update(extra)

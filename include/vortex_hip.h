@@ -189,6 +189,19 @@ int32_t vx_ntt_plan_set(uint32_t log_n, uint32_t n_passes, const uint32_t* m, co
 uint32_t vx_ntt_plan_get(uint32_t log_n, uint32_t* m, uint32_t* c);
 void vx_ntt_plan_reset(void);
 
+/* Chunk depth of the Merkle stage kernels (no reference counterpart): a wave
+ * hashes 2^c chunks of 128 items before its 64 -> 1 reduction. The depth is
+ * picked per launch by the items it covers (all proofs of a batch): a tuned
+ * value when one is set, else the largest c leaving 3072 waves. vx_merkle_autotune
+ * times whole tree commitments of 2^log items (12 <= log <= 27) at every
+ * depth on the device and keeps the fastest; plan returns the tuned depth of
+ * 2^log_items items (-1: default rule); plan_set sets one (-1 clears it). Any
+ * depth gives the same roots. */
+int32_t vx_merkle_autotune(uint32_t log_min, uint32_t log_max, float* best_ms);
+int32_t vx_merkle_stage_plan(uint32_t log_items);
+int32_t vx_merkle_stage_plan_set(uint32_t log_items, int32_t logc);
+void vx_merkle_stage_plan_reset(void);
+
 /* ===================== Part 3: fused device-resident prover ================= */
 
 typedef struct vx_prover vx_prover;

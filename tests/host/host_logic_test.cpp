@@ -10,6 +10,7 @@
 
 #include "../../vortexstark_amd/csrc/ntt_plan.hpp"
 #include "../../vortexstark_amd/csrc/result_ring.hpp"
+#include "../../vortexstark_amd/csrc/stage_plan.hpp"
 
 #define CHECK(c)                                                         \
     do {                                                                 \
@@ -128,7 +129,37 @@ static void test_ring() {
     }
 }
 
+static void test_stage_plan() {
+    for (uint32_t i = 0; i < STAGE_PLAN_LOGS; i++) stage_plan_table()[i].store(0);
+    // default rule: the largest depth leaving STAGE_MIN_WAVES waves
+    CHECK(pick_logc(1ull << 26) == 5);  // 16384 waves
+    CHECK(pick_logc(1ull << 23) == 4);  // 4096 waves (depth 5 leaves 2048)
+    CHECK(pick_logc(1ull << 19) == 0);
+    CHECK(pick_logc(1ull << 26, 4) == 4);
+    CHECK(pick_logc(1ull << 16, 5, 256) == 5);  // 2^24 items in the launch, 16 waves per proof
+    CHECK(pick_logc(1ull << 12, 5, 1024) == 3);  // two waves per proof at most depth 3
+    for (uint32_t l = 8; l <= 30; l++)
+        for (uint32_t batch : {1u, 3u, 64u, 1024u})
+            for (uint32_t mx = 0; mx <= MERKLE_LOGC_MAX; mx++) {
+                const uint64_t n = 1ull << l;
+                const uint32_t c = pick_logc(n, mx, batch);
+                CHECK(c <= mx);
+                CHECK(c == 0 || (n >> (7 + c)) >= 2);
+                CHECK((n >> (7 + c)) <= max_stage_nodes(n, 2048) || n <= 2048);
+            }
+    // tuned depths are used, clipped to the source's maximum and to two waves per proof
+    stage_plan_table()[20].store(2 + 1);
+    CHECK(pick_logc(1ull << 20) == 2);
+    CHECK(pick_logc(1ull << 14, 5, 64) == 2);  // 2^20 items in the launch
+    CHECK(pick_logc(1ull << 20, 1) == 1);
+    stage_plan_table()[9].store(5 + 1);
+    CHECK(pick_logc(1ull << 9) == 1);  // 2^9 items: depth 1 is the deepest with two waves
+    for (uint32_t i = 0; i < STAGE_PLAN_LOGS; i++) stage_plan_table()[i].store(0);
+    CHECK(max_stage_nodes(2048, 2048) == 0 && max_stage_nodes(4096, 2048) == 32);
+}
+
 int main() {
+    test_stage_plan();
     test_plans();
     test_ring();
     std::printf("host logic: ok\n");

@@ -396,6 +396,27 @@ def ntt_plan_set(log_n: int, passes) -> None:
         raise HipError("vx_ntt_plan_set failed")
 
 
+def merkle_autotune(log_min: int, log_max: int) -> dict:
+    """Times whole tree commitments of 2^log_min .. 2^log_max items at every
+    stage chunk depth and installs the fastest; returns {log: (ms, depth)}."""
+    best = np.zeros(log_max - log_min + 1, np.float32)
+    if lib.vx_merkle_autotune(log_min, log_max, best.ctypes.data):
+        check("vx_merkle_autotune")
+        raise HipError("vx_merkle_autotune failed")
+    return {k: (float(best[k - log_min]), int(lib.vx_merkle_stage_plan(k))) for k in range(log_min, log_max + 1)}
+
+
+def merkle_stage_plan(log_items: int) -> int:
+    """Tuned stage chunk depth for 2^log_items items per launch (-1: default rule)."""
+    return int(lib.vx_merkle_stage_plan(log_items))
+
+
+def merkle_stage_plan_set(log_items: int, logc: int) -> None:
+    if lib.vx_merkle_stage_plan_set(log_items, logc):
+        check("vx_merkle_stage_plan_set")
+        raise HipError("vx_merkle_stage_plan_set failed")
+
+
 def measure_valu_peak() -> float:
     """int32 VALU issue rate of the device, T lane-ops/s (microbenchmark)."""
     tops = C.c_double(0.0)

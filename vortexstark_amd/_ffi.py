@@ -89,6 +89,10 @@ SIGNATURES = {
     "vx_prover_capacity": (u32, [vp]),
     "vx_prover_device_bytes": (u64, [vp]),
     "vx_prover_estimate_bytes": (u64, [u32, u32, u32, u32]),
+    "vx_merkle_autotune": (i32, [u32, u32, vp]),
+    "vx_merkle_stage_plan": (i32, [u32]),
+    "vx_merkle_stage_plan_set": (i32, [u32, i32]),
+    "vx_merkle_stage_plan_reset": (None, []),
     "vx_prover_prove_batch_device": (i32, [vp, vp, vp, u32]),
     "vx_prove_fibonacci_batch": (i32, [vp, vp, u32]),
     "vx_prover_set_graph": (i32, [vp, i32]),

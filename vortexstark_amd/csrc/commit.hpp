@@ -7,21 +7,6 @@
 
 namespace vx {
 
-// Chunks per wave (2^c, 128 items each) of a stage over n items per proof for
-// `batch` proofs: the largest c that still leaves MIN_WAVES waves in the launch
-// (and at least two per proof). A wave ends with a 64 -> 1 reduction on narrow
-// levels, so larger chunks keep more of its work on full-width levels; fewer
-// than three waves per SIMD (the most the stage kernel's LDS allows) leave
-// its VALU idle behind memory and LDS waits.
-inline uint32_t pick_logc(uint64_t n, uint32_t logc_max = MERKLE_LOGC_MAX, uint32_t batch = 1) {
-    const uint64_t MIN_WAVES = 3072;
-    for (int c = (int)logc_max; c > 0; c--) {
-        const uint64_t w = n >> (7 + c);
-        if (w >= 2 && w * batch >= MIN_WAVES) return (uint32_t)c;
-    }
-    return 0;
-}
-
 // Trees of at most this many items go straight to the tail kernel; larger
 // ones are reduced by stage kernels until this many nodes remain (one more
 // tail level is cheaper than one more stage launch).
@@ -71,19 +56,8 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
     mark("tail");
 }
 
-// Largest stage output of a tree of n_leaves items whose leaf stage uses at most
-// leaf_logc_max (node stages use the default). A batch picks chunks at least
-// this large, so its stage outputs are no larger.
-inline uint64_t max_stage_nodes(uint64_t n_leaves, uint32_t leaf_logc_max = MERKLE_LOGC_MAX) {
-    if (n_leaves <= PROVER_TAIL_ITEMS) return 0;
-    uint64_t mx = 0, n = n_leaves;
-    bool leaf = true;
-    while (n > PROVER_TAIL_ITEMS) {
-        n >>= 7 + pick_logc(n, leaf ? leaf_logc_max : MERKLE_LOGC_MAX);
-        leaf = false;
-        if (n > mx) mx = n;
-    }
-    return mx;
-}
+// Largest stage output of a tree of n_leaves items (per proof), for sizing
+// the stage scratch buffers.
+inline uint64_t max_stage_nodes(uint64_t n_leaves) { return max_stage_nodes(n_leaves, PROVER_TAIL_ITEMS); }
 
 }  // namespace vx

@@ -25,6 +25,7 @@
 #pragma once
 #include <type_traits>
 
+#include "stage_plan.hpp"
 #include "blake2s.hpp"
 #include "field.hpp"
 #include "transcript.hpp"
@@ -40,7 +41,6 @@ namespace vx {
     } while (0)
 
 constexpr uint32_t MERKLE_TAIL_MAX = 2048;  // most items one tail-kernel block reduces
-constexpr uint32_t MERKLE_LOGC_MAX = 5;     // chunks per wave in a stage: up to 32
 constexpr uint32_t MERKLE_STAGE_WAVES = 4;  // waves per stage workgroup
 
 // Nodes of levels [from, depth] of a tree with 2^depth leaves, level-major.

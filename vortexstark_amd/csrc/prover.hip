@@ -896,6 +896,71 @@ int32_t vx_prover_enqueue_copy(vx_prover* p, uint32_t count, void* host_proofs, 
     return 0;
 }
 
+int32_t vx_merkle_autotune(uint32_t log_min, uint32_t log_max, float* best_ms) {
+    if (log_min < 12 || log_max < log_min || log_max > 27) {
+        set_error(-2, "vx_merkle_autotune: sizes must satisfy 12 <= log_min <= log_max <= 27");
+        return -1;
+    }
+    const uint64_t N = 1ull << log_max, mx = max_stage_nodes(N);
+    hipStream_t st = nullptr;
+    uint32_t *data = nullptr, *sa = nullptr, *sb = nullptr, *root = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate") &&
+              check(hipMalloc((void**)&data, 4 * N), "hipMalloc") && check(hipMalloc((void**)&sa, 32 * mx), "hipMalloc") &&
+              check(hipMalloc((void**)&sb, 32 * mx), "hipMalloc") && check(hipMalloc((void**)&root, 32), "hipMalloc") &&
+              check(hipEventCreate(&e0), "hipEvent") && check(hipEventCreate(&e1), "hipEvent") &&
+              check(hipMemsetAsync(data, 0x11, 4 * N, st), "hipMemsetAsync");  // canonical M31 words
+    const LevelStore none = {nullptr, 0, 0};
+    const TailOut out = {root, nullptr, nullptr, 0};
+    auto no_mark = [](const char*) {};
+    // smaller sizes first: a tree's node stages use the depths already tuned for their sizes
+    for (uint32_t lt = log_min; ok && lt <= log_max; lt++) {
+        const uint64_t n = 1ull << lt;
+        float best = 1e30f;
+        uint32_t bc = 0;
+        for (uint32_t c = 0; ok && c <= MERKLE_LOGC_MAX && (n >> (7 + c)) >= 2; c++) {
+            stage_plan_table()[lt].store((uint8_t)(c + 1));
+            commit_tree(SrcCol1{data}, n, none, out, Scratch{sa, sb}, st, no_mark);  // warm-up
+            const int reps = 3;
+            ok = check(hipEventRecord(e0, st), "hipEventRecord");
+            for (int r = 0; r < reps; r++) commit_tree(SrcCol1{data}, n, none, out, Scratch{sa, sb}, st, no_mark);
+            ok = ok && check(hipEventRecord(e1, st), "hipEventRecord") && check(hipEventSynchronize(e1), "sync");
+            float ms = 0.f;
+            ok = ok && check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime") && peek_error() == 0;
+            if (ok && ms / reps < best) {
+                best = ms / reps;
+                bc = c;
+            }
+        }
+        stage_plan_table()[lt].store(ok ? (uint8_t)(bc + 1) : (uint8_t)0);
+        if (best_ms) best_ms[lt - log_min] = best;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamSynchronize(st);
+    for (uint32_t* b : {data, sa, sb, root})
+        if (b) (void)hipFree(b);
+    if (st) (void)hipStreamDestroy(st);
+    return ok ? 0 : -1;
+}
+
+int32_t vx_merkle_stage_plan(uint32_t log_items) {
+    return log_items < STAGE_PLAN_LOGS ? (int32_t)stage_plan_table()[log_items].load() - 1 : -1;
+}
+
+int32_t vx_merkle_stage_plan_set(uint32_t log_items, int32_t logc) {
+    if (log_items >= STAGE_PLAN_LOGS || logc < -1 || logc > (int32_t)MERKLE_LOGC_MAX) {
+        set_error(-2, "vx_merkle_stage_plan_set: log_items < 40 and -1 <= logc <= 5");
+        return -1;
+    }
+    stage_plan_table()[log_items].store((uint8_t)(logc + 1));
+    return 0;
+}
+
+void vx_merkle_stage_plan_reset(void) {
+    for (uint32_t i = 0; i < STAGE_PLAN_LOGS; i++) stage_plan_table()[i].store(0);
+}
+
 uint64_t vx_krak_proof_size(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer) {
     if (log_n < 2 || log_n + 2 > 30 || log_last_layer > log_n + 1) return 0;
     vx_prover shape;  // layout only: no device resources

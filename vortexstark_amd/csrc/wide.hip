@@ -355,8 +355,7 @@ vx_wide* vx_wide_create(uint32_t n_cols, const uint32_t* log_sizes, uint32_t log
     ok = ok && check(hipHostMalloc((void**)&w->h_src_ptrs, 8ull * n_cols), "hipHostMalloc");
     w->layer.assign(w->J, nullptr);
     for (uint32_t j = 0; ok && j < w->J; j++) ok = w->alloc((void**)&w->layer[j], 16 * w->layer_size(j));
-    const uint64_t mxn = max_stage_nodes(
-        1ull << S, std::min(StageLogcMax<SrcLifted>::value, StageLogcMax<SrcWideFold>::value));
+    const uint64_t mxn = max_stage_nodes(1ull << S);
     ok = ok && w->alloc((void**)&w->scratch.a, 32 * (mxn ? mxn : 1)) &&
          w->alloc((void**)&w->scratch.b, 32 * (mxn ? mxn : 1));
     ok = ok && w->alloc((void**)&w->ch, sizeof(DevChannel)) && w->alloc((void**)&w->chal, 16ull * (w->J + 1)) &&
