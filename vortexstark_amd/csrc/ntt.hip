@@ -83,6 +83,7 @@ struct NttPassArgs {
     uint32_t k;      // log2 of the transform size
     uint32_t scale;  // multiply on store when non-zero
     uint32_t vec;    // every column pointer is 16-byte aligned: move tiles 4 words per lane
+    uint32_t zero_top2;  // forward, top pass, src_n = n / 4: the two top layers only copy
 };
 
 // LDS word index with one pad word per 8, which keeps the strided radix-8
@@ -110,8 +111,12 @@ __device__ __forceinline__ uint32_t ntt_twiddle(const NttPassArgs& a, uint32_t b
 // J and R are compile-time, so the 2^R LDS addresses of a group are one base
 // plus constant offsets, and the twiddle of butterfly d in layer t is one
 // per-layer base index plus the constant d >> (t + 1).
+// With zero_top set (the first round of a forward transform whose input is
+// zero past its first quarter), the values of the group's two upper row bits
+// are zero on entry, so the top two layers only copy: (x, 0) -> (x, x). Those
+// rounds load two values and replicate them instead of running 8 butterflies.
 template <bool FWD, int M, int C, int R, int J, bool STWO>
-__device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi) {
+__device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi, bool zero_top = false) {
     constexpr uint32_t GROUPS = 1u << (M + C - R);
     constexpr uint32_t STEP = 1u << (J + C);  // tile index distance between rows d and d + 1
     for (uint32_t g = threadIdx.x; g < GROUPS; g += blockDim.x) {
@@ -126,11 +131,20 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
         for (int d = 0; d < (1 << R); d++) {
             // lpad is linear in d once the row step is a multiple of 8 words
             addr[d] = (J + C >= 3) ? lpad(e0) + (uint32_t)d * (STEP + STEP / 8) : lpad(e0 + (uint32_t)d * STEP);
-            v[d] = tile[addr[d]];
+        }
+        if (FWD && R == 3 && zero_top) {
+            v[0] = tile[addr[0]];
+            v[1] = tile[addr[1]];
+#pragma unroll
+            for (int d = 2; d < (1 << R); d++) v[d] = v[d & 1];
+        } else {
+#pragma unroll
+            for (int d = 0; d < (1 << R); d++) v[d] = tile[addr[d]];
         }
 #pragma unroll
         for (int tt = 0; tt < R; tt++) {
             const int t = FWD ? (R - 1 - tt) : tt;  // layer inside the group
+            if (FWD && R == 3 && t >= 1 && zero_top) continue;
             const uint32_t rb = J + t;                 // row bit of the layer
             const uint32_t b = a.s + rb;               // global stride bit
             // h(d) = (hi << (M - rb - 1)) + (row >> (rb + 1)) = hb + (d >> (t + 1))
@@ -163,7 +177,8 @@ template <bool FWD, int M, int C, bool STWO, int TOP>
 __device__ __forceinline__ void fwd_rounds(uint32_t* tile, const NttPassArgs& a, uint64_t hi) {
     if constexpr (TOP > 0) {
         constexpr int R = TOP >= 3 ? 3 : TOP;
-        ntt_round<FWD, M, C, R, TOP - R, STWO>(tile, a, hi);
+        // only the pass's first round (its top three row bits) may skip its top two layers
+        ntt_round<FWD, M, C, R, TOP - R, STWO>(tile, a, hi, TOP == M && a.zero_top2 != 0);
         __syncthreads();
         fwd_rounds<FWD, M, C, STWO, TOP - R>(tile, a, hi);
     }
@@ -317,6 +332,9 @@ static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t 
         a.k = k;
         a.vec = vec ? 1u : 0u;
         a.scale = (!fwd && i + 1 == ps.size() && inv_n != 1u) ? inv_n : 0u;
+        // the top pass of an evaluation of n/4 coefficients: its two top layers copy
+        a.zero_top2 = (fwd && i == 0 && ps[pi].m >= 3 && ps[pi].s + ps[pi].m == k && k >= 2 && src_n == (1ull << k) / 4)
+                          ? 1u : 0u;
         launch_pass(fwd, t.stwo, a, ps[pi].m, ps[pi].c, k, n_cols, st);
     }
     if (!fwd && ps.empty() && src != dst && src_n) {
