@@ -1,10 +1,12 @@
 // The small FRI layers of a proof in one launch.
 //
-// Below 1024 values a layer's fold, leaf hashes, tree and transcript step fit
-// one workgroup, and launching a kernel per layer costs more than the layer
+// Up to 2048 values a layer's fold, leaf hashes, tree and transcript step fit
+// one workgroup, and launching kernels per layer costs more than the layer
 // itself. fri_tail_kernel runs every remaining line fold of a proof in one
-// workgroup: fold (cuda/fri.cu:10-39) into the next layer, hash it as an SoA4
-// tree (level-store nodes written as by the stage/tail kernels), reduce to the
+// workgroup of 1024 threads: fold (cuda/fri.cu:10-39) into the next layer,
+// hash it as an SoA4 tree (level-store nodes written as by the stage/tail
+// kernels; levels wider than the block's 256 quads with one compression per
+// thread, the rest quad-cooperatively across the whole block), reduce to the
 // root, mix the root into the transcript and draw the next challenge, which
 // stays in LDS for the next layer (and is also written to device memory).
 // blockIdx.y selects the proof of a batch (buffers bs words apart).
@@ -14,8 +16,8 @@
 namespace vx {
 
 constexpr uint32_t FRI_TAIL_MAX_LAYERS = 12;
-constexpr uint32_t FRI_TAIL_MAX_ITEMS = 1024;  // values of the largest layer handled here
-constexpr uint32_t FRI_TAIL_THREADS = 512;
+constexpr uint32_t FRI_TAIL_MAX_ITEMS = 2048;  // values of the largest layer handled here
+constexpr uint32_t FRI_TAIL_THREADS = 1024;
 
 struct FriTailLayer {
     const uint32_t* in[4];   // previous layer, 2n values per column
@@ -24,7 +26,7 @@ struct FriTailLayer {
     uint32_t* root_dst;      // 8 words
     uint32_t* alpha_dst;     // next layer's challenge (null after the last layer)
     LevelStore st;
-    uint32_t n;              // values of this layer: a power of two in [2, 1024]
+    uint32_t n;              // values of this layer: a power of two in [2, FRI_TAIL_MAX_ITEMS]
 };
 struct FriTailArgs {
     FriTailLayer L[FRI_TAIL_MAX_LAYERS];
@@ -33,10 +35,51 @@ struct FriTailArgs {
     DevChannel* channel;
 };
 
+// Block-wide form of narrow_reduce: n (a power of two) nodes stored node-major
+// in LDS buffer a are reduced to one, every quad of the block taking one
+// compression per level (ping-ponging with b), so a level of up to
+// THREADS / 4 compressions costs one quad-cooperative compression of latency.
+// All threads of the block must call it; returns the buffer holding the root.
+template <uint32_t THREADS>
+__device__ __forceinline__ uint32_t* block_quad_reduce(uint32_t* a, uint32_t* b, uint32_t n, uint32_t lvl,
+                                                       const LevelStore& st) {
+    constexpr uint32_t QUADS = THREADS / 4;
+    const uint32_t quad = threadIdx.x >> 2, l = threadIdx.x & 3;
+    const uint32_t wave_q0 = (threadIdx.x >> 6) * 16;  // first quad of this wave
+    while (n > 1) {
+        const uint32_t half = n / 2;
+        for (uint32_t q0 = 0; q0 < half; q0 += QUADS) {
+            if (q0 + wave_q0 >= half) continue;  // wave-uniform: no quad of this wave has a node
+            const uint32_t q = q0 + quad;
+            const uint32_t qq = q < half ? q : half - 1;  // whole quads stay active for the DPP permutes
+            uint32_t hl, hh;
+            b2s_quad_init(hl, hh);
+            b2s_compress_quad(a + 16 * qq, 64u, 0xFFFFFFFFu, hl, hh);
+            if (q < half) {
+                b[8 * q + l] = hl;
+                b[8 * q + 4 + l] = hh;
+                if (st.want(lvl + 1)) {
+                    uint32_t* dst = st.at(lvl + 1, q);
+                    dst[l] = hl;
+                    dst[4 + l] = hh;
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t* t = a;
+        a = b;
+        b = t;
+        n = half;
+        lvl++;
+    }
+    return a;
+}
+
 static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTailArgs args, uint64_t bs) {
-    constexpr uint32_t W = FRI_TAIL_MAX_ITEMS / 2;
-    __shared__ uint32_t nodes[8 * W];  // word-major: word q of node j at q * W + j
-    __shared__ uint32_t na[8 * 64], nb[8 * 64];
+    constexpr uint32_t W = FRI_TAIL_MAX_ITEMS / 2;  // nodes of the widest level
+    constexpr uint32_t QUADS = FRI_TAIL_THREADS / 4;
+    __shared__ uint4 bufa[2 * W];      // node-major: node j in bufa[2j], bufa[2j + 1]
+    __shared__ uint4 bufb[W];          // half of that
     __shared__ uint32_t msg[16];
     __shared__ uint32_t al[4];
     const uint32_t tid = threadIdx.x;
@@ -53,6 +96,8 @@ static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTa
         const uint32_t* i1 = L.in[1] + boff;
         const uint32_t* i2 = L.in[2] + boff;
         const uint32_t* i3 = L.in[3] + boff;
+        uint32_t* cur = (uint32_t*)bufa;
+        uint32_t* nxt = (uint32_t*)bufb;
         // level 1: parents of leaf pairs (2j, 2j+1), each leaf a fold of two values
         for (uint32_t j = tid; j < half; j += FRI_TAIL_THREADS) {
             const uint32_t k = 2 * j;
@@ -71,38 +116,35 @@ static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTa
             b2s_leaf4(f1.v[0], f1.v[1], f1.v[2], f1.v[3], r);
             b2s_node(l, r, h);
             if (st.want(1)) store8(st.at(1, j), h);
-#pragma unroll
-            for (int q = 0; q < 8; q++) nodes[q * W + j] = h[q];
+            bufa[2 * j] = make_uint4(h[0], h[1], h[2], h[3]);
+            bufa[2 * j + 1] = make_uint4(h[4], h[5], h[6], h[7]);
         }
         __syncthreads();
         uint32_t m = half, lvl = 1;
-        while (m > 64) {
+        // wide levels (more compressions than quads): one per thread, all lanes busy
+        while (m > 2 * QUADS) {
             const uint32_t hm = m / 2;
-            uint32_t l[8], r[8];
-            if (tid < hm) {
-#pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    l[q] = nodes[q * W + 2 * tid];
-                    r[q] = nodes[q * W + 2 * tid + 1];
-                }
-            }
-            __syncthreads();
-            lvl++;
-            if (tid < hm) {
+            for (uint32_t j = tid; j < hm; j += FRI_TAIL_THREADS) {
+                const uint4* c4 = (const uint4*)cur + 4 * j;
+                const uint4 p0 = c4[0], p1 = c4[1], p2 = c4[2], p3 = c4[3];
+                const uint32_t lh[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+                const uint32_t rh[8] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
                 uint32_t h[8];
-                b2s_node(l, r, h);
-#pragma unroll
-                for (int q = 0; q < 8; q++) nodes[q * W + tid] = h[q];
-                if (st.want(lvl)) store8(st.at(lvl, tid), h);
+                b2s_node(lh, rh, h);
+                if (st.want(lvl + 1)) store8(st.at(lvl + 1, j), h);
+                uint4* n4 = (uint4*)nxt + 2 * j;
+                n4[0] = make_uint4(h[0], h[1], h[2], h[3]);
+                n4[1] = make_uint4(h[4], h[5], h[6], h[7]);
             }
             __syncthreads();
+            uint32_t* t = cur;
+            cur = nxt;
+            nxt = t;
             m = hm;
+            lvl++;
         }
-        if (tid < 64) {  // one wave: narrow levels, root, transcript
-            if (tid < m)
-                for (int q = 0; q < 8; q++) na[tid * 8 + q] = nodes[q * W + tid];
-            VX_WAVE_SYNC();
-            const uint32_t* root = narrow_reduce(na, nb, m, lvl, 0, st);
+        const uint32_t* root = block_quad_reduce<FRI_TAIL_THREADS>(cur, nxt, m, lvl, st);
+        if (tid < 64) {  // one wave: root and transcript
             if (tid < 8) L.root_dst[boff + tid] = root[tid];
             uint32_t* adst = L.alpha_dst ? L.alpha_dst + boff : nullptr;
             quad_transcript(ch, root, msg, false, adst);
