@@ -43,6 +43,9 @@ VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 # VALU ops of one Blake2s compression on gfx950: 10 rounds x 8 G x 12 ops (v_add3,
 # v_add, v_xor, v_alignbit) + 8 three-input xors of the feed-forward (v_bitop3)
 OPS_PER_COMPRESSION = 10 * 8 * 12 + 8
+# one radix-2 M31 butterfly: product (v_mad_u64_u32, alignbit, and, add, sub,
+# min), modular add (add, sub, min) and modular subtract (sub, add, sub, min)
+OPS_PER_BUTTERFLY = 13
 
 
 def load_traffic() -> dict:
@@ -191,6 +194,18 @@ def bench_plain(vx, args) -> dict:
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get("lde:evaluate"),
                         "phase_ms": round(ntt_ms, 4), "passes": MEASURED.get("ntt_plans", {}).get(str(log_n + 2))}
+    # the same evaluate priced against the VALU: its butterflies are the work
+    # (the top two layers of a zero-padded input only copy and are not counted)
+    roofline_ntt = None
+    if ntt_ms:
+        k = log_n + 2
+        bfly = (k - 2) * (E // 2)
+        ops = bfly * OPS_PER_BUTTERFLY
+        achieved = ops / (ntt_ms * 1e-3) / 1e12
+        roofline_ntt = {"bound": "valu", "kernel": "ntt_pass_kernel<true> (evaluate 2^%d, all passes)" % k,
+                        "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS, "unit": "Tops/s (int32 lane-ops)",
+                        "frac": round(achieved / VALU_PEAK_TOPS, 4), "butterflies": bfly,
+                        "ops_per_butterfly": OPS_PER_BUTTERFLY, "phase_ms": round(ntt_ms, 4)}
     # pipeline-level bound: 6E Blake2s compressions at the VALU peak
     t_valu_ms = 6 * E * OPS_PER_COMPRESSION / (VALU_PEAK_TOPS * 1e12) * 1e3
 
@@ -220,7 +235,7 @@ def bench_plain(vx, args) -> dict:
         log(f"cpu baseline, 1 thread: {(1 << l1) / secs1 / 1e6:.3f} Melem/s at 2^{l1} rows ({secs1:.2f}s)")
     prover.close()
     return {"melem_s": melem_s, "ms_per_step": ms_per_step, "elapsed": elapsed, "proofs_per_s": args.steps / elapsed,
-            "latency": quantiles(lat), "roofline": roofline, "roofline_hbm": roofline_hbm,
+            "latency": quantiles(lat), "roofline": roofline, "roofline_hbm": roofline_hbm, "roofline_ntt": roofline_ntt,
             "t_valu_ms": t_valu_ms, "phases": phases, "parity": parity, "cpu": cpu}
 
 
@@ -490,6 +505,7 @@ def main() -> int:
         "latency_ms": plain["latency"],
         "roofline": plain["roofline"],
         "roofline_hbm": plain["roofline_hbm"],
+        "roofline_ntt_valu": plain["roofline_ntt"],
         "pipeline_valu_bound_ms": round(plain["t_valu_ms"], 3),
         "pipeline_frac_of_valu_bound": round(plain["t_valu_ms"] / plain["ms_per_step"], 4),
         "phases_ms": {k: round(v, 4) for k, v in plain["phases"].items()},
