@@ -161,6 +161,31 @@ def bench_plain(vx, args) -> dict:
         parity = hashlib.sha256(proof).hexdigest() == golden["sha256"]
         log(f"proof bytes identical to the oracle's golden summary: {parity}")
 
+    # the same proofs with two in flight (two provers, two streams): one
+    # proof's latency-bound FRI chain overlaps the other's hashing
+    pipelined = None
+    if not args.no_pipelined:
+        second = vx.Prover(log_n, nq, last)
+        trace2 = vx.fibonacci_trace_device(a, b, log_n)
+        pair = [(prover, trace), (second, trace2)]
+        for i in range(2):
+            pair[i][0].prove_device(pair[i][1].addr, a, b)
+        vx.synchronize()
+        t = time.perf_counter()
+        for i in range(args.steps):
+            pv, tr = pair[i % 2]
+            pv.prove_device(tr.addr, a, b)
+        vx.synchronize()
+        el2 = time.perf_counter() - t
+        vx.check("pipelined steps")
+        same = second.proof_bytes() == proof
+        pipelined = {"streams": 2, "ms_per_proof": round(1e3 * el2 / args.steps, 4),
+                     "melem_s": round((1 << log_n) * args.steps / el2 / 1e6, 2), "identical_proof": same}
+        log(f"pipelined (2 streams): {1e3 * el2 / args.steps:.3f} ms/proof, "
+            f"{(1 << log_n) * args.steps / el2 / 1e6:.1f} Melem/s, identical proof: {same}")
+        second.close()
+        trace2.free()
+
     # single-proof latency (each proof synchronised on its own)
     lat = []
     for _ in range(max(5, min(args.steps, 20))):
@@ -236,6 +261,7 @@ def bench_plain(vx, args) -> dict:
     prover.close()
     return {"melem_s": melem_s, "ms_per_step": ms_per_step, "elapsed": elapsed, "proofs_per_s": args.steps / elapsed,
             "latency": quantiles(lat), "roofline": roofline, "roofline_hbm": roofline_hbm, "roofline_ntt": roofline_ntt,
+            "pipelined": pipelined,
             "t_valu_ms": t_valu_ms, "phases": phases, "parity": parity, "cpu": cpu}
 
 
@@ -438,6 +464,7 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=100)
     ap.add_argument("--log-last", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-pipelined", action="store_true", help="skip the two-stream plain measurement")
     ap.add_argument("--cpu-log-n", type=int, default=24)
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--skip", default="", help="comma list of workloads to skip: plain,wide,batch,serve "
@@ -506,6 +533,7 @@ def main() -> int:
         "roofline": plain["roofline"],
         "roofline_hbm": plain["roofline_hbm"],
         "roofline_ntt_valu": plain["roofline_ntt"],
+        "pipelined_two_streams": plain["pipelined"],
         "pipeline_valu_bound_ms": round(plain["t_valu_ms"], 3),
         "pipeline_frac_of_valu_bound": round(plain["t_valu_ms"] / plain["ms_per_step"], 4),
         "phases_ms": {k: round(v, 4) for k, v in plain["phases"].items()},
