@@ -87,3 +87,52 @@ def test_plain_gpu_workload_log24(vx):
     assert hashlib.sha256(ev.tobytes()).hexdigest() == c["lde_sha256"]
     assert O.verify(got, c["n_queries"], c["log_last"]) == 0
     p.close()
+
+
+def test_two_provers_on_two_streams_concurrently(vx):
+    """Proofs enqueued alternately on two provers (two streams, running
+    concurrently on the device) are the oracle's, each prover's last proof
+    identical."""
+    log_n = 13
+    want = O.prove(7, 11, log_n, 80, 3)
+    p1, p2 = vx.Prover(log_n, 80, 3), vx.Prover(log_n, 80, 3)
+    t1 = vx.fibonacci_trace_device(7, 11, log_n)
+    t2 = vx.fibonacci_trace_device(7, 11, log_n)
+    try:
+        for i in range(6):
+            (p1, p2)[i % 2].prove_device((t1, t2)[i % 2].addr, 7, 11)
+        vx.synchronize()
+        assert p1.proof_bytes() == want and p2.proof_bytes() == want
+    finally:
+        p1.close()
+        p2.close()
+        t1.free()
+        t2.free()
+
+
+def test_provers_used_from_two_threads(vx):
+    """One prover per thread (the library keeps its error state and launch
+    stream per thread): concurrent proofs from two host threads are the
+    oracle's."""
+    import threading
+
+    log_n = 12
+    cases = [(3, 5), (8, 13)]
+    want = [O.prove(a, b, log_n, 80, 3) for a, b in cases]
+    got = [None, None]
+
+    def work(k):
+        p = vx.Prover(log_n, 80, 3)
+        try:
+            a, b = cases[k]
+            for _ in range(4):
+                got[k] = p.prove(a, b)
+        finally:
+            p.close()
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=60)
+    assert got == want
