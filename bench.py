@@ -14,10 +14,12 @@ The same JSON line carries the other workloads of the metric:
   batch   256 jobs of 1-8 Fibonacci columns, log_n uniform in [10, 18]:
           batched proving (graph replay) vs the same proofs one at a time;
   serve   Poisson arrivals of jobs with 1-34 columns, log_n uniform in
-          [12, 26], admitted under the HBM budget: proofs/s and p50/p99.
+          [12, 26], admitted under the HBM budget: proofs/s and p50/p99;
+  grind   256 Blake2s proof-of-work searches at 26 bits from channel
+          states on the device: nonces/s against the VALU roofline.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
-                    [--skip wide,batch,serve]
+                    [--skip wide,batch,serve,grind]
 """
 
 from __future__ import annotations
@@ -46,6 +48,12 @@ OPS_PER_COMPRESSION = 10 * 8 * 12 + 8
 # one radix-2 M31 butterfly: product (v_mad_u64_u32, alignbit, and, add, sub,
 # min), modular add (add, sub, min) and modular subtract (sub, add, sub, min)
 OPS_PER_BUTTERFLY = 13
+# VALU ops of one nonce of the proof-of-work search for pow_bits <= 32 (the loop
+# body of grind_kernel in its gfx950 ISA: 279 v_xor, 279 v_alignbit, 280 adds
+# and the check): the uniform part of round 0 runs once per wave and only digest
+# word 0 is formed
+OPS_PER_NONCE = 847
+POW_BITS = 26  # the reference transcripts' PoW (prover.rs:96)
 
 
 def load_traffic() -> dict:
@@ -455,6 +463,65 @@ def bench_serve(vx, args) -> dict:
             "peak_device_bytes": st["peak_device_bytes"], "batches": st["batches"]}
 
 
+def bench_grind(vx, args) -> dict:
+    """Blake2s PoW grinding at the reference's 26 bits: K channel states (splitmix64
+    seed 7), each grind one vx_grind_blake2s call (prefix on the device, search,
+    nonce in HBM), K calls enqueued back to back on the null stream and timed
+    between synchronisations. The work is the nonces a search must hash, found + 1
+    per state; VALU roofline at OPS_PER_NONCE."""
+    import inputs
+    import oracle_lib as O
+
+    K = 256
+    seeds = inputs.splitmix64(7, 8 * K).astype(np.uint32).reshape(K, 8)
+    chans = [vx.DeviceBuffer.from_host(np.concatenate([st, np.zeros(8, np.uint32)])) for st in seeds]
+    res = vx.DeviceBuffer(2 * K)
+
+    def run(n):
+        for i in range(n):
+            vx.grind_device(chans[i].addr, POW_BITS, res.addr + 8 * i)
+        vx.synchronize()
+
+    run(8)  # warm-up
+    best = None
+    for _ in range(3):
+        t = time.perf_counter()
+        run(K)
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    words = res.to_host(2 * K).astype(np.uint64)
+    nonces = [int(x) for x in (words[0::2] | (words[1::2] << np.uint64(32)))]
+    tried = sum(n + 1 for n in nonces)
+    rate = tried / best
+    achieved = rate * OPS_PER_NONCE / 1e12
+    # parity: every nonce valid; the first two are the smallest valid ones (oracle search)
+    valid = all(O.pow_zeros(O.pow_prefix(seeds[i], POW_BITS), nonces[i]) >= POW_BITS for i in range(K))
+    first = all(O.pow_search(O.pow_prefix(seeds[i], POW_BITS), POW_BITS, False, 0, nonces[i] + 1) == nonces[i]
+                for i in range(2))
+    log(f"grind: {K} x {POW_BITS}-bit PoW in {best * 1e3:.1f} ms ({best / K * 1e3:.3f} ms each), "
+        f"{rate / 1e9:.2f} G nonces/s, valid={valid} first={first}")
+    out = {"workload": f"{K} Blake2s PoW grinds at pow_bits {POW_BITS} (channel states splitmix64 seed 7), "
+                       "prefix + search + nonce on the device, one call each",
+           "ms_per_grind": round(best / K * 1e3, 4), "nonces_per_grind_mean": round(tried / K),
+           "g_nonces_per_s": round(rate / 1e9, 3),
+           "roofline": {"bound": "valu", "kernel": "grind_kernel<false,false,true>", "achieved": round(achieved, 3),
+                        "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s (int32 lane-ops)",
+                        "frac": round(achieved / VALU_PEAK_TOPS, 4), "ops_per_nonce": OPS_PER_NONCE,
+                        "peak_measured": round(MEASURED.get("valu_peak", 0.0), 3)},
+           "parity_vs_oracle": {"all_valid": valid, "first_two_smallest": first}}
+    if not args.no_cpu:
+        threads = int(O.lib().orc_get_threads())
+        sample = 1 << 27
+        pre = O.pow_prefix(seeds[0], POW_BITS)
+        t = time.perf_counter()
+        O.pow_search(pre, 129, False, 0, sample)  # unreachable target: every nonce hashed
+        secs = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(sample / secs / 1e9, 5), "unit": "G nonces/s", "cores": threads,
+                               "kind": "port", "sample": "2^27 nonces, target never met (oracle orc_pow_search)"}
+        log(f"grind cpu: {sample / secs / 1e6:.1f} M nonces/s on {threads} threads")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,7 +534,7 @@ def main() -> int:
     ap.add_argument("--no-pipelined", action="store_true", help="skip the two-stream plain measurement")
     ap.add_argument("--cpu-log-n", type=int, default=24)
     ap.add_argument("--cpu-reps", type=int, default=2)
-    ap.add_argument("--skip", default="", help="comma list of workloads to skip: plain,wide,batch,serve "
+    ap.add_argument("--skip", default="", help="comma list of workloads to skip: plain,wide,batch,serve,grind "
                                                "(skipping plain leaves value null: for profiling runs)")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="offered load, jobs/s")
@@ -499,7 +566,8 @@ def main() -> int:
             f"{ {k: v['logc'] for k, v in MEASURED['merkle_stage_plans'].items()} }")
     extra = {}
     if "plain" in skip:  # profiling runs of the other workloads only
-        for name, fn in (("wide", bench_wide), ("batch", bench_batch), ("serve", bench_serve)):
+        for name, fn in (("wide", bench_wide), ("batch", bench_batch), ("serve", bench_serve),
+                         ("grind", bench_grind)):
             if name not in skip:
                 extra[name] = fn(vx, args)
         print(json.dumps({"metric": METRIC, "value": None, "unit": "Melem/s", "n_gpus": 1, **extra}), flush=True)
@@ -511,6 +579,8 @@ def main() -> int:
         extra["batch"] = bench_batch(vx, args)
     if "serve" not in skip:
         extra["serve"] = bench_serve(vx, args)
+    if "grind" not in skip:
+        extra["grind"] = bench_grind(vx, args)
     log_n, nq, last = args.log_n, args.queries, args.log_last
     out = {
         "metric": METRIC,

@@ -152,6 +152,20 @@ void cuda_bit_reverse_qm31(const uint32_t* inp, uint32_t* out, uint32_t n, uint3
 void cuda_pack_leaves(const uint32_t* const* input_col_ptrs, uint32_t* const* output_col_ptrs,
                       uint32_t n);                                                                 /* ffi.rs:960 */
 
+/* Blake2s proof-of-work nonce search (ffi.rs:627-639, 965-976; cuda/grind.cu,
+ * cuda/grind_m31_output.cu). Tries the nonces [batch_offset, batch_offset +
+ * n_threads) against prefixed_digest (device, 8 words: Blake2s(0x12345678 LE ||
+ * 12 zero bytes || channel state || pow_bits LE), reduced word-wise mod p for the
+ * M31 channel) and lowers *result (device, set to 2^64 - 1 by the caller before
+ * the first batch) to the smallest nonce whose Blake2s(prefixed_digest || nonce
+ * LE) has >= pow_bits trailing zeros in its first 16 bytes (u128 LE; the M31
+ * variant reduces those four words mod p first). Nonces not below *result are
+ * skipped, so a batch after a success does no work. Asynchronous. */
+void cuda_grind_pow(const uint32_t* prefixed_digest, uint64_t* result, uint32_t pow_bits, uint64_t batch_offset,
+                    uint32_t n_threads);                                                           /* ffi.rs:632 */
+void cuda_grind_pow_m31_output(const uint32_t* prefixed_digest, uint64_t* result, uint32_t pow_bits,
+                               uint64_t batch_offset, uint32_t n_threads);                         /* ffi.rs:970 */
+
 /* ===================== Part 2: runtime (replaces ffi.rs:6-76 cuda* calls) ==== */
 
 int32_t vx_get_last_error(void);                 /* 0 = none; reading clears it */
@@ -201,6 +215,17 @@ int32_t vx_merkle_autotune(uint32_t log_min, uint32_t log_max, float* best_ms);
 int32_t vx_merkle_stage_plan(uint32_t log_items);
 int32_t vx_merkle_stage_plan_set(uint32_t log_items, int32_t logc);
 void vx_merkle_stage_plan_reset(void);
+
+/* Whole PoW step on the device (GrindOps::grind, grind_ops.rs:125-180, then
+ * Channel::mix_u64, channel.rs:117-128): channel is a device transcript whose
+ * first 8 words are the channel state (a vx DevChannel, 64 bytes, qualifies).
+ * The prefixed digest is formed on the device, *result (device) is set to the
+ * smallest valid nonce of [start, start + count) or 2^64 - 1, and with mix != 0
+ * a found nonce is mixed into the state (n_draws reset to 0). Asynchronous on
+ * stream (null: the default stream); one launch for any count, no host round
+ * trip. Returns 0, or -1 with the error recorded. */
+int32_t vx_grind_blake2s(void* channel, uint32_t pow_bits, int32_t m31_output, uint64_t start, uint64_t count,
+                         uint64_t* result, int32_t mix, void* stream);
 
 /* ===================== Part 3: fused device-resident prover ================= */
 

@@ -566,6 +566,14 @@ struct Channel {
         blake2s_single(in, 64, state);
         n_draws = 0;
     }
+    void mix_u64(uint64_t v) {  // :117-128, the PoW nonce: H(state || lo LE || hi LE)
+        uint8_t in[40];
+        memcpy(in, state, 32);
+        put_le32(in + 32, (uint32_t)v);
+        put_le32(in + 36, (uint32_t)(v >> 32));
+        blake2s_single(in, 40, state);
+        n_draws = 0;
+    }
     void mix_felts(const uint32_t* w, uint64_t n_felts) {  // :48-61
         if (n_felts == 0) return;
         std::vector<uint8_t> in(32 + 16 * n_felts);
@@ -1349,6 +1357,11 @@ void orc_channel_mix_digest(orc_channel* c, const uint32_t digest[8]) {
     ch.mix_digest(digest);
     from_ch(ch, c);
 }
+void orc_channel_mix_u64(orc_channel* c, uint64_t v) {
+    Channel ch = to_ch(c);
+    ch.mix_u64(v);
+    from_ch(ch, c);
+}
 void orc_channel_mix_felts(orc_channel* c, const uint32_t* felts, uint64_t n_felts) {
     Channel ch = to_ch(c);
     ch.mix_felts(felts, n_felts);
@@ -1474,6 +1487,71 @@ uint64_t orc_splitmix64(uint64_t* s) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
+}
+
+// ---- Blake2s proof-of-work grinding (src/channel.rs:140-168, cuda/grind.cu:13-70,
+// cuda/grind_m31_output.cu:14-68, crates/cuda-backend/src/grind_ops.rs:20-34, 146-161)
+
+// m31_reduce of a 32-bit word, step for step (cuda/include/m31.cuh:9-14): x mod p.
+static uint32_t m31_reduce64(uint64_t x) {
+    const uint32_t lo = (uint32_t)(x & P), hi = (uint32_t)(x >> 31);
+    const uint32_t r = lo + hi;
+    return r >= P ? r - P : r;
+}
+
+uint32_t orc_pow_reduce_word(uint32_t x) { return m31_reduce64(x); }
+
+// prefixed_digest = Blake2s(0x12345678 LE || 12 zero bytes || state || pow_bits LE), 52 bytes
+// (channel.rs:150-155). With m31_output every word of the digest is reduced mod p
+// (the reduction grind_ops.rs:161 applies before the search).
+void orc_pow_prefix(const uint32_t state[8], uint32_t pow_bits, int m31_output, uint32_t out[8]) {
+    uint8_t in[52] = {0};
+    const uint32_t prefix = 0x12345678u;
+    memcpy(in, &prefix, 4);
+    memcpy(in + 16, state, 32);
+    memcpy(in + 48, &pow_bits, 4);
+    uint8_t h[32];
+    blake2s_full(in, 52, h);
+    memcpy(out, h, 32);
+    if (m31_output)
+        for (int i = 0; i < 8; i++) out[i] = m31_reduce64(out[i]);
+}
+
+// Trailing zeros of the first 16 bytes (u128 LE) of Blake2s(prefixed_digest || nonce LE),
+// 40 bytes (channel.rs:157-167, grind.cu:56-69); with m31_output the four words are
+// reduced mod p first (grind_m31_output.cu:48-66).
+uint32_t orc_pow_zeros(const uint32_t prefixed[8], uint64_t nonce, int m31_output) {
+    uint8_t in[40];
+    memcpy(in, prefixed, 32);
+    memcpy(in + 32, &nonce, 8);
+    uint8_t h[32];
+    blake2s_full(in, 40, h);
+    uint32_t w[4];
+    memcpy(w, h, 16);
+    for (int i = 0; i < 4; i++) {
+        if (m31_output) w[i] = m31_reduce64(w[i]);
+        if (w[i]) return 32u * i + (uint32_t)__builtin_ctz(w[i]);
+    }
+    return 128;
+}
+
+// Smallest nonce in [start, start + count) whose hash has >= pow_bits trailing zeros, or
+// UINT64_MAX. Threads split the range; the minimum over them is the answer.
+uint64_t orc_pow_search(const uint32_t prefixed[8], uint32_t pow_bits, int m31_output, uint64_t start,
+                        uint64_t count) {
+    std::atomic<uint64_t> best{UINT64_MAX};
+    parallel_for(count, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; i++) {
+            if (start + i >= best.load(std::memory_order_relaxed)) return;
+            if (orc_pow_zeros(prefixed, start + i, m31_output) >= pow_bits) {
+                uint64_t cur = best.load();
+                while (start + i < cur && !best.compare_exchange_weak(cur, start + i)) {
+                }
+                return;
+            }
+        }
+    });
+    return best.load();
 }
 
 }  // extern "C"

@@ -93,6 +93,7 @@ typedef struct {
 void orc_channel_init(orc_channel* ch);
 void orc_channel_mix_digest(orc_channel* ch, const uint32_t digest[8]);
 void orc_channel_mix_felts(orc_channel* ch, const uint32_t* felts, uint64_t n_felts);
+void orc_channel_mix_u64(orc_channel* ch, uint64_t v);
 void orc_channel_draw_felt(orc_channel* ch, uint32_t out[4]);
 uint64_t orc_channel_draw_number(orc_channel* ch, uint64_t bound);
 
@@ -128,6 +129,14 @@ uint32_t orc_wide_commit(const uint32_t* const* trace, const uint32_t* log_sizes
                          uint32_t log_last, uint32_t* root8, uint32_t* fri_roots, uint32_t* last, uint32_t* beta4);
 double orc_time_wide_commit(const uint32_t* const* trace, const uint32_t* log_sizes, uint32_t n_cols, uint32_t blowup,
                             uint32_t log_last);
+
+/* ---- Blake2s proof-of-work grinding (channel.rs:140-168, cuda/grind*.cu) ---- */
+uint32_t orc_pow_reduce_word(uint32_t x); /* m31_reduce of one digest word (m31.cuh:9-14) */
+void orc_pow_prefix(const uint32_t state[8], uint32_t pow_bits, int m31_output, uint32_t out[8]);
+uint32_t orc_pow_zeros(const uint32_t prefixed[8], uint64_t nonce, int m31_output);
+/* smallest nonce in [start, start + count) with >= pow_bits zeros, else UINT64_MAX */
+uint64_t orc_pow_search(const uint32_t prefixed[8], uint32_t pow_bits, int m31_output, uint64_t start,
+                        uint64_t count);
 
 /* ---- splitmix64 input generator ---- */
 uint64_t orc_splitmix64(uint64_t* state);

@@ -56,6 +56,7 @@ _SIGS = {
     "orc_channel_init": (None, [C.POINTER(ChannelStruct)]),
     "orc_channel_mix_digest": (None, [C.POINTER(ChannelStruct), P32]),
     "orc_channel_mix_felts": (None, [C.POINTER(ChannelStruct), P32, u64]),
+    "orc_channel_mix_u64": (None, [C.POINTER(ChannelStruct), u64]),
     "orc_channel_draw_felt": (None, [C.POINTER(ChannelStruct), P32]),
     "orc_channel_draw_number": (u64, [C.POINTER(ChannelStruct), u64]),
     "orc_prove_fibonacci": (u64, [u32, u32, u32, u32, u32, vp, u64, P32]),
@@ -71,6 +72,10 @@ _SIGS = {
     "orc_wide_commit": (u32, [vp, vp, u32, u32, u32, P32, vp, vp, vp]),
     "orc_time_wide_commit": (C.c_double, [vp, vp, u32, u32, u32]),
     "orc_splitmix64": (u64, [C.POINTER(u64)]),
+    "orc_pow_reduce_word": (u32, [u32]),
+    "orc_pow_prefix": (None, [P32, u32, i32, P32]),
+    "orc_pow_zeros": (u32, [P32, u64, i32]),
+    "orc_pow_search": (u64, [P32, u32, i32, u64, u64]),
 }
 
 _lib = None
@@ -237,3 +242,19 @@ def wide_commit(cols, log_sizes, log_blowup: int, log_last: int):
                                last.ctypes.data, beta.ctypes.data)
     assert nf == S - log_last, "oracle rejected the wide-commit shape"
     return root, fri[: 8 * nf].reshape(-1, 8), last.reshape(-1, 4), beta
+
+
+def pow_prefix(state, pow_bits: int, m31_output: bool = False) -> np.ndarray:
+    """prefixed_digest words of a channel state (8 words) for a PoW of pow_bits."""
+    st, out = arr(state), np.zeros(8, np.uint32)
+    lib().orc_pow_prefix(p32(st), pow_bits, int(m31_output), p32(out))
+    return out
+
+
+def pow_zeros(prefixed, nonce: int, m31_output: bool = False) -> int:
+    return lib().orc_pow_zeros(p32(arr(prefixed)), nonce, int(m31_output))
+
+
+def pow_search(prefixed, pow_bits: int, m31_output: bool = False, start: int = 0, count: int = 1 << 24) -> int:
+    """Smallest valid nonce in [start, start + count), or 2^64 - 1."""
+    return lib().orc_pow_search(p32(arr(prefixed)), pow_bits, int(m31_output), start, count)

@@ -21,7 +21,8 @@ def test_header_declares_reference_symbols():
                  "cuda_fold_line_soa", "cuda_fold_circle_into_line_soa", "cuda_fibonacci_quotient",
                  "cuda_batch_inverse_m31", "vx_prover_create", "vx_prover_prove_device",
                  "cuda_stwo_ntt_evaluate", "cuda_stwo_ntt_interpolate", "cuda_build_leaves_lifted",
-                 "cuda_pack_leaves", "cuda_bit_reverse_qm31", "vx_wide_create", "vx_wide_commit_device"]:
+                 "cuda_pack_leaves", "cuda_bit_reverse_qm31", "vx_wide_create", "vx_wide_commit_device",
+                 "cuda_grind_pow", "cuda_grind_pow_m31_output", "vx_grind_blake2s"]:
         assert must in names
 
 

@@ -417,6 +417,36 @@ def merkle_stage_plan_set(log_items: int, logc: int) -> None:
         raise HipError("vx_merkle_stage_plan_set failed")
 
 
+NONCE_NONE = (1 << 64) - 1
+
+
+def grind(state, pow_bits: int, m31_output: bool = False, start: int = 0, count: int = NONCE_NONE):
+    """Proof-of-work nonce for a Blake2s channel state (8 words, host), as
+    GrindOps::grind (grind_ops.rs:125-180): the smallest nonce of
+    [start, start + count) whose Blake2s(prefixed_digest || nonce) has >= pow_bits
+    trailing zeros, searched on the device in one launch; None if the range has none."""
+    ch = DeviceBuffer.from_host(np.concatenate([np.asarray(state, np.uint32), np.zeros(8, np.uint32)]))
+    res = DeviceBuffer(2)
+    if lib.vx_grind_blake2s(ch.ptr, pow_bits, int(m31_output), start, count, res.ptr, 0, None):
+        check("vx_grind_blake2s")
+        raise HipError("vx_grind_blake2s failed")
+    synchronize()
+    lo, hi = res.to_host(2)
+    nonce = int(lo) | (int(hi) << 32)
+    return None if nonce == NONCE_NONE else nonce
+
+
+def grind_device(channel_addr: int, pow_bits: int, result_addr: int, m31_output: bool = False, start: int = 0,
+                 count: int = NONCE_NONE, mix: bool = False, stream: int = 0) -> None:
+    """Asynchronous device PoW step on a device transcript (first 8 words = state):
+    the nonce goes to result_addr (u64, 2^64 - 1 if none); mix = True then mixes it
+    into the transcript as Channel::mix_u64 (channel.rs:117-128)."""
+    if lib.vx_grind_blake2s(C.c_void_p(channel_addr), pow_bits, int(m31_output), start, count,
+                            C.c_void_p(result_addr), int(mix), C.c_void_p(stream) if stream else None):
+        check("vx_grind_blake2s")
+        raise HipError("vx_grind_blake2s failed")
+
+
 def measure_valu_peak() -> float:
     """int32 VALU issue rate of the device, T lane-ops/s (microbenchmark)."""
     tops = C.c_double(0.0)
