@@ -152,6 +152,34 @@ void cuda_bit_reverse_qm31(const uint32_t* inp, uint32_t* out, uint32_t n, uint3
 void cuda_pack_leaves(const uint32_t* const* input_col_ptrs, uint32_t* const* output_col_ptrs,
                       uint32_t n);                                                                 /* ffi.rs:960 */
 
+/* stwo DEEP quotient (ffi.rs:840-897, 957-958; cuda/quotient.cu, cuda/accumulate_lift.cu;
+ * callers quotient_ops.rs:35-223, accumulation_ops.rs:46-88). All pointers are device
+ * pointers; col_ptrs / numer_ptrs* are device arrays of device pointers; QM31 values
+ * are 4 M31 words {a.re, a.im, b.re, b.im} of a + b u.
+ * accumulate_numerators: out*[row] = sum_i (c_i * col_ptrs[col_indices[i]][row] - b_i)
+ *   (SoA QM31 out, n_rows each).
+ * compute_quotients_combine: for each row of the 2^lifting_log_size domain (points
+ *   (domain_xs[row], domain_ys[row]), bit-reversed canonic order),
+ *   out*[row] = sum_j (numer_j[lift_j(row)] - first_linear_acc_j * y) / den_j, with
+ *   den_j = (Re(px_j) - x) Im(py_j) - (Re(py_j) - y) Im(px_j) in CM31 (a zero den
+ *   contributes 0) and lift_j(row) = ((row >> (lifting_log_size - acc_log_sizes[j] + 1)) << 1)
+ *   + (row & 1).
+ * accumulate_lift: col[i] += curr[((i >> (log_ratio + 1)) << 1) | (i & 1)] for i < col_n,
+ *   col and curr arrays of QM31 elements (4 consecutive words each).
+ * Asynchronous on the stream set with vx_set_stream. */
+void cuda_accumulate_numerators(const uint32_t* const* col_ptrs, const uint32_t* col_indices,
+                                const uint32_t* b_coeffs, const uint32_t* c_coeffs, uint32_t n_batch_cols,
+                                uint32_t n_rows, uint32_t* out0, uint32_t* out1, uint32_t* out2,
+                                uint32_t* out3);                                                   /* ffi.rs:850 */
+void cuda_compute_quotients_combine(const uint32_t* sample_points_x, const uint32_t* sample_points_y,
+                                    const uint32_t* first_linear_acc, const uint32_t* const* numer_ptrs0,
+                                    const uint32_t* const* numer_ptrs1, const uint32_t* const* numer_ptrs2,
+                                    const uint32_t* const* numer_ptrs3, const uint32_t* acc_log_sizes,
+                                    uint32_t n_accs, const uint32_t* domain_xs, const uint32_t* domain_ys,
+                                    uint32_t lifting_log_size, uint32_t n_rows, uint32_t* out0,
+                                    uint32_t* out1, uint32_t* out2, uint32_t* out3);               /* ffi.rs:878 */
+void cuda_accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, uint32_t log_ratio); /* ffi.rs:958 */
+
 /* Blake2s proof-of-work nonce search (ffi.rs:627-639, 965-976; cuda/grind.cu,
  * cuda/grind_m31_output.cu). Tries the nonces [batch_offset, batch_offset +
  * n_threads) against prefixed_digest (device, 8 words: Blake2s(0x12345678 LE ||

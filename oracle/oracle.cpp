@@ -1554,4 +1554,84 @@ uint64_t orc_pow_search(const uint32_t prefixed[8], uint32_t pow_bits, int m31_o
     return best.load();
 }
 
+// ---- stwo DEEP quotient pieces (cuda/quotient.cu:25-253, cuda/accumulate_lift.cu:13-51;
+// callers crates/cuda-backend/src/quotient_ops.rs:35-223, accumulation_ops.rs:46-88)
+
+// result[row] = sum_i (c_i * col[col_indices[i]][row] - b_i)   (quotient.cu:25-80)
+void orc_accumulate_numerators(const uint32_t* const* cols, const uint32_t* col_indices, const uint32_t* b,
+                               const uint32_t* c, uint32_t n_batch, uint32_t n_rows, uint32_t* out0,
+                               uint32_t* out1, uint32_t* out2, uint32_t* out3) {
+    parallel_for(n_rows, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t row = lo; row < hi; row++) {
+            QM31 acc = {{0, 0}, {0, 0}};
+            for (uint32_t i = 0; i < n_batch; i++) {
+                const uint32_t f = cols[col_indices[i]][row];
+                acc = qadd(acc, qsub(qmul_m31(qfrom(c + 4 * i), f), qfrom(b + 4 * i)));
+            }
+            uint32_t v[4];
+            qto(acc, v);
+            out0[row] = v[0];
+            out1[row] = v[1];
+            out2[row] = v[2];
+            out3[row] = v[3];
+        }
+    });
+}
+
+// quotient[row] = sum_j (numer_j[lifted(row)] - a_j * dy) * den_j^-1, with the CM31
+// den_j = (Re(px) - dx) * Im(py) - (Re(py) - dy) * Im(px)          (quotient.cu:82-205)
+void orc_quotients_combine(const uint32_t* spx, const uint32_t* spy, const uint32_t* fla,
+                           const uint32_t* const* n0, const uint32_t* const* n1, const uint32_t* const* n2,
+                           const uint32_t* const* n3, const uint32_t* acc_logs, uint32_t n_accs,
+                           const uint32_t* dxs, const uint32_t* dys, uint32_t lifting_log, uint32_t n_rows,
+                           uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3) {
+    parallel_for(n_rows, [&](uint64_t lo, uint64_t hi) {
+        for (uint64_t row = lo; row < hi; row++) {
+            const uint32_t dx = dxs[row], dy = dys[row];
+            QM31 q = {{0, 0}, {0, 0}};
+            for (uint32_t j = 0; j < n_accs; j++) {
+                const QM31 px = qfrom(spx + 4 * j), py = qfrom(spy + 4 * j);
+                const CM31 t1 = cmul(csub(px.a, {dx, 0}), py.b);
+                const CM31 t2 = cmul(csub(py.a, {dy, 0}), px.b);
+                const CM31 dinv = cinv(csub(t1, t2));
+                const uint32_t r = lifting_log - acc_logs[j];
+                const uint64_t li = ((row >> (r + 1)) << 1) + (row & 1);
+                const uint32_t nv[4] = {n0[j][li], n1[j][li], n2[j][li], n3[j][li]};
+                const QM31 full = qsub(qfrom(nv), qmul_m31(qfrom(fla + 4 * j), dy));
+                q = qadd(q, {cmul(full.a, dinv), cmul(full.b, dinv)});
+            }
+            uint32_t v[4];
+            qto(q, v);
+            out0[row] = v[0];
+            out1[row] = v[1];
+            out2[row] = v[2];
+            out3[row] = v[3];
+        }
+    });
+}
+
+// col[i] += curr[((i >> (log_ratio + 1)) << 1) | (i & 1)], QM31 elements of 4 words
+// (accumulate_lift.cu:15-40)
+void orc_accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, uint32_t log_ratio) {
+    for (uint64_t i = 0; i < col_n; i++) {
+        const uint64_t s = ((i >> (log_ratio + 1)) << 1) | (i & 1);
+        const QM31 r = qadd(qfrom(col + 4 * i), qfrom(curr + 4 * s));
+        qto(r, col + 4 * i);
+    }
+}
+
+// Points of CanonicCoset(log).circle_domain() at bit_reverse_index(row) for every row
+// (quotient_ops.rs:140-147): natural index k < n/2 is half_odds(log - 1).at(k), the
+// rest are the conjugates (x, -y) of the first half.
+void orc_circle_domain_brev(uint32_t log_n, uint32_t* xs, uint32_t* ys) {
+    const uint64_t n = 1ull << log_n, half = n / 2;
+    std::vector<uint32_t> hx, hy;
+    all_points(half_odds(log_n - 1), hx, hy);
+    for (uint64_t row = 0; row < n; row++) {
+        const uint64_t k = bitrev((uint32_t)row, log_n);
+        xs[row] = hx[k < half ? k : k - half];
+        ys[row] = k < half ? hy[k] : neg(hy[k - half]);
+    }
+}
+
 }  // extern "C"

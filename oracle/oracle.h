@@ -138,6 +138,18 @@ uint32_t orc_pow_zeros(const uint32_t prefixed[8], uint64_t nonce, int m31_outpu
 uint64_t orc_pow_search(const uint32_t prefixed[8], uint32_t pow_bits, int m31_output, uint64_t start,
                         uint64_t count);
 
+/* ---- stwo DEEP quotient pieces (cuda/quotient.cu, cuda/accumulate_lift.cu) ---- */
+void orc_accumulate_numerators(const uint32_t* const* cols, const uint32_t* col_indices, const uint32_t* b,
+                               const uint32_t* c, uint32_t n_batch, uint32_t n_rows, uint32_t* out0,
+                               uint32_t* out1, uint32_t* out2, uint32_t* out3);
+void orc_quotients_combine(const uint32_t* spx, const uint32_t* spy, const uint32_t* fla,
+                           const uint32_t* const* n0, const uint32_t* const* n1, const uint32_t* const* n2,
+                           const uint32_t* const* n3, const uint32_t* acc_logs, uint32_t n_accs,
+                           const uint32_t* dxs, const uint32_t* dys, uint32_t lifting_log, uint32_t n_rows,
+                           uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3);
+void orc_accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, uint32_t log_ratio);
+void orc_circle_domain_brev(uint32_t log_n, uint32_t* xs, uint32_t* ys);
+
 /* ---- splitmix64 input generator ---- */
 uint64_t orc_splitmix64(uint64_t* state);
 

@@ -72,6 +72,10 @@ _SIGS = {
     "orc_wide_commit": (u32, [vp, vp, u32, u32, u32, P32, vp, vp, vp]),
     "orc_time_wide_commit": (C.c_double, [vp, vp, u32, u32, u32]),
     "orc_splitmix64": (u64, [C.POINTER(u64)]),
+    "orc_accumulate_numerators": (None, [vp, vp, vp, vp, u32, u32, vp, vp, vp, vp]),
+    "orc_quotients_combine": (None, [vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp, u32, u32, vp, vp, vp, vp]),
+    "orc_accumulate_lift": (None, [vp, vp, u32, u32]),
+    "orc_circle_domain_brev": (None, [u32, vp, vp]),
     "orc_pow_reduce_word": (u32, [u32]),
     "orc_pow_prefix": (None, [P32, u32, i32, P32]),
     "orc_pow_zeros": (u32, [P32, u64, i32]),
@@ -258,3 +262,39 @@ def pow_zeros(prefixed, nonce: int, m31_output: bool = False) -> int:
 def pow_search(prefixed, pow_bits: int, m31_output: bool = False, start: int = 0, count: int = 1 << 24) -> int:
     """Smallest valid nonce in [start, start + count), or 2^64 - 1."""
     return lib().orc_pow_search(p32(arr(prefixed)), pow_bits, int(m31_output), start, count)
+
+
+def accumulate_numerators(cols, col_indices, b, c, n_rows: int) -> np.ndarray:
+    """[4, n_rows] SoA partial numerators sum_i (c_i * col[idx_i][row] - b_i)."""
+    ptrs, keep = ptr_array(cols)
+    idx, bb, cc = arr(col_indices), arr(b), arr(c)
+    out = np.zeros((4, n_rows), np.uint32)
+    lib().orc_accumulate_numerators(ptrs, idx.ctypes.data, bb.ctypes.data, cc.ctypes.data, len(idx), n_rows,
+                                    *[out[k].ctypes.data for k in range(4)])
+    return out
+
+
+def quotients_combine(spx, spy, fla, numers, acc_logs, dxs, dys, lifting_log: int) -> np.ndarray:
+    """[4, n_rows] combined quotient; numers[j] is a [4, 2^acc_logs[j]] SoA array."""
+    numers = [np.ascontiguousarray(n, dtype=np.uint32) for n in numers]
+    pk = [(vp * len(numers))(*[n[k].ctypes.data for n in numers]) for k in range(4)]
+    a, b, f, lg = arr(spx), arr(spy), arr(fla), arr(acc_logs)
+    dx, dy = arr(dxs), arr(dys)
+    out = np.zeros((4, dx.size), np.uint32)
+    lib().orc_quotients_combine(a.ctypes.data, b.ctypes.data, f.ctypes.data, pk[0], pk[1], pk[2], pk[3],
+                                lg.ctypes.data, len(numers), dx.ctypes.data, dy.ctypes.data, lifting_log, dx.size,
+                                *[out[k].ctypes.data for k in range(4)])
+    return out
+
+
+def accumulate_lift(col, curr, log_ratio: int) -> np.ndarray:
+    """col (n x 4 QM31, AoS) += curr at the lifted index; returns the new col."""
+    c = np.ascontiguousarray(col, dtype=np.uint32).copy()
+    lib().orc_accumulate_lift(c.ctypes.data, arr(curr).ctypes.data, c.size // 4, log_ratio)
+    return c
+
+
+def circle_domain_brev(log_n: int):
+    xs, ys = np.zeros(1 << log_n, np.uint32), np.zeros(1 << log_n, np.uint32)
+    lib().orc_circle_domain_brev(log_n, xs.ctypes.data, ys.ctypes.data)
+    return xs, ys

@@ -1,0 +1,149 @@
+"""stwo DEEP-quotient kernels on the MI355X against the oracle, bit-exact
+(cuda/quotient.cu, cuda/accumulate_lift.cu; ffi.rs:840-897, 957-958): edge
+sizes (1 and 2 rows, rows not a multiple of a workgroup's), empty batches,
+repeated columns, zero denominators, misaligned QM31 arrays, and the wide
+commitment's sizes (2^16, 2^20, 2^22 lifted to 2^22)."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+P = (1 << 31) - 1
+
+
+def rnd(rng, *shape):
+    return rng.integers(0, P, shape, dtype=np.uint64).astype(np.uint32)
+
+
+def dev_ptrs(vx, bufs):
+    addrs = np.array([b.addr for b in bufs], dtype=np.uint64)
+    d = vx.DeviceBuffer(2 * max(1, len(bufs)))
+    if len(bufs):
+        vx.lib.vx_memcpy_h2d(d.ptr, addrs.ctypes.data, 8 * len(bufs))
+    return d
+
+
+def gpu_numerators(vx, cols, idx, b, c, n_rows):
+    dcols = [vx.DeviceBuffer.from_host(x) for x in cols]
+    dp = dev_ptrs(vx, dcols)
+    di = vx.DeviceBuffer.from_host(np.asarray(idx, np.uint32) if len(idx) else np.zeros(1, np.uint32))
+    db = vx.DeviceBuffer.from_host(np.asarray(b, np.uint32) if len(b) else np.zeros(4, np.uint32))
+    dc = vx.DeviceBuffer.from_host(np.asarray(c, np.uint32) if len(c) else np.zeros(4, np.uint32))
+    outs = [vx.DeviceBuffer(n_rows) for _ in range(4)]
+    vx.lib.cuda_accumulate_numerators(dp.ptr, di.ptr, db.ptr, dc.ptr, len(idx), n_rows, *[o.ptr for o in outs])
+    vx.synchronize()
+    vx.check("cuda_accumulate_numerators")
+    return np.stack([o.to_host() for o in outs])
+
+
+@pytest.mark.parametrize("n_rows,n_batch", [(1, 1), (2, 0), (255, 2), (257, 3), (1000, 4), (1 << 12, 7),
+                                            (1 << 20, 5)])
+def test_accumulate_numerators(vx, n_rows, n_batch):
+    rng = np.random.default_rng(n_rows + n_batch)
+    cols = [rnd(rng, n_rows) for _ in range(6)]
+    idx = rng.integers(0, 6, n_batch).astype(np.uint32)  # repeats allowed
+    b, c = rnd(rng, 4 * n_batch), rnd(rng, 4 * n_batch)
+    exp = O.accumulate_numerators(cols, idx, b, c, n_rows)
+    got = gpu_numerators(vx, cols, idx, b, c, n_rows)
+    assert np.array_equal(got, exp)
+
+
+def gpu_combine(vx, spx, spy, fla, numers, acc_logs, dxs, dys, L, n_rows):
+    keep = []
+    ptr_bufs = []
+    for k in range(4):
+        bufs = [vx.DeviceBuffer.from_host(n[k]) for n in numers]
+        keep += bufs
+        ptr_bufs.append(dev_ptrs(vx, bufs))
+    d = [vx.DeviceBuffer.from_host(np.asarray(x, np.uint32)) for x in (spx, spy, fla, acc_logs, dxs, dys)]
+    outs = [vx.DeviceBuffer(n_rows) for _ in range(4)]
+    vx.lib.cuda_compute_quotients_combine(d[0].ptr, d[1].ptr, d[2].ptr, *[p.ptr for p in ptr_bufs], d[3].ptr,
+                                          len(numers), d[4].ptr, d[5].ptr, L, n_rows, *[o.ptr for o in outs])
+    vx.synchronize()
+    vx.check("cuda_compute_quotients_combine")
+    return np.stack([o.to_host() for o in outs])
+
+
+@pytest.mark.parametrize("L,acc_logs,n_rows", [(1, [1], 2), (3, [1, 3], 8), (6, [6, 4, 2, 6], 64),
+                                               (10, [10, 7], 1000), (12, [12, 9, 12], 1 << 12),
+                                               (20, [20, 16, 18], 1 << 20)])
+def test_quotients_combine(vx, L, acc_logs, n_rows):
+    rng = np.random.default_rng(L * 7 + n_rows)
+    n = len(acc_logs)
+    spx, spy, fla = rnd(rng, 4 * n), rnd(rng, 4 * n), rnd(rng, 4 * n)
+    numers = [rnd(rng, 4, 1 << l) for l in acc_logs]
+    dxs, dys = O.circle_domain_brev(L)
+    dxs, dys = dxs[:n_rows], dys[:n_rows]
+    exp = O.quotients_combine(spx, spy, fla, numers, acc_logs, dxs, dys, L)
+    got = gpu_combine(vx, spx, spy, fla, numers, acc_logs, dxs, dys, L, n_rows)
+    assert np.array_equal(got, exp)
+
+
+def test_quotients_combine_zero_denominators(vx):
+    """A sample point with M31 coordinates makes every denominator 0 (its terms
+    vanish, as cm31_inv(0) = 0 gives); a domain point placed on a sample line makes
+    one row's denominator 0 among nonzero neighbours."""
+    rng = np.random.default_rng(5)
+    L = 9
+    dxs, dys = O.circle_domain_brev(L)
+    spx, spy, fla = rnd(rng, 8), rnd(rng, 8), rnd(rng, 8)
+    spx[2:4] = 0  # accumulation 0: Im(px) = Im(py) = 0
+    spy[2:4] = 0
+    # accumulation 1: choose Re(px), Re(py) so that den vanishes at row 37:
+    # (Re(px) - x) Im(py) = (Re(py) - y) Im(px) with Re(px) = x, Re(py) = y
+    spx[4:6] = [dxs[37], 0]
+    spy[4:6] = [dys[37], 0]
+    numers = [rnd(rng, 4, 1 << L), rnd(rng, 4, 1 << (L - 2))]
+    exp = O.quotients_combine(spx, spy, fla, numers, [L, L - 2], dxs, dys, L)
+    got = gpu_combine(vx, spx, spy, fla, numers, [L, L - 2], dxs, dys, L, 1 << L)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("misaligned", [False, True])
+@pytest.mark.parametrize("col_log,curr_log", [(1, 1), (3, 1), (10, 4), (18, 18), (20, 12)])
+def test_accumulate_lift(vx, col_log, curr_log, misaligned):
+    rng = np.random.default_rng(col_log * 31 + curr_log)
+    col, curr = rnd(rng, 1 << col_log, 4), rnd(rng, 1 << curr_log, 4)
+    exp = O.accumulate_lift(col, curr, col_log - curr_log)
+    off = 1 if misaligned else 0  # words: 4-byte aligned, not 16-byte aligned
+    dcol = vx.DeviceBuffer.from_host(np.concatenate([np.zeros(off, np.uint32), col.ravel()]))
+    dcur = vx.DeviceBuffer.from_host(np.concatenate([np.zeros(off, np.uint32), curr.ravel()]))
+    vx.lib.cuda_accumulate_lift(C.c_void_p(dcol.addr + 4 * off), C.c_void_p(dcur.addr + 4 * off), 1 << col_log,
+                                col_log - curr_log)
+    vx.synchronize()
+    vx.check("cuda_accumulate_lift")
+    assert np.array_equal(dcol.to_host()[off:], exp.ravel())
+
+
+def test_wide_shape_end_to_end(vx):
+    """The wide commitment's LDE sizes (4 x 2^16, 10 x 2^20, 20 x 2^22), two sample
+    points, a batch per (point, size): numerators and the combined 2^22-row quotient
+    against the oracle."""
+    rng = np.random.default_rng(11)
+    sizes = [16] * 4 + [20] * 10 + [22] * 20
+    L = 22
+    batches = []
+    for pt in range(2):
+        for s in (16, 20, 22):
+            cols = [rnd(rng, 1 << s) for i, l in enumerate(sizes) if l == s]
+            n = len(cols)
+            batches.append((s, cols, rnd(rng, 4 * n), rnd(rng, 4 * n), pt))
+    spx_pts, spy_pts = rnd(rng, 2, 4), rnd(rng, 2, 4)
+    numers, exp_numers = [], []
+    for s, cols, b, c, _ in batches:
+        idx = np.arange(len(cols), dtype=np.uint32)
+        exp_numers.append(O.accumulate_numerators(cols, idx, b, c, 1 << s))
+        numers.append(gpu_numerators(vx, cols, idx, b, c, 1 << s))
+        assert np.array_equal(numers[-1], exp_numers[-1]), s
+    spx = np.concatenate([spx_pts[pt] for *_, pt in batches])
+    spy = np.concatenate([spy_pts[pt] for *_, pt in batches])
+    fla = rnd(rng, 4 * len(batches))
+    logs = [s for s, *_ in batches]
+    dxs, dys = O.circle_domain_brev(L)
+    exp = O.quotients_combine(spx, spy, fla, exp_numers, logs, dxs, dys, L)
+    got = gpu_combine(vx, spx, spy, fla, numers, logs, dxs, dys, L, 1 << L)
+    assert np.array_equal(got, exp)

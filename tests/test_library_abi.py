@@ -22,7 +22,8 @@ def test_header_declares_reference_symbols():
                  "cuda_batch_inverse_m31", "vx_prover_create", "vx_prover_prove_device",
                  "cuda_stwo_ntt_evaluate", "cuda_stwo_ntt_interpolate", "cuda_build_leaves_lifted",
                  "cuda_pack_leaves", "cuda_bit_reverse_qm31", "vx_wide_create", "vx_wide_commit_device",
-                 "cuda_grind_pow", "cuda_grind_pow_m31_output", "vx_grind_blake2s"]:
+                 "cuda_grind_pow", "cuda_grind_pow_m31_output", "vx_grind_blake2s",
+                 "cuda_accumulate_numerators", "cuda_compute_quotients_combine", "cuda_accumulate_lift"]:
         assert must in names
 
 

@@ -53,6 +53,10 @@ SIGNATURES = {
     "cuda_merkle_tiled_soa4": (None, [vp, vp, vp, vp, vp, u32]),
     "cuda_merkle_tiled_soa4_stream": (None, [vp, vp, vp, vp, vp, u32, vp]),
     "cuda_merkle_tiled_generic": (None, [vp, vp, u32, u32]),
+    "cuda_accumulate_numerators": (None, [vp, vp, vp, vp, u32, u32, vp, vp, vp, vp]),
+    "cuda_compute_quotients_combine": (None, [vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp, u32, u32,
+                                              vp, vp, vp, vp]),
+    "cuda_accumulate_lift": (None, [vp, vp, u32, u32]),
     "cuda_grind_pow": (None, [vp, vp, u32, u64, u32]),
     "cuda_grind_pow_m31_output": (None, [vp, vp, u32, u64, u32]),
     "vx_grind_blake2s": (i32, [vp, u32, i32, u64, u64, vp, i32, vp]),
