@@ -15,11 +15,13 @@ The same JSON line carries the other workloads of the metric:
           batched proving (graph replay) vs the same proofs one at a time;
   serve   Poisson arrivals of jobs with 1-34 columns, log_n uniform in
           [12, 26], admitted under the HBM budget: proofs/s and p50/p99;
+  deep    stwo DEEP quotient of the wide shape's 34 LDE columns at two
+          sample points on the 2^22 lifting domain;
   grind   256 Blake2s proof-of-work searches at 26 bits from channel
           states on the device: nonces/s against the VALU roofline.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
-                    [--skip wide,batch,serve,grind]
+                    [--skip wide,batch,serve,deep,grind]
 """
 
 from __future__ import annotations
@@ -54,6 +56,10 @@ OPS_PER_BUTTERFLY = 13
 # word 0 is formed
 OPS_PER_NONCE = 847
 POW_BITS = 26  # the reference transcripts' PoW (prover.rs:96)
+# VALU instructions per (row, accumulation) of quotients_combine_kernel in its
+# gfx950 ISA: denominator 31, numerator, product and sum 163, M31 inversion
+# chain and batch-inverse steps shared by the thread's 8 rows 28
+OPS_PER_ROW_ACC = 222
 
 
 def load_traffic() -> dict:
@@ -463,6 +469,112 @@ def bench_serve(vx, args) -> dict:
             "peak_device_bytes": st["peak_device_bytes"], "batches": st["batches"]}
 
 
+def bench_deep(vx, args) -> dict:
+    """stwo DEEP quotient of the wide commitment's LDE columns (4 x 2^16, 10 x 2^20,
+    20 x 2^22, splitmix64 seed 8) at two sample points: a batch per (point, size),
+    six cuda_accumulate_numerators launches, then cuda_compute_quotients_combine
+    over the 2^22-row lifting domain. HIP events on the null stream time each part."""
+    import inputs
+    import oracle_lib as O
+
+    sizes = [16] * 4 + [20] * 10 + [22] * 20
+    L = 22
+    cols = inputs.m31_columns(8, sizes)
+    rng = np.random.default_rng(8)
+    q = lambda n: rng.integers(0, vx_p(), n, dtype=np.uint64).astype(np.uint32)
+    dcols = [vx.DeviceBuffer.from_host(c) for c in cols]
+    keep = []
+
+    def dev(a):
+        b = vx.DeviceBuffer.from_host(np.ascontiguousarray(a, dtype=np.uint32))
+        keep.append(b)
+        return b
+
+    def dev_ptrs(bufs):
+        addrs = np.array([b.addr for b in bufs], dtype=np.uint64)
+        return dev(addrs.view(np.uint32))
+
+    pts = [(q(4), q(4)), (q(4), q(4))]
+    batches = []  # (log size, column indices, b, c, point)
+    for pt in range(2):
+        for lg in (16, 20, 22):
+            idx = np.array([i for i, l in enumerate(sizes) if l == lg], np.uint32)
+            batches.append((lg, idx, q(4 * len(idx)), q(4 * len(idx)), pt))
+    d_colptrs = dev_ptrs(dcols)
+    numer = [[vx.DeviceBuffer(1 << lg) for _ in range(4)] for lg, *_ in batches]
+    d_args = [(dev(idx), dev(b), dev(c)) for _, idx, b, c, _ in batches]
+    spx = np.concatenate([pts[pt][0] for *_, pt in batches])
+    spy = np.concatenate([pts[pt][1] for *_, pt in batches])
+    fla = q(4 * len(batches))
+    logs = np.array([lg for lg, *_ in batches], np.uint32)
+    dxs, dys = O.circle_domain_brev(L)  # the domain points are an input of the ABI
+    d_spx, d_spy, d_fla, d_logs, d_dx, d_dy = (dev(x) for x in (spx, spy, fla, logs, dxs, dys))
+    d_np = [dev_ptrs([numer[j][k] for j in range(len(batches))]) for k in range(4)]
+    out = [vx.DeviceBuffer(1 << L) for _ in range(4)]
+    e = [vx.Event() for _ in range(3)]
+
+    def step(timed):
+        if timed:
+            e[0].record()
+        for (lg, idx, *_), (di, db, dc), nb in zip(batches, d_args, numer):
+            vx.lib.cuda_accumulate_numerators(d_colptrs.ptr, di.ptr, db.ptr, dc.ptr, len(idx), 1 << lg,
+                                              *[x.ptr for x in nb])
+        if timed:
+            e[1].record()
+        vx.lib.cuda_compute_quotients_combine(d_spx.ptr, d_spy.ptr, d_fla.ptr, *[p.ptr for p in d_np], d_logs.ptr,
+                                              len(batches), d_dx.ptr, d_dy.ptr, L, 1 << L, *[o.ptr for o in out])
+        if timed:
+            e[2].record()
+            return e[0].elapsed_ms(e[1]), e[1].elapsed_ms(e[2])
+        return None
+
+    for _ in range(3):
+        step(False)
+    vx.synchronize()
+    vx.check("deep")
+    K = 20
+    t_num = t_comb = 0.0
+    for _ in range(K):
+        a, b = step(True)
+        t_num += a
+        t_comb += b
+    t_num /= K
+    t_comb /= K
+    rows = 1 << L
+    num_bytes = sum(4 * len(idx) * (1 << int(lg)) + 16 * (1 << int(lg)) for lg, idx, *_ in batches)
+    comb_bytes = rows * (8 + 16) + sum(16 * (1 << int(lg)) for lg in logs)
+    bw = num_bytes / (t_num * 1e-3) / 1e9
+    ops = OPS_PER_ROW_ACC * rows * len(batches) / (t_comb * 1e-3) / 1e12
+    # parity: the combined quotient against the oracle on the same inputs
+    got = np.stack([o.to_host() for o in out])
+    t = time.perf_counter()
+    exp_num = [O.accumulate_numerators(cols, idx, b, c, 1 << lg) for lg, idx, b, c, _ in batches]
+    exp = O.quotients_combine(spx, spy, fla, exp_num, logs, dxs, dys, L)
+    cpu_s = time.perf_counter() - t
+    parity = bool(np.array_equal(got, exp))
+    ms = t_num + t_comb
+    log(f"deep: numerators {t_num:.3f} ms ({bw:.0f} GB/s), combine {t_comb:.3f} ms ({ops:.2f} Tops/s), "
+        f"{rows / (ms * 1e-3) / 1e6:.0f} Mrow/s, parity={parity}; oracle {cpu_s:.2f} s")
+    res = {"workload": "stwo DEEP quotient: 34 M31 LDE columns (4x2^16, 10x2^20, 20x2^22, splitmix64 seed 8), "
+                       "2 sample points, 6 accumulations (one per point and size), combined on the 2^22 "
+                       "lifting domain; reference ABI kernels, inputs resident in HBM",
+           "ms_per_quotient": round(ms, 4), "mrows_per_s": round(rows / (ms * 1e-3) / 1e6, 2),
+           "numerators_ms": round(t_num, 4), "combine_ms": round(t_comb, 4),
+           "roofline_numerators": {"bound": "hbm", "kernel": "accumulate_numerators_kernel (6 launches)",
+                                   "achieved": round(bw, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(bw / HBM_PEAK_GBS, 4), "bytes": num_bytes},
+           "roofline_combine": {"bound": "valu", "kernel": "quotients_combine_kernel",
+                                "achieved": round(ops, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                                "unit": "Tops/s (int32 lane-ops)", "frac": round(ops / VALU_PEAK_TOPS, 4),
+                                "ops_per_row_acc": OPS_PER_ROW_ACC, "hbm_bytes": comb_bytes},
+           "parity_vs_oracle": parity}
+    if not args.no_cpu:
+        res["cpu_baseline"] = {"value": round(rows / cpu_s / 1e6, 3), "unit": "Mrow/s",
+                               "cores": int(O.lib().orc_get_threads()), "kind": "port",
+                               "sample": "the whole workload (oracle accumulate_numerators + quotients_combine)"}
+    return res
+
+
 def bench_grind(vx, args) -> dict:
     """Blake2s PoW grinding at the reference's 26 bits: K channel states (splitmix64
     seed 7), each grind one vx_grind_blake2s call (prefix on the device, search,
@@ -534,7 +646,7 @@ def main() -> int:
     ap.add_argument("--no-pipelined", action="store_true", help="skip the two-stream plain measurement")
     ap.add_argument("--cpu-log-n", type=int, default=24)
     ap.add_argument("--cpu-reps", type=int, default=2)
-    ap.add_argument("--skip", default="", help="comma list of workloads to skip: plain,wide,batch,serve,grind "
+    ap.add_argument("--skip", default="", help="comma list of workloads to skip: plain,wide,batch,serve,deep,grind "
                                                "(skipping plain leaves value null: for profiling runs)")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="offered load, jobs/s")
@@ -567,7 +679,7 @@ def main() -> int:
     extra = {}
     if "plain" in skip:  # profiling runs of the other workloads only
         for name, fn in (("wide", bench_wide), ("batch", bench_batch), ("serve", bench_serve),
-                         ("grind", bench_grind)):
+                         ("deep", bench_deep), ("grind", bench_grind)):
             if name not in skip:
                 extra[name] = fn(vx, args)
         print(json.dumps({"metric": METRIC, "value": None, "unit": "Melem/s", "n_gpus": 1, **extra}), flush=True)
@@ -579,6 +691,8 @@ def main() -> int:
         extra["batch"] = bench_batch(vx, args)
     if "serve" not in skip:
         extra["serve"] = bench_serve(vx, args)
+    if "deep" not in skip:
+        extra["deep"] = bench_deep(vx, args)
     if "grind" not in skip:
         extra["grind"] = bench_grind(vx, args)
     log_n, nq, last = args.log_n, args.queries, args.log_last

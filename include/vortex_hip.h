@@ -214,6 +214,12 @@ int32_t vx_stream_create(void** stream);
 int32_t vx_stream_synchronize(void* stream);
 int32_t vx_stream_destroy(void* stream);
 void vx_set_stream(void* stream);                /* per-thread stream for Part 1 launches */
+/* HIP events for timing launches on a stream (null: the null stream); elapsed
+ * waits for end. Replace the reference's use of host timers around syncs. */
+int32_t vx_event_create(void** event);
+int32_t vx_event_record(void* event, void* stream);
+int32_t vx_event_elapsed_ms(void* start, void* end, float* ms);
+int32_t vx_event_destroy(void* event);
 void* vx_get_stream(void);
 
 /* Pass plans of the circle NTT (native and stwo basis). A transform of 2^log_n

@@ -274,6 +274,32 @@ class WideCommit:
             pass
 
 
+class Event:
+    """A HIP event (vx_event_*), for timing launches on a stream."""
+
+    def __init__(self):
+        self.ptr = C.c_void_p()
+        if lib.vx_event_create(C.byref(self.ptr)):
+            check("vx_event_create")
+
+    def record(self, stream: int = 0) -> None:
+        if lib.vx_event_record(self.ptr, C.c_void_p(stream) if stream else None):
+            check("vx_event_record")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float(0.0)
+        if lib.vx_event_elapsed_ms(self.ptr, end.ptr, C.byref(ms)):
+            check("vx_event_elapsed_ms")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ptr.value:
+                lib.vx_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
 def synchronize_stream(stream: int) -> None:
     if lib.vx_stream_synchronize(C.c_void_p(stream)):
         check("vx_stream_synchronize")

@@ -84,6 +84,10 @@ SIGNATURES = {
     "vx_stream_synchronize": (i32, [vp]),
     "vx_stream_destroy": (i32, [vp]),
     "vx_set_stream": (None, [vp]),
+    "vx_event_create": (i32, [C.POINTER(vp)]),
+    "vx_event_record": (i32, [vp, vp]),
+    "vx_event_elapsed_ms": (i32, [vp, vp, C.POINTER(C.c_float)]),
+    "vx_event_destroy": (i32, [vp]),
     "vx_ntt_autotune": (i32, [u32, u32, vp]),
     "vx_measure_valu_peak": (i32, [vp, vp]),
     "vx_ntt_plan_set": (i32, [u32, u32, vp, vp]),

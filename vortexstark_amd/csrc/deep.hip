@@ -7,9 +7,10 @@
 //  quotients_combine       out[row] = sum_j (numer_j[lift_j(row)] - a_j * y) / den_j(row)
 //  accumulate_lift         col[i] += curr[lift(i)]   (QM31, 4 words per element)
 //
-// accumulate_numerators is bound by HBM (4 bytes of column a term): the sum of
-// products c_i * f is kept in 64 bits (one v_mad_u64_u32 per coordinate and
-// column) and reduced once every three columns, and sum_i b_i is formed once
+// accumulate_numerators is bound by HBM (4 bytes of column a term): each thread
+// takes four consecutive rows (uint4 loads), the sum of products c_i * f is kept
+// in 64 bits (one v_mad_u64_u32 per coordinate and column, a partial reduction
+// every three columns, the full one at the end), and sum_i b_i is formed once
 // per thread. quotients_combine is bound by the VALU: den_j(row) is the CM31
 //   (Re(px) - x) * Im(py) - (Re(py) - y) * Im(px)
 //     = (Re(px) Im(py) - Re(py) Im(px)) - x Im(py) + y Im(px),
@@ -32,47 +33,89 @@ __device__ __forceinline__ uint32_t reduce64(uint64_t x) {
     return canon((uint32_t)(s & P) + (uint32_t)(s >> 31));
 }
 
+// ROWS consecutive rows per thread: 4 (uint4 loads and stores; n_rows % 4 == 0 and
+// 16-byte aligned outputs, checked on the host; a column whose pointer is not
+// 16-byte aligned is read word by word, a wave-uniform branch) or 1.
+template <uint32_t ROWS>
+__device__ __forceinline__ void load_rows(const uint32_t* col, uint64_t row0, uint32_t f[ROWS]) {
+    if (ROWS == 4 && ((uintptr_t)col & 15) == 0) {
+        const uint4 v = *(const uint4*)(col + row0);
+        f[0] = v.x;
+        f[1] = v.y;
+        f[2] = v.z;
+        f[3] = v.w;
+    } else {
+#pragma unroll
+        for (uint32_t r = 0; r < ROWS; r++) f[r] = col[row0 + r];
+    }
+}
+
+template <uint32_t ROWS>
 __global__ __launch_bounds__(DEEP_THREADS) void accumulate_numerators_kernel(
     const uint32_t* const* __restrict__ col_ptrs, const uint32_t* __restrict__ col_indices,
     const uint32_t* __restrict__ b_coeffs, const uint32_t* __restrict__ c_coeffs, uint32_t n_batch, uint32_t n_rows,
     uint32_t* __restrict__ out0, uint32_t* __restrict__ out1, uint32_t* __restrict__ out2,
     uint32_t* __restrict__ out3) {
-    const uint32_t row = blockIdx.x * DEEP_THREADS + threadIdx.x;
-    if (row >= n_rows) return;
+    const uint64_t row0 = ((uint64_t)blockIdx.x * DEEP_THREADS + threadIdx.x) * ROWS;
+    if (row0 >= n_rows) return;
     // sum_i b_i: the same for every row
     uint32_t bs[4] = {0, 0, 0, 0};
     for (uint32_t i = 0; i < n_batch; i++)
 #pragma unroll
         for (int k = 0; k < 4; k++) bs[k] = madd(bs[k], b_coeffs[4 * i + k]);
-    uint64_t acc[4] = {0, 0, 0, 0};
+    // The accumulators stay below 2^62 + 4 between groups: a group adds three
+    // products of at most (p - 1)^2 < 2^62 - 2^32 (three v_mad_u64_u32, the 64-bit
+    // addend chained), which stays below 2^64, and then folds bits 62..63 back
+    // onto bit 0 (2^62 = 1 mod p): a partial reduction of a few 32-bit ops.
+    constexpr uint64_t M62 = (1ull << 62) - 1;
+    uint64_t acc[ROWS][4] = {};
     uint32_t i = 0;
-    // three products of < 2^62 on top of a reduced value stay below 2^64
     for (; i + 3 <= n_batch; i += 3) {
-        const uint32_t f0 = col_ptrs[col_indices[i]][row];
-        const uint32_t f1 = col_ptrs[col_indices[i + 1]][row];
-        const uint32_t f2 = col_ptrs[col_indices[i + 2]][row];
+        uint32_t f[3][ROWS];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            uint64_t a = acc[k];
-            a += (uint64_t)c_coeffs[4 * i + k] * f0;
-            a += (uint64_t)c_coeffs[4 * (i + 1) + k] * f1;
-            a += (uint64_t)c_coeffs[4 * (i + 2) + k] * f2;
-            acc[k] = reduce64(a);
-        }
+        for (int g = 0; g < 3; g++) load_rows<ROWS>(col_ptrs[col_indices[i + g]], row0, f[g]);
+#pragma unroll
+        for (uint32_t r = 0; r < ROWS; r++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                uint64_t a = acc[r][k];
+                a = (uint64_t)c_coeffs[4 * i + k] * f[0][r] + a;
+                a = (uint64_t)c_coeffs[4 * (i + 1) + k] * f[1][r] + a;
+                a = (uint64_t)c_coeffs[4 * (i + 2) + k] * f[2][r] + a;
+                acc[r][k] = (a & M62) + (a >> 62);
+            }
     }
     for (; i < n_batch; i++) {
-        const uint32_t f = col_ptrs[col_indices[i]][row];
+        uint32_t f[ROWS];
+        load_rows<ROWS>(col_ptrs[col_indices[i]], row0, f);
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc[k] = reduce64(acc[k] + (uint64_t)c_coeffs[4 * i + k] * f);
+        for (uint32_t r = 0; r < ROWS; r++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t a = (uint64_t)c_coeffs[4 * i + k] * f[r] + acc[r][k];
+                acc[r][k] = (a & M62) + (a >> 62);
+            }
     }
-    out0[row] = msub((uint32_t)acc[0], bs[0]);
-    out1[row] = msub((uint32_t)acc[1], bs[1]);
-    out2[row] = msub((uint32_t)acc[2], bs[2]);
-    out3[row] = msub((uint32_t)acc[3], bs[3]);
+    uint32_t o[4][ROWS];
+#pragma unroll
+    for (uint32_t r = 0; r < ROWS; r++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k][r] = msub(reduce64(acc[r][k]), bs[k]);
+    uint32_t* outs[4] = {out0, out1, out2, out3};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (ROWS == 4)
+            *(uint4*)(outs[k] + row0) = make_uint4(o[k][0], o[k][1], o[k][2], o[k][3]);
+        else
+            outs[k][row0] = o[k][0];
+    }
 }
 
-// Thread t of workgroup g takes rows base + 64 * r + lane, r < COMBINE_ROWS, where
-// base = (g * 4 + wave) * 64 * COMBINE_ROWS: consecutive lanes, consecutive rows.
+// Thread t of workgroup g takes rows base + 64 * r + lane, r < R, where
+// base = (g * 4 + wave) * 64 * R: consecutive lanes, consecutive rows. Each
+// accumulation's numerators are loaded before its denominators are formed and
+// inverted, so the loads are in flight during that work.
+template <uint32_t R>
 __global__ __launch_bounds__(DEEP_THREADS) void quotients_combine_kernel(
     const uint32_t* __restrict__ spx, const uint32_t* __restrict__ spy, const uint32_t* __restrict__ fla,
     const uint32_t* const* __restrict__ n0, const uint32_t* const* __restrict__ n1,
@@ -80,7 +123,6 @@ __global__ __launch_bounds__(DEEP_THREADS) void quotients_combine_kernel(
     const uint32_t* __restrict__ acc_logs, uint32_t n_accs, const uint32_t* __restrict__ dxs,
     const uint32_t* __restrict__ dys, uint32_t lifting_log, uint32_t n_rows, uint32_t* __restrict__ out0,
     uint32_t* __restrict__ out1, uint32_t* __restrict__ out2, uint32_t* __restrict__ out3) {
-    constexpr uint32_t R = COMBINE_ROWS;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t base = ((uint64_t)blockIdx.x * (DEEP_THREADS / 64) + (threadIdx.x >> 6)) * (64u * R);
     if (base >= n_rows) return;
@@ -97,12 +139,22 @@ __global__ __launch_bounds__(DEEP_THREADS) void quotients_combine_kernel(
 #pragma unroll
     for (uint32_t r = 0; r < R; r++) q[r] = {{0, 0, 0, 0}};
     for (uint32_t j = 0; j < n_accs; j++) {
+        const uint32_t sh = lifting_log - acc_logs[j] + 1;
+        const uint32_t *p0 = n0[j], *p1 = n1[j], *p2 = n2[j], *p3 = n3[j];
+        uint32_t nv[R][4];
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            const uint64_t row = base + 64u * r + lane;
+            const uint64_t li = live[r] ? ((row >> sh) << 1) + (row & 1) : 0;
+            nv[r][0] = p0[li];
+            nv[r][1] = p1[li];
+            nv[r][2] = p2[li];
+            nv[r][3] = p3[li];
+        }
         const CM31 prx = {spx[4 * j], spx[4 * j + 1]}, pix = {spx[4 * j + 2], spx[4 * j + 3]};
         const CM31 pry = {spy[4 * j], spy[4 * j + 1]}, piy = {spy[4 * j + 2], spy[4 * j + 3]};
         const CM31 k0 = csub(cmul(prx, piy), cmul(pry, pix));
-        const QM31 a = {{fla[4 * j], fla[4 * j + 1], fla[4 * j + 2], fla[4 * j + 3]}};
-        const uint32_t sh = lifting_log - acc_logs[j] + 1;
-        const uint32_t *p0 = n0[j], *p1 = n1[j], *p2 = n2[j], *p3 = n3[j];
+        const QM31 a = {{fla[4 * j], fla[4 * j + 1], fla[4 * j + 2], fla[4 * j + 3]}};  // canonical
         CM31 den[R];
         uint32_t nrm[R], pre[R];  // norms |den|^2 and their prefix products
         uint32_t run = 1;
@@ -123,14 +175,17 @@ __global__ __launch_bounds__(DEEP_THREADS) void quotients_combine_kernel(
         for (int r = R - 1; r >= 0; r--) {
             const uint32_t ninv = nrm[r] ? mmul(inv, pre[r]) : 0u;
             inv = mmul(inv, nrm[r] ? nrm[r] : 1u);
-            const CM31 dinv = {mmul(den[r].a, ninv), mneg(mmul(den[r].b, ninv))};
-            if (!live[r]) continue;
-            const uint64_t row = base + 64u * r + lane;
-            const uint64_t li = ((row >> sh) << 1) + (row & 1);
-            const QM31 nv = {{p0[li], p1[li], p2[li], p3[li]}};
-            const QM31 full = qsub(nv, qmul_m31(a, dy[r]));
-            const CM31 fa = cmul({full.v[0], full.v[1]}, dinv), fb = cmul({full.v[2], full.v[3]}, dinv);
-            q[r] = qadd(q[r], {{fa.a, fa.b, fb.a, fb.b}});
+            // 1 / den = d0 - d1 i
+            const uint64_t d0 = mmul(den[r].a, ninv), d1 = mmul(den[r].b, ninv);
+            // full = numer - a y, then full * (d0 - d1 i), one reduction per coordinate:
+            // (f0 + f1 i)(d0 - d1 i) = (f0 d0 + f1 d1) + (f1 d0 - f0 d1) i, likewise for f2, f3
+            const uint64_t y = dy[r];
+            const uint64_t f0 = reduce64(nv[r][0] + (P - a.v[0]) * y), f1 = reduce64(nv[r][1] + (P - a.v[1]) * y);
+            const uint64_t f2 = reduce64(nv[r][2] + (P - a.v[2]) * y), f3 = reduce64(nv[r][3] + (P - a.v[3]) * y);
+            q[r].v[0] = madd(q[r].v[0], reduce64(f0 * d0 + f1 * d1));
+            q[r].v[1] = madd(q[r].v[1], reduce64(f1 * d0 + (P - f0) * d1));
+            q[r].v[2] = madd(q[r].v[2], reduce64(f2 * d0 + f3 * d1));
+            q[r].v[3] = madd(q[r].v[3], reduce64(f3 * d0 + (P - f2) * d1));
         }
     }
 #pragma unroll
@@ -173,8 +228,13 @@ void cuda_accumulate_numerators(const uint32_t* const* col_ptrs, const uint32_t*
                                 const uint32_t* b_coeffs, const uint32_t* c_coeffs, uint32_t n_batch_cols,
                                 uint32_t n_rows, uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3) {
     if (!n_rows) return;
-    accumulate_numerators_kernel<<<ceil_div(n_rows, DEEP_THREADS), DEEP_THREADS, 0, current_stream()>>>(
-        col_ptrs, col_indices, b_coeffs, c_coeffs, n_batch_cols, n_rows, out0, out1, out2, out3);
+    const bool vec = n_rows % 4 == 0 && (((uintptr_t)out0 | (uintptr_t)out1 | (uintptr_t)out2 | (uintptr_t)out3) & 15) == 0;
+    if (vec)
+        accumulate_numerators_kernel<4><<<ceil_div(n_rows, 4 * DEEP_THREADS), DEEP_THREADS, 0, current_stream()>>>(
+            col_ptrs, col_indices, b_coeffs, c_coeffs, n_batch_cols, n_rows, out0, out1, out2, out3);
+    else
+        accumulate_numerators_kernel<1><<<ceil_div(n_rows, DEEP_THREADS), DEEP_THREADS, 0, current_stream()>>>(
+            col_ptrs, col_indices, b_coeffs, c_coeffs, n_batch_cols, n_rows, out0, out1, out2, out3);
     check_launch("cuda_accumulate_numerators");
 }
 
@@ -187,7 +247,7 @@ void cuda_compute_quotients_combine(const uint32_t* sample_points_x, const uint3
                                     uint32_t* out2, uint32_t* out3) {
     if (!n_rows) return;
     const uint64_t rows_per_block = (uint64_t)DEEP_THREADS * COMBINE_ROWS;
-    quotients_combine_kernel<<<ceil_div(n_rows, rows_per_block), DEEP_THREADS, 0, current_stream()>>>(
+    quotients_combine_kernel<COMBINE_ROWS><<<ceil_div(n_rows, rows_per_block), DEEP_THREADS, 0, current_stream()>>>(
         sample_points_x, sample_points_y, first_linear_acc, numer_ptrs0, numer_ptrs1, numer_ptrs2, numer_ptrs3,
         acc_log_sizes, n_accs, domain_xs, domain_ys, lifting_log_size, n_rows, out0, out1, out2, out3);
     check_launch("cuda_compute_quotients_combine");

@@ -83,6 +83,20 @@ int32_t vx_stream_synchronize(void* s) {
     return check(hipStreamSynchronize((hipStream_t)s), "hipStreamSynchronize") ? 0 : -1;
 }
 int32_t vx_stream_destroy(void* s) { return check(hipStreamDestroy((hipStream_t)s), "hipStreamDestroy") ? 0 : -1; }
+int32_t vx_event_create(void** e) {
+    hipEvent_t ev = nullptr;
+    if (!e || !check(hipEventCreate(&ev), "hipEventCreate")) return -1;
+    *e = (void*)ev;
+    return 0;
+}
+int32_t vx_event_record(void* e, void* s) {
+    return check(hipEventRecord((hipEvent_t)e, (hipStream_t)s), "hipEventRecord") ? 0 : -1;
+}
+int32_t vx_event_elapsed_ms(void* start, void* end, float* ms) {
+    if (!ms || !check(hipEventSynchronize((hipEvent_t)end), "hipEventSynchronize")) return -1;
+    return check(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end), "hipEventElapsedTime") ? 0 : -1;
+}
+int32_t vx_event_destroy(void* e) { return check(hipEventDestroy((hipEvent_t)e), "hipEventDestroy") ? 0 : -1; }
 void vx_set_stream(void* s) { t_stream = (hipStream_t)s; }
 void* vx_get_stream(void) { return (void*)t_stream; }
 
