@@ -60,6 +60,12 @@ POW_BITS = 26  # the reference transcripts' PoW (prover.rs:96)
 # gfx950 ISA: denominator 31, numerator, product and sum 163, M31 inversion
 # chain and batch-inverse steps shared by the thread's 8 rows 28
 OPS_PER_ROW_ACC = 222
+# deep_fused_kernel<2,4>, from its gfx950 ISA: the column loop issues 265 VALU
+# instructions per 3 columns x 4 rows x 2 points (11 per row, column and point);
+# per (row, point): denominator 31, numerator and product 90, final accumulator
+# reduction 40, share of the M31 inversion chain 22
+OPS_PER_ROW_COL_POINT = 11
+OPS_PER_ROW_POINT = 183
 
 
 def load_traffic() -> dict:
@@ -495,20 +501,36 @@ def bench_deep(vx, args) -> dict:
         return dev(addrs.view(np.uint32))
 
     pts = [(q(4), q(4)), (q(4), q(4))]
+    lc = q(2 * len(sizes) * 12).reshape(2, len(sizes), 12)  # a, b, c of column c at point p
     batches = []  # (log size, column indices, b, c, point)
     for pt in range(2):
         for lg in (16, 20, 22):
             idx = np.array([i for i, l in enumerate(sizes) if l == lg], np.uint32)
-            batches.append((lg, idx, q(4 * len(idx)), q(4 * len(idx)), pt))
+            batches.append((lg, idx, lc[pt, idx, 4:8].ravel().copy(), lc[pt, idx, 8:12].ravel().copy(), pt))
     d_colptrs = dev_ptrs(dcols)
     numer = [[vx.DeviceBuffer(1 << lg) for _ in range(4)] for lg, *_ in batches]
     d_args = [(dev(idx), dev(b), dev(c)) for _, idx, b, c, _ in batches]
     spx = np.concatenate([pts[pt][0] for *_, pt in batches])
     spy = np.concatenate([pts[pt][1] for *_, pt in batches])
-    fla = q(4 * len(batches))
+    fla = np.concatenate([(lc[pt, idx, 0:4].astype(np.uint64).sum(axis=0) % vx_p()).astype(np.uint32)
+                          for _, idx, _, _, pt in batches])
     logs = np.array([lg for lg, *_ in batches], np.uint32)
     dxs, dys = O.circle_domain_brev(L)  # the domain points are an input of the ABI
     d_spx, d_spy, d_fla, d_logs, d_dx, d_dy = (dev(x) for x in (spx, spy, fla, logs, dxs, dys))
+    # the fused form on the same inputs (its domain points built on the device)
+    fx, fy = vx.circle_domain_points(L)
+    fused_out = [vx.DeviceBuffer(1 << L) for _ in range(4)]
+    dq = vx.DeepQuotient(len(sizes), 2)
+    points = np.stack([np.concatenate(p) for p in pts])
+
+    def fused(timed):
+        if timed:
+            e[0].record()
+        dq.quotient([c.addr for c in dcols], sizes, points, lc, L, fx.addr, fy.addr, [o.addr for o in fused_out])
+        if timed:
+            e[1].record()
+            return e[0].elapsed_ms(e[1])
+        return None
     d_np = [dev_ptrs([numer[j][k] for j in range(len(batches))]) for k in range(4)]
     out = [vx.DeviceBuffer(1 << L) for _ in range(4)]
     e = [vx.Event() for _ in range(3)]
@@ -540,6 +562,9 @@ def bench_deep(vx, args) -> dict:
         t_comb += b
     t_num /= K
     t_comb /= K
+    for _ in range(3):
+        fused(False)
+    t_fused = sum(fused(True) for _ in range(K)) / K
     rows = 1 << L
     num_bytes = sum(4 * len(idx) * (1 << int(lg)) + 16 * (1 << int(lg)) for lg, idx, *_ in batches)
     comb_bytes = rows * (8 + 16) + sum(16 * (1 << int(lg)) for lg in logs)
@@ -552,9 +577,15 @@ def bench_deep(vx, args) -> dict:
     exp = O.quotients_combine(spx, spy, fla, exp_num, logs, dxs, dys, L)
     cpu_s = time.perf_counter() - t
     parity = bool(np.array_equal(got, exp))
+    fused_same = bool(np.array_equal(np.stack([o.to_host() for o in fused_out]), got))
+    dq.close()
     ms = t_num + t_comb
+    # fused: each column read once, one denominator per (row, point)
+    fused_bytes = sum(4 * (1 << l) for l in sizes) + rows * (8 + 16)
+    fused_ops = (OPS_PER_ROW_COL_POINT * 2 * len(sizes) + OPS_PER_ROW_POINT * 2) * rows
     log(f"deep: numerators {t_num:.3f} ms ({bw:.0f} GB/s), combine {t_comb:.3f} ms ({ops:.2f} Tops/s), "
-        f"{rows / (ms * 1e-3) / 1e6:.0f} Mrow/s, parity={parity}; oracle {cpu_s:.2f} s")
+        f"{rows / (ms * 1e-3) / 1e6:.0f} Mrow/s, parity={parity}; fused {t_fused:.3f} ms "
+        f"({fused_ops / (t_fused * 1e-3) / 1e12:.2f} Tops/s), same={fused_same}; oracle {cpu_s:.2f} s")
     res = {"workload": "stwo DEEP quotient: 34 M31 LDE columns (4x2^16, 10x2^20, 20x2^22, splitmix64 seed 8), "
                        "2 sample points, 6 accumulations (one per point and size), combined on the 2^22 "
                        "lifting domain; reference ABI kernels, inputs resident in HBM",
@@ -567,7 +598,20 @@ def bench_deep(vx, args) -> dict:
                                 "achieved": round(ops, 3), "peak": round(VALU_PEAK_TOPS, 2),
                                 "unit": "Tops/s (int32 lane-ops)", "frac": round(ops / VALU_PEAK_TOPS, 4),
                                 "ops_per_row_acc": OPS_PER_ROW_ACC, "hbm_bytes": comb_bytes},
-           "parity_vs_oracle": parity}
+           "parity_vs_oracle": parity,
+           "fused": {"kernel": "deep_fused_kernel<2,4> (vx_deep_quotient: one launch, columns read once, "
+                               "one denominator per row and point)",
+                     "ms_per_quotient": round(t_fused, 4), "mrows_per_s": round(rows / (t_fused * 1e-3) / 1e6, 2),
+                     "speedup_vs_reference_pipeline": round(ms / t_fused, 3), "same_words_as_abi": fused_same,
+                     "roofline": {"bound": "valu", "achieved": round(fused_ops / (t_fused * 1e-3) / 1e12, 3),
+                                  "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s (int32 lane-ops)",
+                                  "frac": round(fused_ops / (t_fused * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
+                                  "ops_per_row_col_point": OPS_PER_ROW_COL_POINT,
+                                  "ops_per_row_point": OPS_PER_ROW_POINT},
+                     "roofline_hbm": {"bound": "hbm", "achieved": round(fused_bytes / (t_fused * 1e-3) / 1e9, 1),
+                                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": round(fused_bytes / (t_fused * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                      "bytes": fused_bytes}}}
     if not args.no_cpu:
         res["cpu_baseline"] = {"value": round(rows / cpu_s / 1e6, 3), "unit": "Mrow/s",
                                "cores": int(O.lib().orc_get_threads()), "kind": "port",

@@ -180,6 +180,29 @@ void cuda_compute_quotients_combine(const uint32_t* sample_points_x, const uint3
                                     uint32_t* out1, uint32_t* out2, uint32_t* out3);               /* ffi.rs:878 */
 void cuda_accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, uint32_t log_ratio); /* ffi.rs:958 */
 
+/* Fused DEEP quotient (no reference counterpart: QuotientOps::accumulate_numerators
+ * for every (point, size) batch followed by compute_quotients_and_combine,
+ * quotient_ops.rs:35-223, in one launch; the same words as that pipeline). cols:
+ * host array of n_cols device pointers to bit-reversed M31 evaluations of
+ * 2^col_log_sizes[c] rows (1 <= log <= lifting_log_size); points: host,
+ * n_points (1..4) x 8 words (x then y, QM31); line_coeffs: host,
+ * n_points x n_cols x 12 words (a, b, c of column c at point p; all zero for a
+ * column not sampled there); domain_xs/ys: device, the lifting domain's points
+ * (vx_circle_domain_points). out*: device SoA QM31, 2^lifting_log_size rows.
+ *   out[row] = sum_p (sum_c (c_pc col_c[lift_c(row)] - b_pc) - (sum_c a_pc) y) / den_p(row)
+ * Asynchronous on stream; a vx_deep holds its metadata staging, so calls on one
+ * object are ordered (a call waits for the previous call's upload). */
+typedef struct vx_deep vx_deep;
+vx_deep* vx_deep_create(uint32_t max_cols, uint32_t max_points);
+int32_t vx_deep_quotient(vx_deep* d, const uint32_t* const* cols, const uint32_t* col_log_sizes, uint32_t n_cols,
+                         uint32_t n_points, const uint32_t* points, const uint32_t* line_coeffs,
+                         uint32_t lifting_log_size, const uint32_t* domain_xs, const uint32_t* domain_ys,
+                         uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3, void* stream);
+void vx_deep_destroy(vx_deep* d);
+/* Points of CanonicCoset(log_size).circle_domain() at bit_reverse_index(row) for
+ * every row, computed on the device (what quotient_ops.rs:140-147 builds on the host). */
+int32_t vx_circle_domain_points(uint32_t log_size, uint32_t* xs, uint32_t* ys, void* stream);
+
 /* Blake2s proof-of-work nonce search (ffi.rs:627-639, 965-976; cuda/grind.cu,
  * cuda/grind_m31_output.cu). Tries the nonces [batch_offset, batch_offset +
  * n_threads) against prefixed_digest (device, 8 words: Blake2s(0x12345678 LE ||

@@ -147,3 +147,68 @@ def test_wide_shape_end_to_end(vx):
     exp = O.quotients_combine(spx, spy, fla, exp_numers, logs, dxs, dys, L)
     got = gpu_combine(vx, spx, spy, fla, numers, logs, dxs, dys, L, 1 << L)
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("log_n", [1, 2, 5, 16, 22])
+def test_circle_domain_points(vx, log_n):
+    xs, ys = vx.circle_domain_points(log_n)
+    vx.synchronize()
+    exs, eys = O.circle_domain_brev(log_n)
+    assert np.array_equal(xs.to_host(), exs) and np.array_equal(ys.to_host(), eys)
+
+
+def oracle_pipeline(cols, logs, points, lc, L, dxs, dys):
+    """The reference pipeline on the oracle: one accumulation per (point, size)."""
+    numers, spx, spy, fla, acc_logs = [], [], [], [], []
+    for p in range(points.shape[0]):
+        for s in sorted(set(logs)):
+            idx = np.array([c for c in range(len(cols)) if logs[c] == s], np.uint32)
+            sub = [cols[c] for c in idx]
+            b = lc[p, idx, 4:8].ravel()
+            c = lc[p, idx, 8:12].ravel()
+            numers.append(O.accumulate_numerators(sub, np.arange(len(idx), dtype=np.uint32), b, c, 1 << s))
+            a = lc[p, idx, 0:4].astype(np.uint64).sum(axis=0) % P
+            spx += list(points[p, :4])
+            spy += list(points[p, 4:])
+            fla += list(a.astype(np.uint32))
+            acc_logs.append(s)
+    return O.quotients_combine(spx, spy, fla, numers, acc_logs, dxs, dys, L)
+
+
+@pytest.mark.parametrize("L,logs,n_points", [(1, [1], 1), (4, [2, 4, 3, 1], 2), (8, [8] * 5 + [5, 6], 3),
+                                             (10, [3, 10, 7, 10, 9], 4), (14, [14, 12, 12, 9], 2),
+                                             (20, [16] * 2 + [18] * 3 + [20] * 4, 2)])
+def test_fused_quotient_matches_reference_pipeline(vx, L, logs, n_points):
+    rng = np.random.default_rng(L * 13 + n_points)
+    cols = [rnd(rng, 1 << l) for l in logs]
+    points = rnd(rng, n_points, 8)
+    lc = rnd(rng, n_points, len(cols), 12)
+    if n_points > 1:
+        lc[1, 0] = 0  # column 0 not sampled at point 1
+    dxs, dys = vx.circle_domain_points(L)
+    exs, eys = O.circle_domain_brev(L)
+    exp = oracle_pipeline(cols, logs, points, lc, L, exs, eys)
+    dcols = [vx.DeviceBuffer.from_host(c) for c in cols]
+    outs = [vx.DeviceBuffer(1 << L) for _ in range(4)]
+    dq = vx.DeepQuotient(len(cols), n_points)
+    dq.quotient([c.addr for c in dcols], logs, points, lc, L, dxs.addr, dys.addr, [o.addr for o in outs])
+    vx.synchronize()
+    vx.check("vx_deep_quotient")
+    assert np.array_equal(np.stack([o.to_host() for o in outs]), exp)
+    # a second call on the same object, other coefficients: the staging is reused safely
+    lc2 = rnd(rng, n_points, len(cols), 12)
+    dq.quotient([c.addr for c in dcols], logs, points, lc2, L, dxs.addr, dys.addr, [o.addr for o in outs])
+    vx.synchronize()
+    assert np.array_equal(np.stack([o.to_host() for o in outs]), oracle_pipeline(cols, logs, points, lc2, L, exs, eys))
+    dq.close()
+
+
+def test_fused_quotient_rejects_bad_shapes(vx):
+    dq = vx.DeepQuotient(2, 2)
+    buf = vx.DeviceBuffer(16)
+    pts, lc = np.zeros((3, 8), np.uint32), np.zeros((3, 2, 12), np.uint32)
+    with pytest.raises(vx.HipError):
+        dq.quotient([buf.addr, buf.addr], [2, 2], pts, lc, 4, buf.addr, buf.addr, [buf.addr] * 4)  # 3 points > 2
+    with pytest.raises(vx.HipError):
+        dq.quotient([buf.addr], [5], pts[:1], lc[:1, :1], 4, buf.addr, buf.addr, [buf.addr] * 4)  # log 5 > 4
+    dq.close()

@@ -23,7 +23,8 @@ def test_header_declares_reference_symbols():
                  "cuda_stwo_ntt_evaluate", "cuda_stwo_ntt_interpolate", "cuda_build_leaves_lifted",
                  "cuda_pack_leaves", "cuda_bit_reverse_qm31", "vx_wide_create", "vx_wide_commit_device",
                  "cuda_grind_pow", "cuda_grind_pow_m31_output", "vx_grind_blake2s",
-                 "cuda_accumulate_numerators", "cuda_compute_quotients_combine", "cuda_accumulate_lift"]:
+                 "cuda_accumulate_numerators", "cuda_compute_quotients_combine", "cuda_accumulate_lift",
+                 "vx_deep_create", "vx_deep_quotient", "vx_circle_domain_points"]:
         assert must in names
 
 

@@ -443,6 +443,52 @@ def merkle_stage_plan_set(log_items: int, logc: int) -> None:
         raise HipError("vx_merkle_stage_plan_set failed")
 
 
+class DeepQuotient:
+    """Fused stwo DEEP quotient (vx_deep_*): QuotientOps::accumulate_numerators and
+    compute_quotients_and_combine (quotient_ops.rs:35-223) in one launch."""
+
+    def __init__(self, max_cols: int, max_points: int = 2):
+        self.handle = lib.vx_deep_create(max_cols, max_points)
+        if not self.handle:
+            check("vx_deep_create")
+            raise HipError("vx_deep_create failed")
+
+    def quotient(self, col_addrs, col_logs, points, line_coeffs, lifting_log: int, domain_xs: int, domain_ys: int,
+                 outs, stream: int = 0) -> None:
+        """col_addrs: device addresses; points: (n_points, 8) words; line_coeffs:
+        (n_points, n_cols, 12) words (a, b, c); outs: four device addresses."""
+        n = len(col_addrs)
+        ptrs = (C.c_void_p * n)(*col_addrs)
+        logs = np.ascontiguousarray(col_logs, dtype=np.uint32)
+        pts = np.ascontiguousarray(points, dtype=np.uint32)
+        lc = np.ascontiguousarray(line_coeffs, dtype=np.uint32)
+        if lib.vx_deep_quotient(self.handle, ptrs, logs.ctypes.data, n, pts.size // 8, pts.ctypes.data,
+                                lc.ctypes.data, lifting_log, C.c_void_p(domain_xs), C.c_void_p(domain_ys),
+                                *[C.c_void_p(o) for o in outs], C.c_void_p(stream) if stream else None):
+            check("vx_deep_quotient")
+            raise HipError("vx_deep_quotient failed")
+
+    def close(self) -> None:
+        if self.handle:
+            lib.vx_deep_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def circle_domain_points(log_size: int, stream: int = 0):
+    """(xs, ys) device buffers: the canonic circle domain at bit-reversed rows."""
+    xs, ys = DeviceBuffer(1 << log_size), DeviceBuffer(1 << log_size)
+    if lib.vx_circle_domain_points(log_size, xs.ptr, ys.ptr, C.c_void_p(stream) if stream else None):
+        check("vx_circle_domain_points")
+        raise HipError("vx_circle_domain_points failed")
+    return xs, ys
+
+
 NONCE_NONE = (1 << 64) - 1
 
 

@@ -57,6 +57,10 @@ SIGNATURES = {
     "cuda_compute_quotients_combine": (None, [vp, vp, vp, vp, vp, vp, vp, vp, u32, vp, vp, u32, u32,
                                               vp, vp, vp, vp]),
     "cuda_accumulate_lift": (None, [vp, vp, u32, u32]),
+    "vx_deep_create": (vp, [u32, u32]),
+    "vx_deep_quotient": (i32, [vp, vp, vp, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp, vp]),
+    "vx_deep_destroy": (None, [vp]),
+    "vx_circle_domain_points": (i32, [u32, vp, vp, vp]),
     "cuda_grind_pow": (None, [vp, vp, u32, u64, u32]),
     "cuda_grind_pow_m31_output": (None, [vp, vp, u32, u64, u32]),
     "vx_grind_blake2s": (i32, [vp, u32, i32, u64, u64, vp, i32, vp]),

@@ -20,6 +20,7 @@
 // accumulation (Montgomery's trick) instead of one per row.
 #include "common.hpp"
 #include "field.hpp"
+#include "kernels.hpp"
 
 namespace vx {
 namespace {
@@ -217,10 +218,160 @@ __global__ __launch_bounds__(DEEP_THREADS) void accumulate_lift_kernel(uint32_t*
     }
 }
 
+// ---------------------------------------------------------------- fused form
+// One launch for the whole quotient (no reference counterpart): for every row
+// of the lifting domain,
+//   out[row] = sum_p (sum_c (c_pc * col_c[lift_c(row)] - b_pc) - (sum_c a_pc) * y) / den_p(row),
+// which is what accumulate_numerators per (point, size) batch followed by
+// quotients_combine compute, summed in another order (exact field arithmetic,
+// so the same words). Each column value is read once for all sample points and
+// each (row, point) forms and inverts its denominator once, where the
+// reference pipeline reads the columns once per point, writes and re-reads the
+// numerators, and inverts once per (row, accumulation).
+//
+// Metadata (device words): per point x[4] y[4] A[4] B[4] (A, B the sums of the
+// point's a and b coefficients); per column ptr lo, ptr hi, lift shift, 0; per
+// (column, point) c[4].
+constexpr uint32_t DEEP_MAX_POINTS = 4;
+
+template <uint32_t NP, uint32_t R>
+__device__ __forceinline__ void fused_columns(const uint32_t* __restrict__ cm, const uint32_t* __restrict__ cf,
+                                              uint32_t c, uint32_t n, const uint64_t row[R], const bool live[R],
+                                              uint64_t acc[NP][R][4]) {
+    uint32_t f[3][R];
+    for (uint32_t g = 0; g < n; g++) {
+        const uint32_t* ptr = (const uint32_t*)(uintptr_t)(cm[4 * (c + g)] | ((uint64_t)cm[4 * (c + g) + 1] << 32));
+        const uint32_t sh = cm[4 * (c + g) + 2];
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) f[g][r] = ptr[live[r] ? ((row[r] >> sh) << 1) + (row[r] & 1) : 0];
+    }
+    constexpr uint64_t M62 = (1ull << 62) - 1;
+#pragma unroll
+    for (uint32_t p = 0; p < NP; p++)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (uint32_t r = 0; r < R; r++) {
+                uint64_t a = acc[p][r][k];
+                for (uint32_t g = 0; g < n; g++) a = (uint64_t)cf[((c + g) * NP + p) * 4 + k] * f[g][r] + a;
+                acc[p][r][k] = (a & M62) + (a >> 62);
+            }
+}
+
+template <uint32_t NP, uint32_t R>
+__global__ __launch_bounds__(DEEP_THREADS) void deep_fused_kernel(const uint32_t* __restrict__ meta, uint32_t n_cols,
+                                                                  const uint32_t* __restrict__ dxs,
+                                                                  const uint32_t* __restrict__ dys, uint32_t n_rows,
+                                                                  uint32_t* __restrict__ out0,
+                                                                  uint32_t* __restrict__ out1,
+                                                                  uint32_t* __restrict__ out2,
+                                                                  uint32_t* __restrict__ out3) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t base = ((uint64_t)blockIdx.x * (DEEP_THREADS / 64) + (threadIdx.x >> 6)) * (64u * R);
+    if (base >= n_rows) return;
+    const uint32_t* pts = meta;
+    const uint32_t* cm = meta + 16 * NP;
+    const uint32_t* cf = cm + 4 * n_cols;
+    uint64_t row[R];
+    bool live[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++) {
+        row[r] = base + 64u * r + lane;
+        live[r] = row[r] < n_rows;
+    }
+    uint64_t acc[NP][R][4] = {};
+    // three products on top of a partially reduced value (< 2^62 + 4) stay below 2^64
+    uint32_t c = 0;
+    for (; c + 3 <= n_cols; c += 3) fused_columns<NP, R>(cm, cf, c, 3, row, live, acc);
+    if (c < n_cols) fused_columns<NP, R>(cm, cf, c, n_cols - c, row, live, acc);
+    uint32_t dx[R], dy[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++) {
+        dx[r] = live[r] ? dxs[row[r]] : 1u;
+        dy[r] = live[r] ? dys[row[r]] : 0u;
+    }
+    QM31 q[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++) q[r] = {{0, 0, 0, 0}};
+#pragma unroll
+    for (uint32_t p = 0; p < NP; p++) {
+        const uint32_t* pp = pts + 16 * p;
+        const CM31 prx = {pp[0], pp[1]}, pix = {pp[2], pp[3]};
+        const CM31 pry = {pp[4], pp[5]}, piy = {pp[6], pp[7]};
+        const CM31 k0 = csub(cmul(prx, piy), cmul(pry, pix));
+        CM31 den[R];
+        uint32_t nrm[R], pre[R];
+        uint32_t run = 1;
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            const uint64_t da = (uint64_t)dy[r] * pix.a + (uint64_t)(P - dx[r]) * piy.a + k0.a;
+            const uint64_t db = (uint64_t)dy[r] * pix.b + (uint64_t)(P - dx[r]) * piy.b + k0.b;
+            den[r] = live[r] ? CM31{reduce64(da), reduce64(db)} : CM31{1u, 0u};
+            nrm[r] = reduce64((uint64_t)den[r].a * den[r].a + (uint64_t)den[r].b * den[r].b);
+            pre[r] = run;
+            run = mmul(run, nrm[r] ? nrm[r] : 1u);  // a zero denominator inverts to 0 (cm31_inv(0))
+        }
+        uint32_t inv = minv(run);
+#pragma unroll
+        for (int r = R - 1; r >= 0; r--) {
+            const uint32_t ninv = nrm[r] ? mmul(inv, pre[r]) : 0u;
+            inv = mmul(inv, nrm[r] ? nrm[r] : 1u);
+            const uint64_t d0 = mmul(den[r].a, ninv), d1 = mmul(den[r].b, ninv);
+            const uint64_t y = dy[r];
+            uint64_t f[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                // numerator - B - A y, A and B canonical
+                const uint32_t num = msub(reduce64(acc[p][r][k]), pp[12 + k]);
+                f[k] = reduce64(num + (P - pp[8 + k]) * y);
+            }
+            q[r].v[0] = madd(q[r].v[0], reduce64(f[0] * d0 + f[1] * d1));
+            q[r].v[1] = madd(q[r].v[1], reduce64(f[1] * d0 + (P - f[0]) * d1));
+            q[r].v[2] = madd(q[r].v[2], reduce64(f[2] * d0 + f[3] * d1));
+            q[r].v[3] = madd(q[r].v[3], reduce64(f[3] * d0 + (P - f[2]) * d1));
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++) {
+        if (!live[r]) continue;
+        out0[row[r]] = q[r].v[0];
+        out1[row[r]] = q[r].v[1];
+        out2[row[r]] = q[r].v[2];
+        out3[row[r]] = q[r].v[3];
+    }
+}
+
+// Points of the canonic circle domain of size 2^log_n at bit-reversed rows:
+// natural index k < n/2 is half_odds(log_n - 1).at(k), the rest their conjugates.
+__global__ __launch_bounds__(DEEP_THREADS) void domain_points_kernel(CPt initial, CPt step, uint32_t log_n,
+                                                                     uint32_t* __restrict__ xs,
+                                                                     uint32_t* __restrict__ ys) {
+    const uint64_t row = (uint64_t)blockIdx.x * DEEP_THREADS + threadIdx.x;
+    if (row >= (1ull << log_n)) return;
+    const uint32_t k = brev_bits((uint32_t)row, log_n), half = (1u << log_n) >> 1;
+    const CPt pt = cp_pow_mul(initial, step, k < half ? k : k - half);
+    xs[row] = pt.x;
+    ys[row] = k < half ? pt.y : mneg(pt.y);
+}
+
+inline uint32_t hadd31(uint32_t a, uint32_t b) {
+    const uint32_t r = a + b;
+    return r >= P ? r - P : r;
+}
+
 }  // namespace
 }  // namespace vx
 
 using namespace vx;
+
+struct vx_deep {
+    uint32_t max_cols = 0, max_points = 0;
+    size_t bytes = 0;
+    uint32_t* h_meta = nullptr;  // pinned staging
+    uint32_t* d_meta = nullptr;
+    hipEvent_t copied = nullptr, done = nullptr;
+    bool pending = false;
+};
 
 extern "C" {
 
@@ -262,6 +413,120 @@ void cuda_accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, u
         accumulate_lift_kernel<false>
             <<<ceil_div(col_n, DEEP_THREADS), DEEP_THREADS, 0, current_stream()>>>(col, curr, col_n, log_ratio);
     check_launch("cuda_accumulate_lift");
+}
+
+vx_deep* vx_deep_create(uint32_t max_cols, uint32_t max_points) {
+    if (!max_cols || max_cols > (1u << 20) || !max_points || max_points > DEEP_MAX_POINTS) {
+        set_error(-2, "vx_deep_create: need 1..2^20 columns and 1..4 sample points");
+        return nullptr;
+    }
+    vx_deep* d = new vx_deep();
+    d->max_cols = max_cols;
+    d->max_points = max_points;
+    d->bytes = 4ull * (16ull * max_points + 4ull * max_cols + 4ull * max_cols * max_points);
+    bool ok = check(hipHostMalloc((void**)&d->h_meta, d->bytes), "hipHostMalloc") &&
+              check(hipMalloc((void**)&d->d_meta, d->bytes), "hipMalloc") &&
+              check(hipEventCreateWithFlags(&d->copied, hipEventDisableTiming), "hipEventCreate") &&
+              check(hipEventCreateWithFlags(&d->done, hipEventDisableTiming), "hipEventCreate");
+    if (!ok) {
+        vx_deep_destroy(d);
+        return nullptr;
+    }
+    return d;
+}
+
+void vx_deep_destroy(vx_deep* d) {
+    if (!d) return;
+    if (d->pending) (void)hipEventSynchronize(d->done);
+    if (d->h_meta) (void)hipHostFree(d->h_meta);
+    if (d->d_meta) (void)hipFree(d->d_meta);
+    if (d->copied) (void)hipEventDestroy(d->copied);
+    if (d->done) (void)hipEventDestroy(d->done);
+    delete d;
+}
+
+int32_t vx_deep_quotient(vx_deep* d, const uint32_t* const* cols, const uint32_t* col_log_sizes, uint32_t n_cols,
+                         uint32_t n_points, const uint32_t* points, const uint32_t* line_coeffs,
+                         uint32_t lifting_log_size, const uint32_t* domain_xs, const uint32_t* domain_ys,
+                         uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3, void* stream) {
+    if (!d || !cols || !col_log_sizes || !points || !line_coeffs || !domain_xs || !domain_ys || !out0 || !out1 ||
+        !out2 || !out3) {
+        set_error(hipErrorInvalidValue, "vx_deep_quotient: null argument");
+        return -1;
+    }
+    if (!n_cols || n_cols > d->max_cols || !n_points || n_points > d->max_points || lifting_log_size < 1 ||
+        lifting_log_size > 30) {
+        set_error(-2, "vx_deep_quotient: column or point count beyond the object's, or lifting log outside 1..30");
+        return -1;
+    }
+    for (uint32_t c = 0; c < n_cols; c++)
+        if (col_log_sizes[c] < 1 || col_log_sizes[c] > lifting_log_size || !cols[c]) {
+            set_error(-2, "vx_deep_quotient: column log sizes must be in 1..lifting_log_size");
+            return -1;
+        }
+    hipStream_t s = (hipStream_t)stream;
+    // the staging may still feed the previous call's copy, and the device copy its kernel
+    if (d->pending && !(check(hipEventSynchronize(d->copied), "hipEventSynchronize") &&
+                        check(hipStreamWaitEvent(s, d->done, 0), "hipStreamWaitEvent")))
+        return -1;
+    uint32_t* h = d->h_meta;
+    const uint32_t NP = n_points;
+    for (uint32_t p = 0; p < NP; p++) {
+        uint32_t* pp = h + 16 * p;
+        for (int k = 0; k < 8; k++) pp[k] = points[8 * p + k] % P;
+        for (int k = 0; k < 8; k++) pp[8 + k] = 0;
+        for (uint32_t c = 0; c < n_cols; c++) {
+            const uint32_t* lc = line_coeffs + 12ull * ((uint64_t)p * n_cols + c);
+            for (int k = 0; k < 4; k++) {
+                pp[8 + k] = hadd31(pp[8 + k], lc[k] % P);       // A += a
+                pp[12 + k] = hadd31(pp[12 + k], lc[4 + k] % P);  // B += b
+            }
+        }
+    }
+    uint32_t* cm = h + 16 * NP;
+    uint32_t* cf = cm + 4 * n_cols;
+    for (uint32_t c = 0; c < n_cols; c++) {
+        const uint64_t ptr = (uint64_t)(uintptr_t)cols[c];
+        cm[4 * c] = (uint32_t)ptr;
+        cm[4 * c + 1] = (uint32_t)(ptr >> 32);
+        cm[4 * c + 2] = lifting_log_size - col_log_sizes[c] + 1;
+        cm[4 * c + 3] = 0;
+        for (uint32_t p = 0; p < NP; p++)
+            for (int k = 0; k < 4; k++)
+                cf[(c * NP + p) * 4 + k] = line_coeffs[12ull * ((uint64_t)p * n_cols + c) + 8 + k] % P;
+    }
+    const size_t bytes = 4ull * (16ull * NP + 4ull * n_cols + 4ull * n_cols * NP);
+    if (!check(hipMemcpyAsync(d->d_meta, h, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync") ||
+        !check(hipEventRecord(d->copied, s), "hipEventRecord"))
+        return -1;
+    const uint32_t n_rows = 1u << lifting_log_size;
+    switch (NP) {
+#define VX_DEEP_LAUNCH(np, r)                                                                                  \
+    case np:                                                                                                   \
+        deep_fused_kernel<np, r><<<ceil_div(n_rows, (uint64_t)DEEP_THREADS * r), DEEP_THREADS, 0, s>>>(        \
+            d->d_meta, n_cols, domain_xs, domain_ys, n_rows, out0, out1, out2, out3);                          \
+        break;
+        VX_DEEP_LAUNCH(1, 4)
+        VX_DEEP_LAUNCH(2, 4)
+        VX_DEEP_LAUNCH(3, 2)
+        VX_DEEP_LAUNCH(4, 2)
+#undef VX_DEEP_LAUNCH
+    }
+    if (!check_launch("deep_fused_kernel") || !check(hipEventRecord(d->done, s), "hipEventRecord")) return -1;
+    d->pending = true;
+    return 0;
+}
+
+int32_t vx_circle_domain_points(uint32_t log_size, uint32_t* xs, uint32_t* ys, void* stream) {
+    if (!xs || !ys || log_size < 1 || log_size > 30) {
+        set_error(-2, "vx_circle_domain_points: log size outside 1..30 or null output");
+        return -1;
+    }
+    HostPt hi, hs;
+    host_coset(2, log_size - 1, hi, hs);  // half_odds(log_size - 1)
+    domain_points_kernel<<<ceil_div(1ull << log_size, DEEP_THREADS), DEEP_THREADS, 0, (hipStream_t)stream>>>(
+        CPt{hi.x, hi.y}, CPt{hs.x, hs.y}, log_size, xs, ys);
+    return check_launch("domain_points_kernel") ? 0 : -1;
 }
 
 }  // extern "C"
