@@ -56,16 +56,12 @@ OPS_PER_BUTTERFLY = 13
 # word 0 is formed
 OPS_PER_NONCE = 847
 POW_BITS = 26  # the reference transcripts' PoW (prover.rs:96)
-# VALU instructions per (row, accumulation) of quotients_combine_kernel in its
-# gfx950 ISA: denominator 31, numerator, product and sum 163, M31 inversion
-# chain and batch-inverse steps shared by the thread's 8 rows 28
-OPS_PER_ROW_ACC = 222
-# deep_fused_kernel<2,4>, from its gfx950 ISA: the column loop issues 265 VALU
-# instructions per 3 columns x 4 rows x 2 points (11 per row, column and point);
-# per (row, point): denominator 31, numerator and product 90, final accumulator
-# reduction 40, share of the M31 inversion chain 22
-OPS_PER_ROW_COL_POINT = 11
-OPS_PER_ROW_POINT = 183
+# VALU lane-ops the DEEP kernels issue, counted with rocprofv3 (SQ_INSTS_VALU x 64
+# per dispatch, profiles/deep_quotient_wide_shape_and_pow_grind_26bit_kernel_stats.md):
+# quotients_combine_kernel<8> per (row, accumulation); deep_fused_kernel<2,4> per
+# row of the wide shape's lifting domain (34 columns, 2 points)
+OPS_PER_ROW_ACC = 259
+FUSED_OPS_PER_ROW = 1496
 
 
 def load_traffic() -> dict:
@@ -154,14 +150,7 @@ def bench_plain(vx, args) -> dict:
     vx.lib.vx_stream_synchronize(stream)
     vx.check("warmup")
     log(f"warmup done ({args.warmup} proofs)")
-    if args.calibrate_copy_mb:
-        nb = args.calibrate_copy_mb << 20
-        src, dst = vx.DeviceBuffer(nb // 4), vx.DeviceBuffer(nb // 4)
-        vx.lib.vx_memcpy_d2d(dst.ptr, src.ptr, nb)
-        vx.synchronize()
-        log(f"calibration copy: {args.calibrate_copy_mb} MiB device to device")
-        src.free()
-        dst.free()
+    calibration_copy(vx, args)
 
     vx.synchronize()
     t_start = time.perf_counter()
@@ -355,6 +344,18 @@ def batch_jobs(seed: int, n_jobs: int, min_log: int, max_log: int, max_cols: int
         pos += 2 * cols
         jobs.append((log_n, [(int(v[2 * c]), int(v[2 * c + 1])) for c in range(cols)]))
     return jobs
+
+
+def calibration_copy(vx, args) -> None:
+    """A device-to-device copy of known size, the scale for the HBM byte counters."""
+    if args.calibrate_copy_mb:
+        nb = args.calibrate_copy_mb << 20
+        src, dst = vx.DeviceBuffer(nb // 4), vx.DeviceBuffer(nb // 4)
+        vx.lib.vx_memcpy_d2d(dst.ptr, src.ptr, nb)
+        vx.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mb} MiB device to device")
+        src.free()
+        dst.free()
 
 
 def vx_p() -> int:
@@ -582,7 +583,7 @@ def bench_deep(vx, args) -> dict:
     ms = t_num + t_comb
     # fused: each column read once, one denominator per (row, point)
     fused_bytes = sum(4 * (1 << l) for l in sizes) + rows * (8 + 16)
-    fused_ops = (OPS_PER_ROW_COL_POINT * 2 * len(sizes) + OPS_PER_ROW_POINT * 2) * rows
+    fused_ops = FUSED_OPS_PER_ROW * rows
     log(f"deep: numerators {t_num:.3f} ms ({bw:.0f} GB/s), combine {t_comb:.3f} ms ({ops:.2f} Tops/s), "
         f"{rows / (ms * 1e-3) / 1e6:.0f} Mrow/s, parity={parity}; fused {t_fused:.3f} ms "
         f"({fused_ops / (t_fused * 1e-3) / 1e12:.2f} Tops/s), same={fused_same}; oracle {cpu_s:.2f} s")
@@ -593,11 +594,13 @@ def bench_deep(vx, args) -> dict:
            "numerators_ms": round(t_num, 4), "combine_ms": round(t_comb, 4),
            "roofline_numerators": {"bound": "hbm", "kernel": "accumulate_numerators_kernel (6 launches)",
                                    "achieved": round(bw, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                   "frac": round(bw / HBM_PEAK_GBS, 4), "bytes": num_bytes},
+                                   "frac": round(bw / HBM_PEAK_GBS, 4), "bytes": num_bytes,
+                                   "traffic": TRAFFIC.get("deep:numerators")},
            "roofline_combine": {"bound": "valu", "kernel": "quotients_combine_kernel",
                                 "achieved": round(ops, 3), "peak": round(VALU_PEAK_TOPS, 2),
                                 "unit": "Tops/s (int32 lane-ops)", "frac": round(ops / VALU_PEAK_TOPS, 4),
-                                "ops_per_row_acc": OPS_PER_ROW_ACC, "hbm_bytes": comb_bytes},
+                                "ops_per_row_acc": OPS_PER_ROW_ACC, "hbm_bytes": comb_bytes,
+                                "traffic": TRAFFIC.get("deep:combine")},
            "parity_vs_oracle": parity,
            "fused": {"kernel": "deep_fused_kernel<2,4> (vx_deep_quotient: one launch, columns read once, "
                                "one denominator per row and point)",
@@ -606,12 +609,11 @@ def bench_deep(vx, args) -> dict:
                      "roofline": {"bound": "valu", "achieved": round(fused_ops / (t_fused * 1e-3) / 1e12, 3),
                                   "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s (int32 lane-ops)",
                                   "frac": round(fused_ops / (t_fused * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4),
-                                  "ops_per_row_col_point": OPS_PER_ROW_COL_POINT,
-                                  "ops_per_row_point": OPS_PER_ROW_POINT},
+                                  "ops_per_row": FUSED_OPS_PER_ROW},
                      "roofline_hbm": {"bound": "hbm", "achieved": round(fused_bytes / (t_fused * 1e-3) / 1e9, 1),
                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                       "frac": round(fused_bytes / (t_fused * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                                      "bytes": fused_bytes}}}
+                                      "bytes": fused_bytes, "traffic": TRAFFIC.get("deep:fused")}}}
     if not args.no_cpu:
         res["cpu_baseline"] = {"value": round(rows / cpu_s / 1e6, 3), "unit": "Mrow/s",
                                "cores": int(O.lib().orc_get_threads()), "kind": "port",
@@ -722,6 +724,7 @@ def main() -> int:
             f"{ {k: v['logc'] for k, v in MEASURED['merkle_stage_plans'].items()} }")
     extra = {}
     if "plain" in skip:  # profiling runs of the other workloads only
+        calibration_copy(vx, args)
         for name, fn in (("wide", bench_wide), ("batch", bench_batch), ("serve", bench_serve),
                          ("deep", bench_deep), ("grind", bench_grind)):
             if name not in skip:
