@@ -15,13 +15,15 @@ The same JSON line carries the other workloads of the metric:
           batched proving (graph replay) vs the same proofs one at a time;
   serve   Poisson arrivals of jobs with 1-34 columns, log_n uniform in
           [12, 26], admitted under the HBM budget: proofs/s and p50/p99;
+  host    the plain workload with its trace in host memory (pinned and
+          pageable), uploads overlapping the previous proof;
   deep    stwo DEEP quotient of the wide shape's 34 LDE columns at two
           sample points on the 2^22 lifting domain;
   grind   256 Blake2s proof-of-work searches at 26 bits from channel
           states on the device: nonces/s against the VALU roofline.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
-                    [--skip wide,batch,serve,deep,grind]
+                    [--skip wide,batch,serve,host,deep,grind]
 """
 
 from __future__ import annotations
@@ -621,6 +623,41 @@ def bench_deep(vx, args) -> dict:
     return res
 
 
+def bench_host(vx, args) -> dict:
+    """The plain workload with the trace handed over in host memory (the
+    reference's flow: trace in a pinned buffer, H2D, prove): vx_prover_prove_batch_host
+    calls back to back, so each call's 2^log_n-word upload overlaps the previous
+    proof. Pinned (vx_malloc_host) and pageable (numpy) traces. PCIe-inclusive:
+    reported beside the headline, never as it."""
+    log_n, nq, last = args.log_n, args.queries, args.log_last
+    a, b = 1262994060, 193768037
+    prover = vx.Prover(log_n, nq, last)
+    d = vx.fibonacci_trace_device(a, b, log_n)
+    vx.synchronize()
+    pinned = vx.PinnedBuffer(1 << log_n)
+    pinned.array[:] = d.to_host()
+    pageable = pinned.array.copy()
+    out = {}
+    for name, trace in (("pinned", pinned.addr), ("pageable", pageable)):
+        for _ in range(2):
+            prover.prove_batch_host([trace], [(a, b)])
+        vx.synchronize_stream(prover.stream)
+        K = max(4, args.steps // 2)
+        t = time.perf_counter()
+        for _ in range(K):
+            prover.prove_batch_host([trace], [(a, b)])
+        vx.synchronize_stream(prover.stream)
+        ms = (time.perf_counter() - t) / K * 1e3
+        out[name] = {"ms_per_proof": round(ms, 4), "melem_s": round((1 << log_n) / (ms * 1e-3) / 1e6, 2)}
+    same = prover.proof_bytes(0) == vx.Prover(log_n, nq, last).prove(a, b)
+    prover.close()
+    log(f"host traces: pinned {out['pinned']['ms_per_proof']:.3f} ms/proof, pageable "
+        f"{out['pageable']['ms_per_proof']:.3f} ms/proof, same proof: {same}")
+    return {"workload": "plain workload (2^%d rows) with the trace in host memory, uploaded per proof on the "
+                        "prover's upload stream while the previous proof runs" % log_n,
+            **out, "trace_bytes": 4 << log_n, "same_proof_as_device_trace": same}
+
+
 def bench_grind(vx, args) -> dict:
     """Blake2s PoW grinding at the reference's 26 bits: K channel states (splitmix64
     seed 7), each grind one vx_grind_blake2s call (prefix on the device, search,
@@ -692,7 +729,7 @@ def main() -> int:
     ap.add_argument("--no-pipelined", action="store_true", help="skip the two-stream plain measurement")
     ap.add_argument("--cpu-log-n", type=int, default=24)
     ap.add_argument("--cpu-reps", type=int, default=2)
-    ap.add_argument("--skip", default="", help="comma list of workloads to skip: plain,wide,batch,serve,deep,grind "
+    ap.add_argument("--skip", default="", help="comma list of workloads to skip: plain,wide,batch,serve,host,deep,grind "
                                                "(skipping plain leaves value null: for profiling runs)")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="offered load, jobs/s")
@@ -726,9 +763,9 @@ def main() -> int:
     if "plain" in skip:  # profiling runs of the other workloads only
         calibration_copy(vx, args)
         for name, fn in (("wide", bench_wide), ("batch", bench_batch), ("serve", bench_serve),
-                         ("deep", bench_deep), ("grind", bench_grind)):
+                         ("host", bench_host), ("deep", bench_deep), ("grind", bench_grind)):
             if name not in skip:
-                extra[name] = fn(vx, args)
+                extra["host_trace" if name == "host" else name] = fn(vx, args)
         print(json.dumps({"metric": METRIC, "value": None, "unit": "Melem/s", "n_gpus": 1, **extra}), flush=True)
         return 0
     plain = bench_plain(vx, args)
@@ -738,6 +775,8 @@ def main() -> int:
         extra["batch"] = bench_batch(vx, args)
     if "serve" not in skip:
         extra["serve"] = bench_serve(vx, args)
+    if "host" not in skip:
+        extra["host_trace"] = bench_host(vx, args)
     if "deep" not in skip:
         extra["deep"] = bench_deep(vx, args)
     if "grind" not in skip:

@@ -320,6 +320,16 @@ int32_t vx_prover_prove_device(vx_prover* p, const uint32_t* d_trace, uint32_t a
  * device pointers) with h_ab[2i], h_ab[2i+1] = (a, b) of proof i; count <= capacity. */
 int32_t vx_prover_prove_batch_device(vx_prover* p, const uint32_t* const* h_traces, const uint32_t* h_ab,
                                      uint32_t count);
+/* Batched form for traces in host memory (the reference's flow: CPU trace in a
+ * pinned buffer, H2D, prove; air.rs:136-189, buffer.rs:414-532, prover.rs:62-79):
+ * h_traces[i] are host pointers of 2^log_n words. They are copied into one of two
+ * device slots on the prover's upload stream while the previous call is still
+ * proving, then proved as vx_prover_prove_batch_device. Memory from
+ * vx_malloc_host is read asynchronously by DMA (keep it unchanged until the
+ * call's proofs are complete); other host memory is copied before this returns.
+ * The first call allocates the two slots (2 x capacity x 2^log_n words). */
+int32_t vx_prover_prove_batch_host(vx_prover* p, const uint32_t* const* h_traces, const uint32_t* h_ab,
+                                   uint32_t count);
 /* Batched Fibonacci proofs with the traces generated on the device from (a, b). */
 int32_t vx_prove_fibonacci_batch(vx_prover* p, const uint32_t* h_ab, uint32_t count);
 /* Replay the launch sequence of a batch as one HIP graph (captured on first use

@@ -84,6 +84,36 @@ class DeviceBuffer:
             pass
 
 
+class PinnedBuffer:
+    """Page-locked host memory of n u32 words (vx_malloc_host; the reference's
+    PinnedBuffer<u32>, buffer.rs:472-532): DMA reads it while the host goes on.
+    `array` is a numpy view of it."""
+
+    def __init__(self, n_words: int):
+        self.n = int(n_words)
+        self.ptr = C.c_void_p()
+        if lib.vx_malloc_host(C.byref(self.ptr), max(4, 4 * self.n)):
+            check("vx_malloc_host")
+            raise HipError("vx_malloc_host failed")
+        self.array = np.ctypeslib.as_array((C.c_uint32 * max(1, self.n)).from_address(self.ptr.value))[: self.n]
+
+    @property
+    def addr(self) -> int:
+        return self.ptr.value or 0
+
+    def free(self) -> None:
+        if self.ptr and self.ptr.value:
+            self.array = None
+            lib.vx_free_host(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def device_copy_words(addr: int, n: int) -> np.ndarray:
     out = np.empty(n, dtype=np.uint32)
     if n and lib.vx_memcpy_d2h(out.ctypes.data, C.c_void_p(addr), 4 * n):
@@ -150,6 +180,24 @@ class Prover:
         flat = (C.c_uint32 * (2 * len(ab)))(*[int(v) % P for pair in ab for v in pair])
         if lib.vx_prover_prove_batch_device(self.handle, ptrs, flat, len(trace_addrs)):
             check("vx_prover_prove_batch_device")
+
+    def prove_batch_host(self, traces, ab) -> None:
+        """Enqueue proofs of traces in host memory: numpy uint32 arrays (copied
+        before this returns) or addresses of vx_malloc_host memory (read by DMA
+        while the previous call proves)."""
+        keep, addrs = [], []
+        for t in traces:
+            if isinstance(t, int):
+                addrs.append(t)
+            else:
+                a = np.ascontiguousarray(t, dtype=np.uint32)
+                keep.append(a)
+                addrs.append(a.ctypes.data)
+        ptrs = (C.c_void_p * len(addrs))(*addrs)
+        flat = (C.c_uint32 * (2 * len(ab)))(*[int(v) % P for pair in ab for v in pair])
+        if lib.vx_prover_prove_batch_host(self.handle, ptrs, flat, len(addrs)):
+            check("vx_prover_prove_batch_host")
+            raise HipError("vx_prover_prove_batch_host failed")
 
     def set_graph(self, on: bool) -> None:
         if lib.vx_prover_set_graph(self.handle, 1 if on else 0):

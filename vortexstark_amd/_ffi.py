@@ -109,6 +109,7 @@ SIGNATURES = {
     "vx_merkle_stage_plan_set": (i32, [u32, i32]),
     "vx_merkle_stage_plan_reset": (None, []),
     "vx_prover_prove_batch_device": (i32, [vp, vp, vp, u32]),
+    "vx_prover_prove_batch_host": (i32, [vp, vp, vp, u32]),
     "vx_prove_fibonacci_batch": (i32, [vp, vp, u32]),
     "vx_prover_set_graph": (i32, [vp, i32]),
     "vx_prover_proof_device_at": (vp, [vp, u32]),

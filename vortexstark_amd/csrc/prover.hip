@@ -357,6 +357,15 @@ struct vx_prover {
     };
     bool use_graph = false;
     std::vector<Captured> graphs;  // most recently used last
+    // Host-trace calls (vx_prover_prove_batch_host): two device slots of
+    // `capacity` traces each, filled over PCIe on their own stream, so one
+    // call's upload overlaps the proving of the call before it; a slot is
+    // refilled only after the call that read it has finished.
+    hipStream_t up_stream = nullptr;
+    uint32_t* up_slot[2] = {nullptr, nullptr};
+    hipEvent_t up_done[2] = {nullptr, nullptr}, slot_free[2] = {nullptr, nullptr};
+    bool slot_used[2] = {false, false};
+    uint32_t up_next = 0;
     Timeline tl;
     void mark(const std::string& label) { tl.mark(label, stream); }
 
@@ -393,7 +402,13 @@ struct vx_prover {
         graphs.clear();
     }
     ~vx_prover() {
+        if (up_stream) (void)hipStreamSynchronize(up_stream);
         if (stream) (void)hipStreamSynchronize(stream);
+        for (int k = 0; k < 2; k++) {
+            if (up_done[k]) (void)hipEventDestroy(up_done[k]);
+            if (slot_free[k]) (void)hipEventDestroy(slot_free[k]);
+        }
+        if (up_stream) (void)hipStreamDestroy(up_stream);
         drop_graphs();
         for (void* p : allocs) (void)hipFree(p);
         if (call_copied) (void)hipEventDestroy(call_copied);
@@ -824,6 +839,50 @@ int32_t vx_prover_prove_batch_device(vx_prover* p, const uint32_t* const* h_trac
         }
     if (!upload_call(p, h_traces, h_ab, count)) return -1;
     return run_batch(p, count);
+}
+
+int32_t vx_prover_prove_batch_host(vx_prover* p, const uint32_t* const* h_traces, const uint32_t* h_ab,
+                                   uint32_t count) {
+    if (!p || !h_traces || !h_ab || count == 0 || count > p->capacity) {
+        set_error(-2, "vx_prover_prove_batch_host: bad arguments");
+        return -1;
+    }
+    for (uint32_t i = 0; i < count; i++)
+        if (!h_traces[i]) {
+            set_error(-2, "vx_prover_prove_batch_host: null trace");
+            return -1;
+        }
+    const uint64_t tb = 4 * p->N;
+    if (!p->up_stream) {  // first host-trace call: upload stream, events, two slots
+        bool ok = check(hipStreamCreateWithFlags(&p->up_stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (int k = 0; ok && k < 2; k++)
+            ok = check(hipEventCreateWithFlags(&p->up_done[k], hipEventDisableTiming), "hipEventCreate") &&
+                 check(hipEventCreateWithFlags(&p->slot_free[k], hipEventDisableTiming), "hipEventCreate") &&
+                 p->alloc((void**)&p->up_slot[k], tb * p->capacity);
+        if (!ok) return -1;
+    }
+    const uint32_t k = p->up_next;
+    hipStream_t u = p->up_stream;
+    if (p->slot_used[k] && !check(hipStreamWaitEvent(u, p->slot_free[k], 0), "hipStreamWaitEvent")) return -1;
+    std::vector<const uint32_t*> slots(count);
+    bool pageable = false;
+    for (uint32_t i = 0; i < count; i++) {
+        slots[i] = p->up_slot[k] + (uint64_t)i * p->N;
+        pageable = pageable || !device_view_of_host((void*)h_traces[i]);
+        if (!check(hipMemcpyAsync((void*)slots[i], h_traces[i], tb, hipMemcpyHostToDevice, u), "H2D trace"))
+            return -1;
+    }
+    if (!check(hipEventRecord(p->up_done[k], u), "hipEventRecord") ||
+        !check(hipStreamWaitEvent(p->stream, p->up_done[k], 0), "hipStreamWaitEvent"))
+        return -1;
+    // memory that is not pinned may be freed or reused as soon as this returns:
+    // wait for its copies (the previous call keeps proving meanwhile)
+    if (pageable && !check(hipEventSynchronize(p->up_done[k]), "hipEventSynchronize")) return -1;
+    if (!upload_call(p, slots.data(), h_ab, count) || run_batch(p, count)) return -1;
+    if (!check(hipEventRecord(p->slot_free[k], p->stream), "hipEventRecord")) return -1;
+    p->slot_used[k] = true;
+    p->up_next ^= 1;
+    return 0;
 }
 
 int32_t vx_prover_prove_device(vx_prover* p, const uint32_t* d_trace, uint32_t a, uint32_t b) {
