@@ -40,8 +40,8 @@ def gpu_numerators(vx, cols, idx, b, c, n_rows):
     return np.stack([o.to_host() for o in outs])
 
 
-@pytest.mark.parametrize("n_rows,n_batch", [(1, 1), (2, 0), (255, 2), (257, 3), (1000, 4), (1 << 12, 7),
-                                            (1 << 20, 5)])
+@pytest.mark.parametrize("n_rows,n_batch", [(1, 1), (2, 0), (255, 2), (257, 3), (512, 3), (1000, 4), (1536, 6),
+                                            (1 << 12, 7), (1 << 20, 5)])
 def test_accumulate_numerators(vx, n_rows, n_batch):
     rng = np.random.default_rng(n_rows + n_batch)
     cols = [rnd(rng, n_rows) for _ in range(6)]

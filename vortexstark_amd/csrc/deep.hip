@@ -8,7 +8,7 @@
 //  accumulate_lift         col[i] += curr[lift(i)]   (QM31, 4 words per element)
 //
 // accumulate_numerators is bound by HBM (4 bytes of column a term): each thread
-// takes four consecutive rows (uint4 loads), the sum of products c_i * f is kept
+// takes eight rows (two uint4 loads a column, a wave reading 2 KiB of it), the sum of products c_i * f is kept
 // in 64 bits (one v_mad_u64_u32 per coordinate and column, a partial reduction
 // every three columns, the full one at the end), and sum_i b_i is formed once
 // per thread. quotients_combine is bound by the VALU: den_j(row) is the CM31
@@ -34,20 +34,32 @@ __device__ __forceinline__ uint32_t reduce64(uint64_t x) {
     return canon((uint32_t)(s & P) + (uint32_t)(s >> 31));
 }
 
-// ROWS consecutive rows per thread: 4 (uint4 loads and stores; n_rows % 4 == 0 and
-// 16-byte aligned outputs, checked on the host; a column whose pointer is not
-// 16-byte aligned is read word by word, a wave-uniform branch) or 1.
+// ROWS rows per thread (row_of): 8 (uint4 loads and stores; n_rows % 512 == 0
+// and 16-byte aligned outputs, checked on the host; a column whose pointer is
+// not 16-byte aligned is read word by word, a wave-uniform branch) or 1.
+// Rows of a thread: ROWS = 1: row0; ROWS = 8: row0 .. row0 + 3 and row0 + 256 ..
+// row0 + 259, so that a wave reads 2 KiB of a column contiguously (lane l at
+// row0 = 512 * wave + 4 * l). Wider contiguous reads per wave and column stream
+// faster from HBM when a thread reads many columns.
+template <uint32_t ROWS>
+__device__ __forceinline__ uint64_t row_of(uint64_t row0, uint32_t r) {
+    return ROWS == 1 ? row0 : row0 + (r & 3) + 256 * (r >> 2);
+}
 template <uint32_t ROWS>
 __device__ __forceinline__ void load_rows(const uint32_t* col, uint64_t row0, uint32_t f[ROWS]) {
-    if (ROWS == 4 && ((uintptr_t)col & 15) == 0) {
-        const uint4 v = *(const uint4*)(col + row0);
+    if (ROWS == 8 && ((uintptr_t)col & 15) == 0) {
+        const uint4 v = *(const uint4*)(col + row0), w = *(const uint4*)(col + row0 + 256);
         f[0] = v.x;
         f[1] = v.y;
         f[2] = v.z;
         f[3] = v.w;
+        f[4] = w.x;
+        f[5] = w.y;
+        f[6] = w.z;
+        f[7] = w.w;
     } else {
 #pragma unroll
-        for (uint32_t r = 0; r < ROWS; r++) f[r] = col[row0 + r];
+        for (uint32_t r = 0; r < ROWS; r++) f[r] = col[row_of<ROWS>(row0, r)];
     }
 }
 
@@ -57,7 +69,9 @@ __global__ __launch_bounds__(DEEP_THREADS) void accumulate_numerators_kernel(
     const uint32_t* __restrict__ b_coeffs, const uint32_t* __restrict__ c_coeffs, uint32_t n_batch, uint32_t n_rows,
     uint32_t* __restrict__ out0, uint32_t* __restrict__ out1, uint32_t* __restrict__ out2,
     uint32_t* __restrict__ out3) {
-    const uint64_t row0 = ((uint64_t)blockIdx.x * DEEP_THREADS + threadIdx.x) * ROWS;
+    const uint64_t row0 = ROWS == 1 ? (uint64_t)blockIdx.x * DEEP_THREADS + threadIdx.x
+                                    : ((uint64_t)blockIdx.x * (DEEP_THREADS / 64) + (threadIdx.x >> 6)) * 64 * ROWS +
+                                          4 * (threadIdx.x & 63);
     if (row0 >= n_rows) return;
     // sum_i b_i: the same for every row
     uint32_t bs[4] = {0, 0, 0, 0};
@@ -105,10 +119,12 @@ __global__ __launch_bounds__(DEEP_THREADS) void accumulate_numerators_kernel(
     uint32_t* outs[4] = {out0, out1, out2, out3};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        if (ROWS == 4)
+        if (ROWS == 8) {
             *(uint4*)(outs[k] + row0) = make_uint4(o[k][0], o[k][1], o[k][2], o[k][3]);
-        else
+            *(uint4*)(outs[k] + row0 + 256) = make_uint4(o[k][4], o[k][5], o[k][6], o[k][7]);
+        } else {
             outs[k][row0] = o[k][0];
+        }
     }
 }
 
@@ -379,9 +395,10 @@ void cuda_accumulate_numerators(const uint32_t* const* col_ptrs, const uint32_t*
                                 const uint32_t* b_coeffs, const uint32_t* c_coeffs, uint32_t n_batch_cols,
                                 uint32_t n_rows, uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3) {
     if (!n_rows) return;
-    const bool vec = n_rows % 4 == 0 && (((uintptr_t)out0 | (uintptr_t)out1 | (uintptr_t)out2 | (uintptr_t)out3) & 15) == 0;
+    // whole 512-row wave chunks and 16-byte aligned outputs: the wide form
+    const bool vec = n_rows % 512 == 0 && (((uintptr_t)out0 | (uintptr_t)out1 | (uintptr_t)out2 | (uintptr_t)out3) & 15) == 0;
     if (vec)
-        accumulate_numerators_kernel<4><<<ceil_div(n_rows, 4 * DEEP_THREADS), DEEP_THREADS, 0, current_stream()>>>(
+        accumulate_numerators_kernel<8><<<ceil_div(n_rows, 8 * DEEP_THREADS), DEEP_THREADS, 0, current_stream()>>>(
             col_ptrs, col_indices, b_coeffs, c_coeffs, n_batch_cols, n_rows, out0, out1, out2, out3);
     else
         accumulate_numerators_kernel<1><<<ceil_div(n_rows, DEEP_THREADS), DEEP_THREADS, 0, current_stream()>>>(
