@@ -280,7 +280,8 @@ void vx_merkle_stage_plan_reset(void);
  * smallest valid nonce of [start, start + count) or 2^64 - 1, and with mix != 0
  * a found nonce is mixed into the state (n_draws reset to 0). Asynchronous on
  * stream (null: the default stream); one launch for any count, no host round
- * trip. Returns 0, or -1 with the error recorded. */
+ * trip. pow_bits above 128 is rejected (no nonce can meet it). Returns 0, or -1
+ * with the error recorded. */
 int32_t vx_grind_blake2s(void* channel, uint32_t pow_bits, int32_t m31_output, uint64_t start, uint64_t count,
                          uint64_t* result, int32_t mix, void* stream);
 

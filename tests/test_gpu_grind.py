@@ -171,13 +171,17 @@ def test_device_grind_mixes_nonce_into_transcript(vx):
     O.lib().orc_channel_mix_u64(C.byref(oc), nonce)
     got = ch.to_host()
     assert got[:8].tobytes() == bytes(oc.state) and got[8] == 0
-    # nothing found: the transcript stays as it was
+    # nothing found in the range (the oracle agrees): the transcript stays as it was
     ch2, res2 = dev_channel(vx, st), u64_buf(vx)
-    vx.grind_device(ch2.addr, 200, res2.addr, count=1 << 16, mix=True)
+    assert O.pow_search(O.pow_prefix(st, 40), 40, False, 0, 1 << 16) == NONE
+    vx.grind_device(ch2.addr, 40, res2.addr, count=1 << 16, mix=True)
     vx.synchronize()
     assert read_u64(res2) == NONE
     assert np.array_equal(ch2.to_host()[:9], np.concatenate([st, [5]]).astype(np.uint32))
     vx.check()
+    # a target no 128-bit value can meet is refused
+    with pytest.raises(vx.HipError):
+        vx.grind_device(ch2.addr, 129, res2.addr, count=1 << 16)
 
 
 def test_python_grind_matches_oracle(vx):

@@ -252,14 +252,14 @@ constexpr uint32_t DEEP_MAX_POINTS = 4;
 
 template <uint32_t NP, uint32_t R>
 __device__ __forceinline__ void fused_columns(const uint32_t* __restrict__ cm, const uint32_t* __restrict__ cf,
-                                              uint32_t c, uint32_t n, const uint64_t row[R], const bool live[R],
+                                              uint32_t c, uint32_t n, const uint32_t row[R],
                                               uint64_t acc[NP][R][4]) {
     uint32_t f[3][R];
     for (uint32_t g = 0; g < n; g++) {
         const uint32_t* ptr = (const uint32_t*)(uintptr_t)(cm[4 * (c + g)] | ((uint64_t)cm[4 * (c + g) + 1] << 32));
         const uint32_t sh = cm[4 * (c + g) + 2];
 #pragma unroll
-        for (uint32_t r = 0; r < R; r++) f[g][r] = ptr[live[r] ? ((row[r] >> sh) << 1) + (row[r] & 1) : 0];
+        for (uint32_t r = 0; r < R; r++) f[g][r] = ptr[((row[r] >> sh) << 1) | (row[r] & 1)];
     }
     constexpr uint64_t M62 = (1ull << 62) - 1;
 #pragma unroll
@@ -283,23 +283,25 @@ __global__ __launch_bounds__(DEEP_THREADS) void deep_fused_kernel(const uint32_t
                                                                   uint32_t* __restrict__ out2,
                                                                   uint32_t* __restrict__ out3) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t base = ((uint64_t)blockIdx.x * (DEEP_THREADS / 64) + (threadIdx.x >> 6)) * (64u * R);
+    // n_rows <= 2^30: row indices fit 32 bits
+    const uint32_t base = (blockIdx.x * (DEEP_THREADS / 64) + (threadIdx.x >> 6)) * (64u * R);
     if (base >= n_rows) return;
     const uint32_t* pts = meta;
     const uint32_t* cm = meta + 16 * NP;
     const uint32_t* cf = cm + 4 * n_cols;
-    uint64_t row[R];
+    uint32_t row[R];  // rows past the end read the last row; their results are dropped
     bool live[R];
 #pragma unroll
     for (uint32_t r = 0; r < R; r++) {
-        row[r] = base + 64u * r + lane;
-        live[r] = row[r] < n_rows;
+        const uint32_t x = base + 64u * r + lane;
+        live[r] = x < n_rows;
+        row[r] = live[r] ? x : n_rows - 1;
     }
     uint64_t acc[NP][R][4] = {};
     // three products on top of a partially reduced value (< 2^62 + 4) stay below 2^64
     uint32_t c = 0;
-    for (; c + 3 <= n_cols; c += 3) fused_columns<NP, R>(cm, cf, c, 3, row, live, acc);
-    if (c < n_cols) fused_columns<NP, R>(cm, cf, c, n_cols - c, row, live, acc);
+    for (; c + 3 <= n_cols; c += 3) fused_columns<NP, R>(cm, cf, c, 3, row, acc);
+    if (c < n_cols) fused_columns<NP, R>(cm, cf, c, n_cols - c, row, acc);
     uint32_t dx[R], dy[R];
 #pragma unroll
     for (uint32_t r = 0; r < R; r++) {

@@ -211,6 +211,10 @@ int32_t vx_grind_blake2s(void* channel, uint32_t pow_bits, int32_t m31_output, u
         set_error(hipErrorInvalidValue, "vx_grind_blake2s: null pointer");
         return -1;
     }
+    if (pow_bits > 128) {  // no 128-bit value has more trailing zeros: the search could never end
+        set_error(-2, "vx_grind_blake2s: pow_bits above 128 can never be met");
+        return -1;
+    }
     hipStream_t s = (hipStream_t)stream;
     grind_init_kernel<<<1, 64, 0, s>>>(result);
     if (!check_launch("grind_init_kernel")) return -1;
