@@ -63,7 +63,7 @@ POW_BITS = 26  # the reference transcripts' PoW (prover.rs:96)
 # quotients_combine_kernel<8> per (row, accumulation); deep_fused_kernel<2,4> per
 # row of the wide shape's lifting domain (34 columns, 2 points)
 OPS_PER_ROW_ACC = 259
-FUSED_OPS_PER_ROW = 1496
+FUSED_OPS_PER_ROW = 1388
 
 
 def load_traffic() -> dict:
