@@ -132,6 +132,9 @@ double orc_time_wide_commit(const uint32_t* const* trace, const uint32_t* log_si
 
 /* ---- Blake2s proof-of-work grinding (channel.rs:140-168, cuda/grind*.cu) ---- */
 uint32_t orc_pow_reduce_word(uint32_t x); /* m31_reduce of one digest word (m31.cuh:9-14) */
+/* m31_output: every word of the digest reduced mod p, as grind_ops.rs:157-161 states stwo's
+ * reduce_to_m31 does; stwo is absent, so that all eight words are reduced is parity unpinned
+ * (the digest and the reduction itself are pinned by hashlib and x mod p). */
 void orc_pow_prefix(const uint32_t state[8], uint32_t pow_bits, int m31_output, uint32_t out[8]);
 uint32_t orc_pow_zeros(const uint32_t prefixed[8], uint64_t nonce, int m31_output);
 /* smallest nonce in [start, start + count) with >= pow_bits zeros, else UINT64_MAX */
