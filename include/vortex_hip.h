@@ -190,8 +190,9 @@ void cuda_accumulate_lift(uint32_t* col, const uint32_t* curr, uint32_t col_n, u
  * column not sampled there); domain_xs/ys: device, the lifting domain's points
  * (vx_circle_domain_points). out*: device SoA QM31, 2^lifting_log_size rows.
  *   out[row] = sum_p (sum_c (c_pc col_c[lift_c(row)] - b_pc) - (sum_c a_pc) y) / den_p(row)
- * Asynchronous on stream; a vx_deep holds its metadata staging, so calls on one
- * object are ordered (a call waits for the previous call's upload). */
+ * Asynchronous on stream. A vx_deep stages each call's metadata in one of two
+ * pinned/device slots in turn: a call waits on the host only for the upload of
+ * the call two back, and its stream for that call's kernel. */
 typedef struct vx_deep vx_deep;
 vx_deep* vx_deep_create(uint32_t max_cols, uint32_t max_points);
 int32_t vx_deep_quotient(vx_deep* d, const uint32_t* const* cols, const uint32_t* col_log_sizes, uint32_t n_cols,

@@ -212,3 +212,41 @@ def test_fused_quotient_rejects_bad_shapes(vx):
     with pytest.raises(vx.HipError):
         dq.quotient([buf.addr], [5], pts[:1], lc[:1, :1], 4, buf.addr, buf.addr, [buf.addr] * 4)  # log 5 > 4
     dq.close()
+
+
+def test_fused_quotient_calls_in_flight_on_two_streams(vx):
+    """Three calls on one object on two streams with no host wait in between:
+    the third call reuses the metadata slot of the first, which ran on the other
+    stream, so its stream must wait for that call's kernel."""
+    rng = np.random.default_rng(21)
+    L, logs, n_points = 12, [12, 10, 12, 11], 2
+    cols = [rnd(rng, 1 << l) for l in logs]
+    dcols = [vx.DeviceBuffer.from_host(c) for c in cols]
+    points = rnd(rng, n_points, 8)
+    dxs, dys = vx.circle_domain_points(L)
+    exs, eys = O.circle_domain_brev(L)
+    vx.synchronize()
+    streams = []
+    for _ in range(2):
+        sp = C.c_void_p()
+        assert vx.lib.vx_stream_create(C.byref(sp)) == 0
+        streams.append(sp.value)
+    dq = vx.DeepQuotient(len(cols), n_points)
+    lcs, outs = [], []
+    for call in range(3):
+        lc = rnd(rng, n_points, len(cols), 12)
+        out = [vx.DeviceBuffer(1 << L) for _ in range(4)]
+        # calls 0, 1, 2 on streams 0, 1, 1: call 2 reuses call 0's slot from the other stream
+        dq.quotient([c.addr for c in dcols], logs, points, lc, L, dxs.addr, dys.addr, [o.addr for o in out],
+                    stream=streams[min(call, 1)])
+        lcs.append(lc)
+        outs.append(out)
+    for sp in streams:
+        vx.synchronize_stream(sp)
+    vx.check("vx_deep_quotient on two streams")
+    for lc, out in zip(lcs, outs):
+        exp = oracle_pipeline(cols, logs, points, lc, L, exs, eys)
+        assert np.array_equal(np.stack([o.to_host() for o in out]), exp)
+    dq.close()
+    for sp in streams:
+        vx.lib.vx_stream_destroy(C.c_void_p(sp))

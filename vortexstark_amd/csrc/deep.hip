@@ -382,13 +382,17 @@ inline uint32_t hadd31(uint32_t a, uint32_t b) {
 
 using namespace vx;
 
+// Metadata goes through two pinned staging / device slots used in turn: a call
+// waits (host) only for the copy of the call two back, and its stream waits for
+// that call's kernel before overwriting the device slot.
 struct vx_deep {
     uint32_t max_cols = 0, max_points = 0;
     size_t bytes = 0;
-    uint32_t* h_meta = nullptr;  // pinned staging
-    uint32_t* d_meta = nullptr;
-    hipEvent_t copied = nullptr, done = nullptr;
-    bool pending = false;
+    uint32_t* h_meta[2] = {nullptr, nullptr};  // pinned staging
+    uint32_t* d_meta[2] = {nullptr, nullptr};
+    hipEvent_t copied[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    uint32_t next = 0;
 };
 
 extern "C" {
@@ -443,10 +447,12 @@ vx_deep* vx_deep_create(uint32_t max_cols, uint32_t max_points) {
     d->max_cols = max_cols;
     d->max_points = max_points;
     d->bytes = 4ull * (16ull * max_points + 4ull * max_cols + 4ull * max_cols * max_points);
-    bool ok = check(hipHostMalloc((void**)&d->h_meta, d->bytes), "hipHostMalloc") &&
-              check(hipMalloc((void**)&d->d_meta, d->bytes), "hipMalloc") &&
-              check(hipEventCreateWithFlags(&d->copied, hipEventDisableTiming), "hipEventCreate") &&
-              check(hipEventCreateWithFlags(&d->done, hipEventDisableTiming), "hipEventCreate");
+    bool ok = true;
+    for (int k = 0; ok && k < 2; k++)
+        ok = check(hipHostMalloc((void**)&d->h_meta[k], d->bytes), "hipHostMalloc") &&
+             check(hipMalloc((void**)&d->d_meta[k], d->bytes), "hipMalloc") &&
+             check(hipEventCreateWithFlags(&d->copied[k], hipEventDisableTiming), "hipEventCreate") &&
+             check(hipEventCreateWithFlags(&d->done[k], hipEventDisableTiming), "hipEventCreate");
     if (!ok) {
         vx_deep_destroy(d);
         return nullptr;
@@ -456,11 +462,13 @@ vx_deep* vx_deep_create(uint32_t max_cols, uint32_t max_points) {
 
 void vx_deep_destroy(vx_deep* d) {
     if (!d) return;
-    if (d->pending) (void)hipEventSynchronize(d->done);
-    if (d->h_meta) (void)hipHostFree(d->h_meta);
-    if (d->d_meta) (void)hipFree(d->d_meta);
-    if (d->copied) (void)hipEventDestroy(d->copied);
-    if (d->done) (void)hipEventDestroy(d->done);
+    for (int k = 0; k < 2; k++) {
+        if (d->pending[k]) (void)hipEventSynchronize(d->done[k]);
+        if (d->h_meta[k]) (void)hipHostFree(d->h_meta[k]);
+        if (d->d_meta[k]) (void)hipFree(d->d_meta[k]);
+        if (d->copied[k]) (void)hipEventDestroy(d->copied[k]);
+        if (d->done[k]) (void)hipEventDestroy(d->done[k]);
+    }
     delete d;
 }
 
@@ -484,11 +492,12 @@ int32_t vx_deep_quotient(vx_deep* d, const uint32_t* const* cols, const uint32_t
             return -1;
         }
     hipStream_t s = (hipStream_t)stream;
-    // the staging may still feed the previous call's copy, and the device copy its kernel
-    if (d->pending && !(check(hipEventSynchronize(d->copied), "hipEventSynchronize") &&
-                        check(hipStreamWaitEvent(s, d->done, 0), "hipStreamWaitEvent")))
+    // this slot may still feed the copy of the call two back, and its device copy that call's kernel
+    const uint32_t slot = d->next;
+    if (d->pending[slot] && !(check(hipEventSynchronize(d->copied[slot]), "hipEventSynchronize") &&
+                              check(hipStreamWaitEvent(s, d->done[slot], 0), "hipStreamWaitEvent")))
         return -1;
-    uint32_t* h = d->h_meta;
+    uint32_t* h = d->h_meta[slot];
     const uint32_t NP = n_points;
     for (uint32_t p = 0; p < NP; p++) {
         uint32_t* pp = h + 16 * p;
@@ -515,15 +524,15 @@ int32_t vx_deep_quotient(vx_deep* d, const uint32_t* const* cols, const uint32_t
                 cf[(c * NP + p) * 4 + k] = line_coeffs[12ull * ((uint64_t)p * n_cols + c) + 8 + k] % P;
     }
     const size_t bytes = 4ull * (16ull * NP + 4ull * n_cols + 4ull * n_cols * NP);
-    if (!check(hipMemcpyAsync(d->d_meta, h, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync") ||
-        !check(hipEventRecord(d->copied, s), "hipEventRecord"))
+    if (!check(hipMemcpyAsync(d->d_meta[slot], h, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync") ||
+        !check(hipEventRecord(d->copied[slot], s), "hipEventRecord"))
         return -1;
     const uint32_t n_rows = 1u << lifting_log_size;
     switch (NP) {
 #define VX_DEEP_LAUNCH(np, r)                                                                                  \
     case np:                                                                                                   \
         deep_fused_kernel<np, r><<<ceil_div(n_rows, (uint64_t)DEEP_THREADS * r), DEEP_THREADS, 0, s>>>(        \
-            d->d_meta, n_cols, domain_xs, domain_ys, n_rows, out0, out1, out2, out3);                          \
+            d->d_meta[slot], n_cols, domain_xs, domain_ys, n_rows, out0, out1, out2, out3);                          \
         break;
         VX_DEEP_LAUNCH(1, 4)
         VX_DEEP_LAUNCH(2, 4)
@@ -531,8 +540,9 @@ int32_t vx_deep_quotient(vx_deep* d, const uint32_t* const* cols, const uint32_t
         VX_DEEP_LAUNCH(4, 2)
 #undef VX_DEEP_LAUNCH
     }
-    if (!check_launch("deep_fused_kernel") || !check(hipEventRecord(d->done, s), "hipEventRecord")) return -1;
-    d->pending = true;
+    if (!check_launch("deep_fused_kernel") || !check(hipEventRecord(d->done[slot], s), "hipEventRecord")) return -1;
+    d->pending[slot] = true;
+    d->next ^= 1;
     return 0;
 }
 
