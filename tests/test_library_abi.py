@@ -63,3 +63,31 @@ def test_prover_estimate_bytes_host_only():
     for log_n in (10, 16, 20):
         E = 4 << log_n
         assert est(log_n, 80, 3, 2) - est(log_n, 80, 3, 1) >= 4 * (E // 4) + 4 * E + 16 * (E // 2)
+
+
+def test_argument_errors_are_reported_without_a_device():
+    """Entry points reject bad arguments on the host, before any device call,
+    and report them through vx_get_last_error (never by aborting)."""
+    import ctypes
+
+    import vortexstark_amd as vx
+
+    lib = vx.lib
+    lib.vx_get_last_error()  # clear
+    fake = ctypes.c_void_p(0x1000)
+    # a PoW target above 128 bits can never be met
+    assert lib.vx_grind_blake2s(fake, 129, 0, 0, 1 << 20, fake, 0, None) == -1
+    assert lib.vx_get_last_error() != 0
+    assert lib.vx_grind_blake2s(None, 10, 0, 0, 1, fake, 0, None) == -1
+    assert lib.vx_get_last_error() != 0
+    # DEEP objects: 0 columns, 5 points
+    assert not lib.vx_deep_create(0, 2) and lib.vx_get_last_error() != 0
+    assert not lib.vx_deep_create(8, 5) and lib.vx_get_last_error() != 0
+    assert lib.vx_deep_quotient(None, None, None, 0, 0, None, None, 0, None, None, None, None, None, None,
+                                None) == -1
+    assert lib.vx_get_last_error() != 0
+    # domain points outside 1..30
+    assert lib.vx_circle_domain_points(31, fake, fake, None) == -1 and lib.vx_get_last_error() != 0
+    # host-trace proving without a prover
+    assert lib.vx_prover_prove_batch_host(None, None, None, 1) == -1 and lib.vx_get_last_error() != 0
+    assert lib.vx_get_last_error() == 0  # reading clears it
