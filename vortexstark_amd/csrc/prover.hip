@@ -853,13 +853,21 @@ int32_t vx_prover_prove_batch_host(vx_prover* p, const uint32_t* const* h_traces
             return -1;
         }
     const uint64_t tb = 4 * p->N;
-    if (!p->up_stream) {  // first host-trace call: upload stream, events, two slots
-        bool ok = check(hipStreamCreateWithFlags(&p->up_stream, hipStreamNonBlocking), "hipStreamCreate");
-        for (int k = 0; ok && k < 2; k++)
-            ok = check(hipEventCreateWithFlags(&p->up_done[k], hipEventDisableTiming), "hipEventCreate") &&
-                 check(hipEventCreateWithFlags(&p->slot_free[k], hipEventDisableTiming), "hipEventCreate") &&
-                 p->alloc((void**)&p->up_slot[k], tb * p->capacity);
-        if (!ok) return -1;
+    if (!p->up_stream) {  // first host-trace call: two slots, events, then the upload stream
+        bool ok = true;
+        for (int k = 0; ok && k < 2; k++) {
+            if (!p->up_slot[k]) ok = p->alloc((void**)&p->up_slot[k], tb * p->capacity);
+            if (ok && !p->up_done[k])
+                ok = check(hipEventCreateWithFlags(&p->up_done[k], hipEventDisableTiming), "hipEventCreate");
+            if (ok && !p->slot_free[k])
+                ok = check(hipEventCreateWithFlags(&p->slot_free[k], hipEventDisableTiming), "hipEventCreate");
+        }
+        // the stream is made last: its presence means everything above exists (a
+        // failed first call leaves what it made for the next call to complete)
+        if (!ok || !check(hipStreamCreateWithFlags(&p->up_stream, hipStreamNonBlocking), "hipStreamCreate")) {
+            p->up_stream = nullptr;
+            return -1;
+        }
     }
     const uint32_t k = p->up_next;
     hipStream_t u = p->up_stream;
