@@ -381,9 +381,12 @@ def bench_batch(vx, args) -> dict:
     for p in solo.values():
         p.close()
     log(f"batch: {n_proofs} proofs one at a time: {t_solo:.3f}s ({n_proofs / t_solo:.1f} proofs/s)")
-    # batched: all jobs submitted at once; provers created and graphs captured by a first pass
+    # batched: all jobs submitted at once; provers made ahead (vx_server_prewarm, two per
+    # size class) and graphs captured by a first pass, so the timed pass sets nothing up
     srv = vx.Server(0, nq, last, result_bytes=4 << 30)
     srv.set_graphs(True)
+    for l in sorted({l for l, _ in jobs}):
+        srv.prewarm(l, 2)
     warm = [srv.submit(l, ab) for l, ab in jobs]
     srv.wait_all()
     for i in warm:

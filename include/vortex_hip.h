@@ -430,6 +430,11 @@ int64_t vx_server_submit(vx_server* s, uint32_t log_n, uint32_t n_cols, const ui
 /* One scheduling step: retire finished batches, launch batches on idle provers,
  * add provers within the budget. Returns the number of actions taken. */
 int32_t vx_server_poll(vx_server* s);
+/* Creates up to n_provers (at most 2) provers of size class log_n ahead of
+ * traffic, within the HBM budget, so the first batches do not pay for twiddle
+ * set-up (the reference's prewarm_eval_pool, prover.rs:62-79). Returns the
+ * number created, or -1. */
+int32_t vx_server_prewarm(vx_server* s, uint32_t log_n, uint32_t n_provers);
 /* Batches replay captured HIP graphs (one per prover and batch count). */
 int32_t vx_server_set_graphs(vx_server* s, int32_t enable);
 int32_t vx_server_wait_all(vx_server* s);

@@ -97,3 +97,26 @@ def test_server_ring_full_without_release_fails_cleanly(vx):
     st = [srv.status(i) for i in ids]
     assert st.count(1) == 2 and all(x == -5 for x in st if x != 1)
     srv.close()
+
+
+def test_prewarm_makes_provers_ahead_of_traffic(vx):
+    """vx_server_prewarm creates up to two provers of a class within the budget;
+    proofs served afterwards are the oracle's."""
+    import oracle_lib as O
+
+    srv = vx.Server(0, 80, 3)
+    assert srv.prewarm(10, 2) == 2
+    assert srv.prewarm(10, 5) == 0  # already at the per-class limit
+    held = srv.stats()["device_bytes"]
+    assert held > 0
+    ab = [(3, 5), (7, 11), (13, 17)]
+    job = srv.submit(10, ab)
+    srv.wait_all()
+    for c, (a, b) in enumerate(ab):
+        assert srv.proof(job, c, 10) == O.prove(a, b, 10, 80, 3)
+    assert srv.stats()["batches"] >= 1
+    srv.close()
+    # a budget that holds no prover makes none
+    small = vx.Server(1 << 20, 80, 3)
+    assert small.prewarm(12, 2) == 0
+    small.close()

@@ -385,6 +385,14 @@ class Server:
             raise HipError("vx_server_submit failed")
         return job
 
+    def prewarm(self, log_n: int, n_provers: int = 2) -> int:
+        """Create provers of a size class before its traffic arrives; returns how many."""
+        n = lib.vx_server_prewarm(self.handle, log_n, n_provers)
+        if n < 0:
+            check("vx_server_prewarm")
+            raise HipError("vx_server_prewarm failed")
+        return n
+
     def poll(self) -> int:
         return lib.vx_server_poll(self.handle)
 

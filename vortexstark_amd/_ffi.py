@@ -150,6 +150,7 @@ SIGNATURES = {
     "vx_server_destroy": (None, [vp]),
     "vx_server_submit": (C.c_int64, [vp, u32, u32, vp]),
     "vx_server_poll": (i32, [vp]),
+    "vx_server_prewarm": (i32, [vp, u32, u32]),
     "vx_server_set_graphs": (i32, [vp, i32]),
     "vx_server_wait_all": (i32, [vp]),
     "vx_server_jobs": (u64, [vp]),

@@ -12,8 +12,9 @@
 //    requests that arrive while a batch runs join the next batch of their
 //    class, and batches of different classes overlap on the device;
 //  * a prover is created when its class has work and its arena fits the HBM
-//    budget; idle provers of classes without work are released (least
-//    recently used first) to make room;
+//    budget (or ahead of traffic by vx_server_prewarm); idle provers of
+//    classes without work are released (least recently used first) to make
+//    room;
 //  * finished proofs go by one strided DMA copy per batch into a pinned host
 //    ring, where they stay until the job is released; a batch waits while the
 //    ring has no room for it;
@@ -323,6 +324,25 @@ int64_t vx_server_submit(vx_server* s, uint32_t log_n, uint32_t n_cols, const ui
 }
 
 int32_t vx_server_poll(vx_server* s) { return s ? s->poll() : -1; }
+
+int32_t vx_server_prewarm(vx_server* s, uint32_t log_n, uint32_t n_provers) {
+    if (!s || log_n < 2 || log_n + 2 > 30 || s->log_last > log_n + 1) {
+        vx::set_error(-2, "vx_server_prewarm: bad size class");
+        return -1;
+    }
+    Class& c = s->classes[log_n];
+    const uint32_t want = std::min<uint32_t>(n_provers, MAX_PROVERS_PER_CLASS);
+    int32_t made = 0;
+    while (c.slots.size() < want) {
+        if (!s->reserve_room(s->estimate_bytes(log_n), log_n)) break;  // the budget is full
+        auto sl = std::make_unique<Slot>();
+        if (!s->make_slot(log_n, *sl)) return -1;
+        sl->last_used = now_s();
+        c.slots.push_back(std::move(sl));
+        made++;
+    }
+    return made;
+}
 
 int32_t vx_server_set_graphs(vx_server* s, int32_t enable) {
     if (!s) return -1;
