@@ -698,6 +698,32 @@ def bench_grind(vx, args) -> dict:
                 for i in range(2))
     log(f"grind: {K} x {POW_BITS}-bit PoW in {best * 1e3:.1f} ms ({best / K * 1e3:.3f} ms each), "
         f"{rate / 1e9:.2f} G nonces/s, valid={valid} first={first}")
+    # the reference's host loop over the ABI (grind_ops.rs:70-115): prefix on the host,
+    # batches of 2^24 nonces, a synchronisation and a result read after each batch
+    R = 32
+    prefixes = [O.pow_prefix(seeds[i], POW_BITS) for i in range(R)]
+    d_pre = vx.DeviceBuffer(8)
+    d_res = vx.DeviceBuffer(2)
+    ref_nonces = []
+    t = time.perf_counter()
+    for i in range(R):
+        vx.lib.vx_memcpy_h2d(d_pre.ptr, prefixes[i].ctypes.data, 32)
+        vx.lib.vx_memcpy_h2d(d_res.ptr, np.array([~np.uint64(0)], np.uint64).ctypes.data, 8)
+        off = 0
+        while True:
+            vx.lib.cuda_grind_pow(d_pre.ptr, d_res.ptr, POW_BITS, off, 1 << 24)
+            vx.synchronize()
+            lo, hi = d_res.to_host(2)
+            n = int(lo) | (int(hi) << 32)
+            if n != (1 << 64) - 1:
+                break
+            off += 1 << 24
+        ref_nonces.append(n)
+    t_ref = time.perf_counter() - t
+    same_as_device = ref_nonces == nonces[:R]
+    ref_tried = sum(n + 1 for n in ref_nonces)
+    log(f"grind, reference host loop (2^24-nonce batches): {t_ref / R * 1e3:.3f} ms each, "
+        f"{ref_tried / t_ref / 1e9:.2f} G nonces/s, same nonces: {same_as_device}")
     out = {"workload": f"{K} Blake2s PoW grinds at pow_bits {POW_BITS} (channel states splitmix64 seed 7), "
                        "prefix + search + nonce on the device, one call each",
            "ms_per_grind": round(best / K * 1e3, 4), "nonces_per_grind_mean": round(tried / K),
@@ -706,7 +732,12 @@ def bench_grind(vx, args) -> dict:
                         "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s (int32 lane-ops)",
                         "frac": round(achieved / VALU_PEAK_TOPS, 4), "ops_per_nonce": OPS_PER_NONCE,
                         "peak_measured": round(MEASURED.get("valu_peak", 0.0), 3)},
-           "parity_vs_oracle": {"all_valid": valid, "first_two_smallest": first}}
+           "parity_vs_oracle": {"all_valid": valid, "first_two_smallest": first},
+           "reference_host_loop": {"what": "cuda_grind_pow in batches of 2^24 with a host synchronisation and "
+                                           "result read per batch, as grind_ops.rs:70-115, first 32 states",
+                                   "ms_per_grind": round(t_ref / R * 1e3, 4),
+                                   "g_nonces_per_s": round(ref_tried / t_ref / 1e9, 3),
+                                   "same_nonces_as_device_step": same_as_device}}
     if not args.no_cpu:
         threads = int(O.lib().orc_get_threads())
         sample = 1 << 27
