@@ -250,3 +250,23 @@ def test_fused_quotient_calls_in_flight_on_two_streams(vx):
     dq.close()
     for sp in streams:
         vx.lib.vx_stream_destroy(C.c_void_p(sp))
+
+
+@pytest.mark.parametrize("n_rows", [2048, 1 << 14])
+def test_accumulate_numerators_misaligned_columns(vx, n_rows):
+    """Columns whose device pointers are 4- but not 16-byte aligned (read word by
+    word), next to aligned ones, in the tiled and the per-thread forms."""
+    rng = np.random.default_rng(n_rows)
+    cols = [rnd(rng, n_rows) for _ in range(5)]
+    bufs = [vx.DeviceBuffer.from_host(np.concatenate([np.zeros(i % 2, np.uint32), c])) for i, c in enumerate(cols)]
+    addrs = np.array([b.addr + 4 * (i % 2) for i, b in enumerate(bufs)], dtype=np.uint64)
+    dp = vx.DeviceBuffer(2 * len(addrs))
+    vx.lib.vx_memcpy_h2d(dp.ptr, addrs.ctypes.data, 8 * len(addrs))
+    idx = np.array([4, 1, 0, 3, 2, 1], np.uint32)
+    b, c = rnd(rng, 4 * len(idx)), rnd(rng, 4 * len(idx))
+    di, db, dc = vx.DeviceBuffer.from_host(idx), vx.DeviceBuffer.from_host(b), vx.DeviceBuffer.from_host(c)
+    outs = [vx.DeviceBuffer(n_rows) for _ in range(4)]
+    vx.lib.cuda_accumulate_numerators(dp.ptr, di.ptr, db.ptr, dc.ptr, len(idx), n_rows, *[o.ptr for o in outs])
+    vx.synchronize()
+    vx.check("cuda_accumulate_numerators")
+    assert np.array_equal(np.stack([o.to_host() for o in outs]), O.accumulate_numerators(cols, idx, b, c, n_rows))
