@@ -75,9 +75,12 @@ struct NttPassArgs {
     const uint32_t* src;
     uint32_t* const* cols;        // batched: column blockIdx.y is written here
     const uint32_t* const* srcs;  // batched: and read from here (null: in place)
-    const uint32_t* line;
-    const uint32_t* circle;
-    uint32_t offs[32];
+    // twiddle table of each layer of this pass, row bit rb = 0 .. m-1 (global
+    // stride bit s + rb): the line layer s + rb, or at stride bit 0 the circle
+    // table (the stwo variant derives its circle twiddles from the first line
+    // layer, which tw[0] then points at). Set on the host, so the kernel reads
+    // one pointer per layer instead of looking offsets up.
+    const uint32_t* tw[NTT_LOW_MAX];
     uint64_t src_n;  // words of src that are read; the rest is zero
     uint32_t s;      // lowest global stride bit of this pass
     uint32_t k;      // log2 of the transform size
@@ -90,12 +93,11 @@ struct NttPassArgs {
 // accesses below free of bank conflicts.
 __device__ __forceinline__ uint32_t lpad(uint32_t e) { return e + (e >> 3); }
 
-// Twiddle h of the layer with stride bit b.
+// Twiddle h of a layer whose table is T; circle0: the layer is stride bit 0.
 template <bool STWO>
-__device__ __forceinline__ uint32_t ntt_twiddle(const NttPassArgs& a, uint32_t b, uint32_t h) {
-    if (b) return a.line[a.offs[b - 1] + h];
-    if (!STWO) return a.circle[h];
-    const uint32_t* pr = a.line + 2 * (h >> 2);
+__device__ __forceinline__ uint32_t ntt_twiddle(const uint32_t* T, bool circle0, uint32_t h) {
+    if (!STWO || !circle0) return T[h];
+    const uint32_t* pr = T + 2 * (h >> 2);
     const uint32_t x = pr[0], y = pr[1];
     switch (h & 3) {
         case 0: return y;
@@ -146,13 +148,14 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
             const int t = FWD ? (R - 1 - tt) : tt;  // layer inside the group
             if (FWD && R == 3 && t >= 1 && zero_top) continue;
             const uint32_t rb = J + t;                 // row bit of the layer
-            const uint32_t b = a.s + rb;               // global stride bit
+            const uint32_t* T = a.tw[rb];
+            const bool circle0 = STWO && a.s + rb == 0;
             // h(d) = (hi << (M - rb - 1)) + (row >> (rb + 1)) = hb + (d >> (t + 1))
             const uint32_t hb = (uint32_t)(hi << (M - rb - 1)) + (row_hi << (R - t - 1));
 #pragma unroll
             for (int d = 0; d < (1 << R); d++) {
                 if (d & (1 << t)) continue;
-                const uint32_t w = ntt_twiddle<STWO>(a, b, hb + ((uint32_t)d >> (t + 1)));
+                const uint32_t w = ntt_twiddle<STWO>(T, circle0, hb + ((uint32_t)d >> (t + 1)));
                 uint32_t& x0 = v[d];
                 uint32_t& x1 = v[d | (1 << t)];
                 if (FWD) {
@@ -325,9 +328,10 @@ static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t 
         a.dst = dst;
         a.src = i == 0 ? src : dst;
         a.src_n = i == 0 ? src_n : (1ull << k);
-        a.line = t.line;
-        a.circle = t.circle;
-        for (int l = 0; l < 32; l++) a.offs[l] = t.offs[l];
+        for (uint32_t rb = 0; rb < ps[pi].m && rb < NTT_LOW_MAX; rb++) {
+            const uint32_t b = ps[pi].s + rb;
+            a.tw[rb] = b ? t.line + t.offs[b - 1] : (t.stwo ? t.line : t.circle);
+        }
         a.s = ps[pi].s;
         a.k = k;
         a.vec = vec ? 1u : 0u;
