@@ -250,17 +250,25 @@ __global__ __launch_bounds__(DEEP_THREADS) void accumulate_lift_kernel(uint32_t*
 // (column, point) c[4].
 constexpr uint32_t DEEP_MAX_POINTS = 4;
 
-template <uint32_t NP, uint32_t R>
-__device__ __forceinline__ void fused_columns(const uint32_t* __restrict__ cm, const uint32_t* __restrict__ cf,
-                                              uint32_t c, uint32_t n, const uint32_t row[R],
-                                              uint64_t acc[NP][R][4]) {
-    uint32_t f[3][R];
-    for (uint32_t g = 0; g < n; g++) {
+// Values of columns c .. c + G - 1 at the rows of this thread (lifted indices).
+template <uint32_t G, uint32_t R>
+__device__ __forceinline__ void fused_load(const uint32_t* __restrict__ cm, uint32_t c, const uint32_t row[R],
+                                           uint32_t f[3][R]) {
+#pragma unroll
+    for (uint32_t g = 0; g < G; g++) {
         const uint32_t* ptr = (const uint32_t*)(uintptr_t)(cm[4 * (c + g)] | ((uint64_t)cm[4 * (c + g) + 1] << 32));
         const uint32_t sh = cm[4 * (c + g) + 2];
 #pragma unroll
         for (uint32_t r = 0; r < R; r++) f[g][r] = ptr[((row[r] >> sh) << 1) | (row[r] & 1)];
     }
+}
+
+// acc += c_pc * f for columns c .. c + G - 1 (G <= 3): three products on top of
+// a partially reduced value (< 2^62 + 4) stay below 2^64, then bits 62..63 fold
+// onto bit 0.
+template <uint32_t G, uint32_t NP, uint32_t R>
+__device__ __forceinline__ void fused_accumulate(const uint32_t* __restrict__ cf, uint32_t c, const uint32_t f[3][R],
+                                                 uint64_t acc[NP][R][4]) {
     constexpr uint64_t M62 = (1ull << 62) - 1;
 #pragma unroll
     for (uint32_t p = 0; p < NP; p++)
@@ -269,7 +277,8 @@ __device__ __forceinline__ void fused_columns(const uint32_t* __restrict__ cm, c
 #pragma unroll
             for (uint32_t r = 0; r < R; r++) {
                 uint64_t a = acc[p][r][k];
-                for (uint32_t g = 0; g < n; g++) a = (uint64_t)cf[((c + g) * NP + p) * 4 + k] * f[g][r] + a;
+#pragma unroll
+                for (uint32_t g = 0; g < G; g++) a = (uint64_t)cf[((c + g) * NP + p) * 4 + k] * f[g][r] + a;
                 acc[p][r][k] = (a & M62) + (a >> 62);
             }
 }
@@ -298,10 +307,33 @@ __global__ __launch_bounds__(DEEP_THREADS) void deep_fused_kernel(const uint32_t
         row[r] = live[r] ? x : n_rows - 1;
     }
     uint64_t acc[NP][R][4] = {};
-    // three products on top of a partially reduced value (< 2^62 + 4) stay below 2^64
-    uint32_t c = 0;
-    for (; c + 3 <= n_cols; c += 3) fused_columns<NP, R>(cm, cf, c, 3, row, acc);
-    if (c < n_cols) fused_columns<NP, R>(cm, cf, c, n_cols - c, row, acc);
+    // The one or two columns past a multiple of three first (while the first
+    // whole group's loads are in flight), then whole groups of three, the next
+    // group's loads in flight while a group is summed.
+    const uint32_t c0 = n_cols % 3;
+    uint32_t fa[3][R], fb[3][R];
+    if (c0 < n_cols) fused_load<3, R>(cm, c0, row, fa);
+    if (c0 == 1) {
+        fused_load<1, R>(cm, 0, row, fb);
+        fused_accumulate<1, NP, R>(cf, 0, fb, acc);
+    } else if (c0 == 2) {
+        fused_load<2, R>(cm, 0, row, fb);
+        fused_accumulate<2, NP, R>(cf, 0, fb, acc);
+    }
+    if (c0 < n_cols) {
+        for (uint32_t c = c0;;) {
+            const bool more_b = c + 3 < n_cols;
+            if (more_b) fused_load<3, R>(cm, c + 3, row, fb);
+            fused_accumulate<3, NP, R>(cf, c, fa, acc);
+            c += 3;
+            if (!more_b) break;
+            const bool more_a = c + 3 < n_cols;
+            if (more_a) fused_load<3, R>(cm, c + 3, row, fa);
+            fused_accumulate<3, NP, R>(cf, c, fb, acc);
+            c += 3;
+            if (!more_a) break;
+        }
+    }
     uint32_t dx[R], dy[R];
 #pragma unroll
     for (uint32_t r = 0; r < R; r++) {
