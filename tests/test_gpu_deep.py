@@ -177,7 +177,11 @@ def oracle_pipeline(cols, logs, points, lc, L, dxs, dys):
 
 @pytest.mark.parametrize("L,logs,n_points", [(1, [1], 1), (4, [2, 4, 3, 1], 2), (8, [8] * 5 + [5, 6], 3),
                                              (10, [3, 10, 7, 10, 9], 4), (14, [14, 12, 12, 9], 2),
-                                             (20, [16] * 2 + [18] * 3 + [20] * 4, 2)])
+                                             (20, [16] * 2 + [18] * 3 + [20] * 4, 2),
+                                             # column counts 2, 3, 6 and 8: no whole group of three, one
+                                             # group and no remainder, two groups, two groups and two left
+                                             (5, [5, 4], 1), (6, [6, 6, 2], 4), (12, [12, 11, 10, 12, 9, 12], 3),
+                                             (9, [9] * 8, 1)])
 def test_fused_quotient_matches_reference_pipeline(vx, L, logs, n_points):
     rng = np.random.default_rng(L * 13 + n_points)
     cols = [rnd(rng, 1 << l) for l in logs]
