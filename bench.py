@@ -536,14 +536,8 @@ def bench_deep(vx, args) -> dict:
     dq = vx.DeepQuotient(len(sizes), 2)
     points = np.stack([np.concatenate(p) for p in pts])
 
-    def fused(timed):
-        if timed:
-            e[0].record()
+    def fused():
         dq.quotient([c.addr for c in dcols], sizes, points, lc, L, fx.addr, fy.addr, [o.addr for o in fused_out])
-        if timed:
-            e[1].record()
-            return e[0].elapsed_ms(e[1])
-        return None
     d_np = [dev_ptrs([numer[j][k] for j in range(len(batches))]) for k in range(4)]
     out = [vx.DeviceBuffer(1 << L) for _ in range(4)]
     e = [vx.Event() for _ in range(3)]
@@ -576,12 +570,12 @@ def bench_deep(vx, args) -> dict:
     t_num /= K
     t_comb /= K
     for _ in range(3):
-        fused(False)
+        fused()
     # K calls back to back between two events: each call's host-side metadata
     # (two staging slots) is prepared while the previous call's kernel runs
     e[0].record()
     for _ in range(K):
-        fused(False)
+        fused()
     e[1].record()
     t_fused = e[0].elapsed_ms(e[1]) / K
     rows = 1 << L
