@@ -661,7 +661,8 @@ struct Cache {
         log_n = ln;
         log_e = ln + 2;  // BLOWUP_BITS = 2 (prover.rs:86)
         build_ntt_twiddles(half_coset(ln), trace_tw, true);
-        build_ntt_twiddles(half_coset(log_e), eval_tw, false);
+        // prover.rs:626-630: the whole group when log_e = 31 (no half coset exists)
+        build_ntt_twiddles(log_e <= 30 ? half_coset(log_e) : subgroup(log_e), eval_tw, false);
         // fri.rs:143-160 (precompute_fri_twiddles(log_e, stop))
         fri_tw.clear();
         fri_tw.emplace_back();
@@ -756,6 +757,164 @@ void prove(uint32_t a, uint32_t b, const Cache& C, uint32_t n_queries, uint32_t 
         decommit(trees[j], lc, 4, fi, pf.fri.back());
         for (auto& v : fi) v /= 2;
     }
+}
+
+// ---------------------------------------------------------------- proof summary, memory-lean
+// The commitments of prove() without the trees: the same trace/quotient roots,
+// FRI roots, last layer and query indices (prover.rs:1721-1927), computed with
+// root-only Merkle commitments over leaves produced on the fly (the quotient
+// from the LDE, as the reference's fused path does, prover.rs:696-733) and each
+// FRI layer freed once the next exists. Twiddles are built lean as well. It
+// serves traces whose full trees do not fit host memory (2^29 rows: a 2^31-leaf
+// tree alone is 128 GiB); at sizes where prove() runs the two must agree.
+
+// Twiddles of build_ntt_twiddles without the all-points arrays: circle[i] =
+// y(pts[i]) for i < n/2; line layer 0 holds x(pts[2i]); every later layer is
+// computed from the one before (circle.rs:394-425: X_{l+1}[i] = 2 X_l[2i]^2 - 1,
+// and layer l keeps the even entries X_l[2i]).
+void build_ntt_twiddles_lean(const Coset& c, NttTw& tw, bool inverses) {
+    uint32_t log_n = c.log_size;
+    uint64_t n = 1ull << log_n;
+    uint32_t n_line = log_n > 0 ? log_n - 1 : 0;
+    tw.log_n = log_n;
+    tw.circle.assign(n / 2, 0);
+    tw.line.assign(n > 0 ? n - 1 : 0, 0);
+    tw.offsets = offsets_for(log_n);
+    tw.sizes.clear();
+    for (uint32_t l = 0; l < n_line; l++) tw.sizes.push_back((uint32_t)(n >> (l + 1)));
+    const uint64_t half = n / 2;
+    const uint32_t nt = n_threads();
+    const uint64_t chunk = (half + nt - 1) / nt;
+    const Pt step2 = pmul(c.step, c.step);
+    parallel_for(nt, [&](uint64_t tb, uint64_t te) {
+        for (uint64_t t = tb; t < te; t++) {
+            uint64_t b = t * chunk, e = b + chunk < half ? b + chunk : half;
+            if (b >= e) continue;
+            Pt p = c.at(b), q = c.at(2 * b);
+            for (uint64_t i = b; i < e; i++) {
+                tw.circle[i] = p.y;
+                if (n_line) tw.line[i] = q.x;
+                p = pmul(p, c.step);
+                q = pmul(q, step2);
+            }
+        }
+    });
+    for (uint32_t l = 1; l < n_line; l++) {
+        const uint32_t* prev = tw.line.data() + tw.offsets[l - 1];
+        uint32_t* cur = tw.line.data() + tw.offsets[l];
+        parallel_for(n >> (l + 1), [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; i++) {
+                uint32_t x = prev[2 * i];
+                uint32_t x2 = mul(x, x);
+                cur[i] = sub(add(x2, x2), 1);
+            }
+        });
+    }
+    if (inverses) {
+        tw.iline.assign(tw.line.size(), 0);
+        tw.icircle.assign(tw.circle.size(), 0);
+        if (!tw.line.empty()) batch_inverse(tw.line.data(), tw.iline.data(), tw.line.size());
+        if (!tw.circle.empty()) batch_inverse(tw.circle.data(), tw.icircle.data(), tw.circle.size());
+    }
+}
+
+// Root of the complete binary tree over n leaves (n a power of two), leaf(i, h)
+// writing leaf i's hash: subtrees of up to 2^14 leaves are reduced on their own
+// threads, then the tree over their roots.
+template <class F>
+void stream_root(uint64_t n, F&& leaf, uint32_t root[8]) {
+    const uint64_t S = n < (1ull << 14) ? n : (1ull << 14);
+    const uint64_t ns = n / S;
+    std::vector<uint32_t> tops(8 * ns);
+    parallel_for(ns, [&](uint64_t b, uint64_t e) {
+        std::vector<uint32_t> buf(8 * S);
+        for (uint64_t s = b; s < e; s++) {
+            for (uint64_t i = 0; i < S; i++) leaf(s * S + i, buf.data() + 8 * i);
+            for (uint64_t m = S; m > 1; m /= 2)
+                for (uint64_t i = 0; i < m / 2; i++) node_hash(&buf[16 * i], &buf[16 * i + 8], &buf[8 * i]);
+            memcpy(&tops[8 * s], buf.data(), 32);
+        }
+    });
+    std::vector<uint32_t> nx;
+    while (tops.size() > 8) {
+        nx.assign(tops.size() / 2, 0);
+        node_layer(tops.data(), tops.size() / 16, nx.data());
+        tops.swap(nx);
+    }
+    memcpy(root, tops.data(), 32);
+}
+inline void soa4_leaf(const Soa4& L, uint64_t i, uint32_t h[8]) {
+    uint32_t v[4] = {L.c[0][i], L.c[1][i], L.c[2][i], L.c[3][i]};
+    leaf_hash_words(v, 4, h);
+}
+
+void prove_summary(uint32_t a, uint32_t b, uint32_t log_n, uint32_t n_queries, uint32_t log_last, Proof& pf) {
+    const uint32_t log_e = log_n + 2;
+    const uint64_t n = 1ull << log_n, E = 1ull << log_e;
+    pf.log_n = log_n;
+    pf.a = a;
+    pf.b = b;
+    std::vector<uint32_t> eval(E, 0);
+    fib_trace(a, b, log_n, eval.data());
+    {
+        NttTw ttw;
+        build_ntt_twiddles_lean(half_coset(log_n), ttw, true);
+        ntt_interpolate(eval.data(), ttw.iline.data(), ttw.icircle.data(), ttw.offsets.data(), log_n);
+    }
+    {
+        NttTw etw;
+        build_ntt_twiddles_lean(log_e <= 30 ? half_coset(log_e) : subgroup(log_e), etw, false);
+        ntt_evaluate(eval.data(), etw.line.data(), etw.circle.data(), etw.offsets.data(), log_e);
+    }
+    const uint32_t* ev = eval.data();
+    stream_root(E, [&](uint64_t i, uint32_t* h) { leaf_hash_words(ev + i, 1, h); }, pf.trace_root);
+    Channel ch;
+    ch.mix_digest(pf.trace_root);
+    const QM31 alpha = ch.draw_felt();
+    auto q_at = [&](uint64_t i) {
+        return qmul_m31(alpha, sub(ev[(i + 2) % E], add(ev[(i + 1) % E], ev[i])));
+    };
+    stream_root(E, [&](uint64_t i, uint32_t* h) {
+        uint32_t v[4];
+        qto(q_at(i), v);
+        leaf_hash_words(v, 4, h);
+    }, pf.quotient_root);
+    ch.mix_digest(pf.quotient_root);
+    const QM31 fa = ch.draw_felt();
+    Soa4 cur;
+    {
+        std::vector<uint32_t> tw;
+        fold_twiddles(log_e <= 30 ? half_coset(log_e) : subgroup(log_e), true, tw);
+        cur.resize(E / 2);
+        parallel_for(E / 2, [&](uint64_t b0, uint64_t e0) {
+            for (uint64_t i = b0; i < e0; i++) cur.set(i, fold_pair(q_at(2 * i), q_at(2 * i + 1), fa, tw[i]));
+        });
+    }
+    std::vector<uint32_t>().swap(eval);
+    pf.fri_roots.clear();
+    auto commit = [&]() {
+        std::array<uint32_t, 8> r;
+        stream_root(cur.size(), [&](uint64_t i, uint32_t* h) { soa4_leaf(cur, i, h); }, r.data());
+        pf.fri_roots.push_back(r);
+        ch.mix_digest(r.data());
+    };
+    commit();
+    for (uint32_t ls = log_e - 1; ls > log_last; ls--) {
+        const QM31 al = ch.draw_felt();
+        std::vector<uint32_t> tw;
+        fold_twiddles(half_odds(ls), false, tw);
+        Soa4 nx;
+        fold_line(cur, tw, al, nx);
+        cur = std::move(nx);
+        commit();
+    }
+    pf.last_layer.clear();
+    for (uint64_t i = 0; i < cur.size(); i++)
+        for (int c = 0; c < 4; c++) pf.last_layer.push_back(cur.c[c][i]);
+    ch.mix_felts(pf.last_layer.data(), cur.size());
+    pf.queries.clear();
+    for (uint32_t i = 0; i < n_queries; i++) pf.queries.push_back(ch.draw_number(E));
+    (void)n;
 }
 
 // stark_cli.rs:907-975 (KRAK v1)
@@ -1398,6 +1557,29 @@ uint64_t orc_prove_fibonacci(uint32_t a, uint32_t b, uint32_t log_n, uint32_t n_
     return bytes.size();
 }
 
+int orc_prove_summary(uint32_t a, uint32_t b, uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer,
+                      uint32_t* roots, uint32_t* last_layer, uint64_t* queries) {
+    if (log_n < 1 || log_n + 2 > 31 || log_last_layer >= log_n + 2 || !roots || !last_layer || !queries) return -1;
+    Proof pf;
+    prove_summary(a % P, b % P, log_n, n_queries, log_last_layer, pf);
+    memcpy(roots, pf.trace_root, 32);
+    memcpy(roots + 8, pf.quotient_root, 32);
+    for (size_t j = 0; j < pf.fri_roots.size(); j++) memcpy(roots + 16 + 8 * j, pf.fri_roots[j].data(), 32);
+    memcpy(last_layer, pf.last_layer.data(), 4 * pf.last_layer.size());
+    for (uint32_t i = 0; i < n_queries; i++) queries[i] = pf.queries[i];
+    return 0;
+}
+
+void orc_ntt_twiddles_lean(int kind, uint32_t log_n, uint32_t* line, uint32_t* circle, uint32_t* iline,
+                           uint32_t* icircle) {
+    NttTw tw;
+    build_ntt_twiddles_lean(coset_of(kind, log_n), tw, iline || icircle);
+    if (line) memcpy(line, tw.line.data(), 4 * tw.line.size());
+    if (circle) memcpy(circle, tw.circle.data(), 4 * tw.circle.size());
+    if (iline) memcpy(iline, tw.iline.data(), 4 * tw.iline.size());
+    if (icircle) memcpy(icircle, tw.icircle.data(), 4 * tw.icircle.size());
+}
+
 double orc_time_prove_fibonacci(uint32_t a, uint32_t b, uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer,
                                 uint32_t reps) {
     if (log_n < 1 || log_n + 2 > 30) return -1.0;
@@ -1410,6 +1592,23 @@ double orc_time_prove_fibonacci(uint32_t a, uint32_t b, uint32_t log_n, uint32_t
     for (uint32_t r = 0; r < reps; r++) prove(a, b, C, n_queries, log_last_layer, pf, trace.data());
     auto t1 = std::chrono::steady_clock::now();
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+int orc_time_prove_each(uint32_t a, uint32_t b, uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer,
+                        uint32_t warmup, uint32_t reps, double* secs) {
+    if (log_n < 1 || log_n + 2 > 30 || !secs) return -1;
+    Cache C;
+    C.build(log_n, log_last_layer);
+    std::vector<uint32_t> trace(1ull << log_n);
+    fib_trace(a, b, log_n, trace.data());
+    Proof pf;
+    for (uint32_t r = 0; r < warmup + reps; r++) {
+        auto t0 = std::chrono::steady_clock::now();
+        prove(a, b, C, n_queries, log_last_layer, pf, trace.data());
+        auto t1 = std::chrono::steady_clock::now();
+        if (r >= warmup) secs[r - warmup] = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return 0;
 }
 
 int orc_verify_krak(const uint8_t* proof, uint64_t len, uint32_t n_queries, uint32_t log_last_layer) {

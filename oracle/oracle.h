@@ -103,10 +103,26 @@ uint64_t orc_channel_draw_number(orc_channel* ch, uint64_t bound);
  * [16] number of FRI layers. */
 uint64_t orc_prove_fibonacci(uint32_t a, uint32_t b, uint32_t log_n, uint32_t n_queries,
                              uint32_t log_last_layer, uint8_t* out, uint64_t cap, uint32_t* stats);
+/* Commitments of the same proof without its trees or decommitments, in
+ * bounded host memory (root-only Merkle commitments, quotient recomputed from
+ * the LDE, FRI layers freed as the next is made): roots = trace root,
+ * quotient root, then one root per FRI layer (8 words each, log_n + 2 -
+ * log_last_layer layers); last_layer = 4 << log_last_layer words; queries =
+ * n_queries indices. Up to log_n = 29 (the 2^31-point domain is subgroup(31)). */
+int orc_prove_summary(uint32_t a, uint32_t b, uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer,
+                      uint32_t* roots, uint32_t* last_layer, uint64_t* queries);
+/* NTT twiddles built layer from layer (no all-points arrays); same words as
+ * orc_ntt_twiddles. */
+void orc_ntt_twiddles_lean(int kind, uint32_t log_n, uint32_t* line, uint32_t* circle, uint32_t* iline,
+                           uint32_t* icircle);
 /* Phase timing variant used by the CPU baseline: same work as orc_prove_fibonacci
  * from an already generated trace with cached twiddles; returns seconds spent. */
 double orc_time_prove_fibonacci(uint32_t a, uint32_t b, uint32_t log_n, uint32_t n_queries,
                                 uint32_t log_last_layer, uint32_t reps);
+/* Each of `reps` proofs timed on its own (steady_clock) after `warmup` untimed
+ * ones, twiddles built once beforehand; secs gets reps entries. */
+int orc_time_prove_each(uint32_t a, uint32_t b, uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer,
+                        uint32_t warmup, uint32_t reps, double* secs);
 /* Verifier (src/verifier.rs:39-310), parameterised by (n_queries, log_last_layer).
  * Returns 0 on accept, a positive reason code on reject. */
 int orc_verify_krak(const uint8_t* proof, uint64_t len, uint32_t n_queries, uint32_t log_last_layer);

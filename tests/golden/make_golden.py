@@ -11,12 +11,20 @@ reference-protocol and configured runs of the plain CPU workload (one
 Fibonacci column, 2^16 rows, a/b from splitmix64 seed 1), and the plain GPU
 workload (2^24 rows, a/b from splitmix64 seed 2, 100 queries, last layer 2^4).
 
+tests/golden/large.json holds commitment summaries (trace and quotient roots,
+FRI roots, last layer, query indices) of the largest proofs, 2^26, 2^28 and
+2^29 rows (the last one evaluated on the whole circle group, subgroup(31)),
+made by the oracle's memory-lean summary path (orc_prove_summary, pinned
+against full proofs by tests/test_oracle_large.py): full trees of those sizes
+do not fit host memory. a/b from splitmix64 seeds 26, 28, 29; 80 queries,
+last layer 2^3.
+
 tests/golden/wide.json holds the batched wide commitment of 34 uniform M31
 columns (4 x 2^14, 10 x 2^18, 20 x 2^20 from splitmix64 seed 3, blowup 4,
 last layer 2^4): lifted root, FRI roots, last layer, beta and the SHA-256 of
 every LDE column.
 
-    python tests/golden/make_golden.py [--skip-large] [--wide-only]
+    python tests/golden/make_golden.py [--skip-large] [--wide-only] [--large-only]
 """
 
 from __future__ import annotations
@@ -79,11 +87,29 @@ def write_wide():
         json.dump(cases, f, separators=(",", ":"))
 
 
+def summary_case(seed: int, log_n: int, nq: int, last: int):
+    t = time.time()
+    a, b = seeded_ab(seed)
+    s = oracle_lib.prove_summary(a, b, log_n, nq, last)
+    print(f"summary log_n={log_n}: {time.time() - t:.1f}s", flush=True)
+    return {"label": f"summary_log{log_n}", "a": a, "b": b, "log_n": log_n, "n_queries": nq, "log_last": last, **s}
+
+
+def write_large():
+    cases = [summary_case(n, n, 80, 3) for n in (26, 28, 29)]
+    with open(os.path.join(HERE, "large.json"), "w") as f:
+        json.dump(cases, f, separators=(",", ":"))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-large", action="store_true")
     ap.add_argument("--wide-only", action="store_true")
+    ap.add_argument("--large-only", action="store_true")
     args = ap.parse_args()
+    if args.large_only:
+        write_large()
+        return
     write_wide()
     if args.wide_only:
         return

@@ -14,6 +14,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 
 u32, u64, i32, vp = C.c_uint32, C.c_uint64, C.c_int32, C.c_void_p
+P = (1 << 31) - 1
 P32 = C.POINTER(C.c_uint32)
 
 
@@ -36,6 +37,8 @@ _SIGS = {
     "orc_circle_mul": (None, [P32, P32, P32]),
     "orc_circle_repeated_double": (None, [P32, u32, P32]),
     "orc_ntt_twiddles": (None, [i32, u32, vp, vp, vp, vp, vp, vp]),
+    "orc_ntt_twiddles_lean": (None, [i32, u32, vp, vp, vp, vp]),
+    "orc_prove_summary": (C.c_int, [u32, u32, u32, u32, u32, vp, vp, vp]),
     "orc_batch_inverse": (None, [vp, vp, u64]),
     "orc_ntt_layer": (None, [vp, vp, u32, u64, i32]),
     "orc_ntt_evaluate": (None, [vp, vp, vp, u32]),
@@ -61,6 +64,7 @@ _SIGS = {
     "orc_channel_draw_number": (u64, [C.POINTER(ChannelStruct), u64]),
     "orc_prove_fibonacci": (u64, [u32, u32, u32, u32, u32, vp, u64, P32]),
     "orc_time_prove_fibonacci": (C.c_double, [u32, u32, u32, u32, u32, u32]),
+    "orc_time_prove_each": (C.c_int, [u32, u32, u32, u32, u32, u32, u32, vp]),
     "orc_verify_krak": (i32, [vp, u64, u32, u32]),
     "orc_lifted_row": (u32, [u32, u32, u32]),
     "orc_stwo_twiddles": (None, [u32, vp, vp]),
@@ -116,6 +120,33 @@ def ptr_array(cols):
     cols = [arr(c) for c in cols]
     ptrs = (vp * len(cols))(*[c.ctypes.data for c in cols])
     return ptrs, cols
+
+
+def ntt_twiddles_lean(kind: int, log_n: int):
+    """(line, circle, iline, icircle) of the layer-from-layer builder."""
+    n = 1 << log_n
+    line = np.zeros(max(n - 1, 1), np.uint32)
+    circle = np.zeros(max(n // 2, 1), np.uint32)
+    iline = np.zeros_like(line)
+    icircle = np.zeros_like(circle)
+    lib().orc_ntt_twiddles_lean(kind, log_n, line.ctypes.data, circle.ctypes.data, iline.ctypes.data,
+                                icircle.ctypes.data)
+    return line, circle, iline, icircle
+
+
+def prove_summary(a: int, b: int, log_n: int, n_queries: int = 80, log_last: int = 3) -> dict:
+    """Commitments of prove(a, b, log_n) without trees (bounded host memory):
+    trace/quotient roots, FRI roots, last layer (4 words per value), queries."""
+    J = log_n + 2 - log_last
+    roots = np.zeros(8 * (2 + J), np.uint32)
+    last = np.zeros(4 << log_last, np.uint32)
+    qs = np.zeros(n_queries, np.uint64)
+    rc = lib().orc_prove_summary(a, b, log_n, n_queries, log_last, roots.ctypes.data, last.ctypes.data,
+                                 qs.ctypes.data)
+    assert rc == 0, "orc_prove_summary rejected its parameters"
+    r = roots.reshape(-1, 8)
+    return {"trace_root": r[0].tolist(), "quotient_root": r[1].tolist(), "fri_roots": r[2:].tolist(),
+            "last_layer": last.reshape(-1, 4).tolist(), "queries": [int(q) for q in qs]}
 
 
 def ntt_twiddles(kind: int, log_n: int):

@@ -151,6 +151,45 @@ def test_twiddle_entry_points(vx, k):
     vx.check()
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1000, 64 * 4096 + 3])
+def test_batch_inverse_zero_chunks(vx, n):
+    """cuda_batch_inverse_m31 against the oracle with zeros at chunk starts,
+    inside chunks and filling chunks: a 64-word chunk led by a zero maps to
+    zero entirely (batch_inverse.cu:20-24, 40), every other zero to zero."""
+    vals = rnd(900 + n, n)
+    rng = np.random.default_rng(n)
+    vals[rng.integers(0, n, max(1, n // 50))] = 0
+    vals[::64][::3] = 0  # every third chunk starts with a zero
+    if n > 200:
+        vals[64:128] = 0
+    dv, di = vx.DeviceBuffer.from_host(vals), vx.DeviceBuffer(n)
+    vx.lib.cuda_batch_inverse_m31(dv.ptr, di.ptr, n)
+    vx.synchronize()
+    e = np.zeros(n, np.uint32)
+    O.lib().orc_batch_inverse(vals.ctypes.data, e.ctypes.data, n)
+    assert np.array_equal(di.to_host(), e)
+    vx.check()
+
+
+def test_subgroup_fold_twiddles(vx):
+    """Circle-fold twiddles on subgroup(k), the fold domain of a 2^31-point
+    evaluation: the identity's y = 0 leads chunk 0, which the batch inverse
+    clears (the oracle and its pins give the same words)."""
+    k = 14
+    n = 1 << k
+    cos = (C.c_uint32 * 4)()
+    O.lib().orc_coset(0, k, cos)
+    src = vx.DeviceBuffer(n // 2)
+    vx.lib.cuda_compute_fold_twiddle_sources(cos[0], cos[1], cos[2], cos[3], src.ptr, n, k, 1)
+    inv = vx.DeviceBuffer(n // 2)
+    vx.lib.cuda_batch_inverse_m31(src.ptr, inv.ptr, n // 2)
+    vx.synchronize()
+    got = inv.to_host()
+    assert np.array_equal(got, O.fold_twiddles(0, k, True))
+    assert np.all(got[:64] == 0) and np.all(got[64:] != 0)
+    vx.check()
+
+
 @pytest.mark.parametrize("k", [3, 9, 16])
 def test_fold_and_quotient_entry_points(vx, k):
     n = 1 << k
@@ -189,6 +228,20 @@ def test_fold_and_quotient_entry_points(vx, k):
     vx.lib.cuda_zero_pad(dev.ptr, pad.ptr, n, 4 * n)
     vx.synchronize()
     assert np.array_equal(pad.to_host(), np.concatenate([ev, np.zeros(3 * n, np.uint32)]))
+    vx.check()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_quotient_wraps_tiny_domains(vx, n):
+    """(i + 1) mod n and (i + 2) mod n for domains shorter than the stencil
+    (constraint.cu:10-36 takes full remainders); checked against the oracle."""
+    ev = rnd(320 + n, n)
+    alpha = list(rnd(330, 4))
+    dev = vx.DeviceBuffer.from_host(ev)
+    qs = [vx.DeviceBuffer(n) for _ in range(4)]
+    vx.lib.cuda_fibonacci_quotient(dev.ptr, *[q.ptr for q in qs], h32(alpha), n)
+    vx.synchronize()
+    assert np.array_equal(np.stack([q.to_host() for q in qs]), O.quotient(ev, alpha))
     vx.check()
 
 

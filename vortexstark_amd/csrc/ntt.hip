@@ -459,11 +459,13 @@ __global__ void circle_twiddles_kernel(CPt initial, CPt step, uint32_t* circle, 
     progression(initial, step, half_n, [&](uint64_t i, CPt p) { circle[i] = p.y; });
 }
 
-// Montgomery batch inverse with the zero conventions of
-// cuda/batch_inverse.cu:8-41 (zeros map to zero, and are skipped in the
-// running product). Thread t of T inverts the BI_K words t + k * T, so each
-// load and store of a wave is contiguous and the prefix products stay in
-// registers; the grouping does not change any output.
+// Montgomery batch inverse with the outputs of cuda/batch_inverse.cu:8-41.
+// That kernel runs one thread per 64-word chunk whose prefix products skip zero
+// words but start from the chunk's first word as it is: zeros map to zero,
+// other words to their inverses, and a chunk whose first word is zero maps
+// entirely to zero (its product is zero). Here thread t of T inverts the BI_K
+// words t + k * T, so each load and store of a wave is contiguous and the
+// prefix products stay in registers, and the chunk rule is applied per word.
 constexpr uint32_t BI_K = 32;
 __global__ __launch_bounds__(128) void batch_inverse_kernel(const uint32_t* in, uint32_t* out, uint64_t n) {
     const uint64_t T = (n + BI_K - 1) / BI_K;
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(128) void batch_inverse_kernel(const uint32_t* in, 
 #pragma unroll
     for (int k = BI_K - 1; k >= 0; k--) {
         const uint64_t i = t + (uint64_t)k * T;
-        if (i < n) out[i] = v[k] ? mmul(iv, pre[k]) : 0;
+        if (i < n) out[i] = (v[k] && in[i & ~63ull]) ? mmul(iv, pre[k]) : 0;
         if (v[k]) iv = mmul(iv, v[k]);
     }
 }
