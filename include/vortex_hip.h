@@ -293,7 +293,10 @@ typedef struct vx_prover vx_prover;
 /* Builds a prover for traces of 2^log_n rows (blowup 4, eval domain 2^(log_n+2)):
  * twiddle tables (the reference's ProverCache, prover.rs:143-189) and all
  * workspace are allocated here once. n_queries / log_last_layer default to the
- * reference protocol (80, 3) when 0 is passed. Returns NULL on failure. */
+ * reference protocol (80, 3) when 0 is passed. 2 <= log_n <= 29; at 29 the
+ * evaluation and first fold domain is the whole circle group subgroup(31)
+ * (prover.rs:626-630, 754-757) and one prover holds about 91 GB. Returns NULL
+ * on failure. */
 vx_prover* vx_prover_create(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer);
 /* Same, with room for `capacity` proofs of this size per launch sequence: one
  * launch of each kernel serves every proof of a batch (each proof owns an
@@ -437,18 +440,29 @@ int32_t vx_server_poll(vx_server* s);
 int32_t vx_server_prewarm(vx_server* s, uint32_t log_n, uint32_t n_provers);
 /* Batches replay captured HIP graphs (one per prover and batch count). */
 int32_t vx_server_set_graphs(vx_server* s, int32_t enable);
+/* Polls until every job has finished, successfully or not; returns 0, or -1
+ * when a job failed since the previous wait (the others still complete; the
+ * first such failure is the thread's recorded error). */
 int32_t vx_server_wait_all(vx_server* s);
+/* Fault injection for failure-isolation tests (no reference counterpart):
+ * proof `col` of a queued job is reported failed by its batch, through the
+ * device-side transcript failure flag a real failure raises (status -3); the
+ * other proofs of that batch and every other job are unaffected. The proof
+ * must not be on the device yet. Returns 0 or -1. */
+int32_t vx_server_inject_fault(vx_server* s, int64_t job, uint32_t col);
 uint64_t vx_server_jobs(const vx_server* s);
 uint64_t vx_server_completed(const vx_server* s);
 /* 1 done, 0 pending, negative: error (-3 transcript, -4 does not fit the budget,
- * -5 result ring full while nothing runs: release finished jobs). */
+ * -5 result ring full while nothing runs: release finished jobs, -6 no prover
+ * of the size class could be created while nothing else held memory). */
 int32_t vx_server_job_status(const vx_server* s, int64_t job);
 double vx_server_job_latency_ms(const vx_server* s, int64_t job);
 uint64_t vx_server_proof_copy(const vx_server* s, int64_t job, uint32_t col, void* out, uint64_t cap);
 void vx_server_release_job(vx_server* s, int64_t job);  /* frees a finished job's ring space */
 /* Replays arrivals in real time (job i at arrival_s[i] seconds, ascending; its
- * n_cols[i] (a, b) pairs follow each other in ab) and returns when all are done;
- * latency_ms (optional, n_jobs entries) gets arrival-to-completion times. */
+ * n_cols[i] (a, b) pairs follow each other in ab) and returns when all are done
+ * (0, or -1 if a job failed or could not be submitted); latency_ms (optional,
+ * n_jobs entries) gets arrival-to-completion times. */
 int32_t vx_server_run_trace(vx_server* s, uint32_t n_jobs, const double* arrival_s, const uint32_t* log_n,
                             const uint32_t* n_cols, const uint32_t* ab, double* latency_ms);
 /* [0] batches, [1] proofs, [2] jobs done, [3] device bytes held, [4] peak, [5] budget */

@@ -120,3 +120,80 @@ def test_prewarm_makes_provers_ahead_of_traffic(vx):
     small = vx.Server(1 << 20, 80, 3)
     assert small.prewarm(12, 2) == 0
     small.close()
+
+
+def test_injected_fault_fails_one_job_alone(vx):
+    """A proof reported failed by the device (the injected fault raises the
+    transcript failure flag of one proof inside a shared batch) fails its job
+    alone: every other job, including the proofs batched with it, completes
+    with the oracle's bytes."""
+    spec = jobs_spec(11, 10, [5, 7], 3)
+    srv = vx.Server(0, 80, 3)
+    ids = [srv.submit(log_n, ab) for log_n, ab in spec]
+    bad = ids[3]
+    srv.inject_fault(bad, len(spec[3][1]) - 1)
+    assert srv.wait_all(raise_on_failure=False) is False
+    for jid, (log_n, ab) in zip(ids, spec):
+        if jid == bad:
+            assert srv.status(jid) == -3
+            with pytest.raises(vx.HipError):
+                srv.proof(jid, 0, log_n)
+            continue
+        assert srv.status(jid) == 1, jid
+        for c, (a, b) in enumerate(ab):
+            assert srv.proof(jid, c, log_n) == O.prove(a, b, log_n, 80, 3), (jid, c)
+    # the failed proof's arena serves the next batch normally
+    again = srv.submit(spec[3][0], spec[3][1])
+    assert srv.wait_all()
+    for c, (a, b) in enumerate(spec[3][1]):
+        assert srv.proof(again, c, spec[3][0]) == O.prove(a, b, spec[3][0], 80, 3)
+    srv.close()
+
+
+def test_class_over_budget_fails_alone(vx):
+    """Admission failure of one size class (its prover cannot fit the budget)
+    fails that class's job with -4 while the jobs of a class that fits finish
+    with the oracle's bytes."""
+    small = 6
+    # provers hold 2^(24 - log_n) proofs, between 2 and 1024
+    fit = int(vx.lib.vx_prover_estimate_bytes(small, 80, 3, 1024))
+    budget = 2 * fit + (1 << 20)
+    assert fit > 0 and int(vx.lib.vx_prover_estimate_bytes(16, 80, 3, 256)) > budget
+    srv = vx.Server(budget, 80, 3)
+    ok_jobs = [srv.submit(small, [(i + 3, 7 * i + 1)]) for i in range(4)]
+    big = srv.submit(16, [(5, 9)])
+    assert srv.wait_all(raise_on_failure=False) is False
+    assert srv.status(big) == -4
+    for i, jid in enumerate(ok_jobs):
+        assert srv.status(jid) == 1
+        assert srv.proof(jid, 0, small) == O.prove(i + 3, 7 * i + 1, small, 80, 3)
+    srv.close()
+
+
+def test_inject_fault_argument_errors(vx):
+    srv = vx.Server(0, 80, 3)
+    with pytest.raises(vx.HipError):
+        srv.inject_fault(0, 0)  # no such job
+    jid = srv.submit(5, [(1, 2)])
+    with pytest.raises(vx.HipError):
+        srv.inject_fault(jid, 1)  # no such column
+    srv.wait_all()
+    with pytest.raises(vx.HipError):
+        srv.inject_fault(jid, 0)  # already finished
+    srv.close()
+
+
+def test_unsupported_protocol_is_refused_at_creation(vx):
+    """More than 4096 queries or a last layer above 2^10 can never make a
+    prover: the server refuses them at creation instead of spinning on jobs it
+    cannot serve; an unsupported size class is refused at submission."""
+    with pytest.raises(vx.HipError):
+        vx.Server(0, 5000, 3)
+    with pytest.raises(vx.HipError):
+        vx.Server(0, 80, 11)
+    srv = vx.Server(0, 80, 3)
+    with pytest.raises(vx.HipError):
+        srv.submit(30, [(1, 2)])  # 2^32 points: beyond the circle group
+    with pytest.raises(vx.HipError):
+        srv.submit(1, [(1, 2)])
+    srv.close()

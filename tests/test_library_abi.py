@@ -51,13 +51,15 @@ def test_prover_estimate_bytes_host_only():
     import vortexstark_amd
 
     est = vortexstark_amd.lib.vx_prover_estimate_bytes
-    assert est(1, 80, 3, 1) == 0 and est(29, 80, 3, 1) == 0 and est(12, 80, 3, 0) == 0
+    assert est(1, 80, 3, 1) == 0 and est(30, 80, 3, 1) == 0 and est(12, 80, 3, 0) == 0
+    # 2^29 rows evaluate on the whole circle group (2^31 points): the largest prover
+    assert 80e9 < est(29, 80, 3, 1) < 100e9
     assert est(12, 80, 11, 1) == 0 and est(12, 5000, 3, 1) == 0
     assert est(12, 0, 0, 1) == est(12, 80, 3, 1)
     one, two, nine = est(12, 80, 3, 1), est(12, 80, 3, 2), est(12, 80, 3, 9)
     arena = two - one
     assert arena > 0 and nine - one == 8 * arena
-    for log_n in range(4, 27):
+    for log_n in range(4, 29):
         assert est(log_n + 1, 80, 3, 1) > est(log_n, 80, 3, 1)
     # at least the trace, its LDE and FRI layer 0 of every proof
     for log_n in (10, 16, 20):
@@ -90,4 +92,9 @@ def test_argument_errors_are_reported_without_a_device():
     assert lib.vx_circle_domain_points(31, fake, fake, None) == -1 and lib.vx_get_last_error() != 0
     # host-trace proving without a prover
     assert lib.vx_prover_prove_batch_host(None, None, None, 1) == -1 and lib.vx_get_last_error() != 0
+    # servers whose protocol no prover supports are refused at creation
+    # (more than 4096 queries, a last layer above 2^10)
+    assert not lib.vx_server_create(0, 0, 5000, 3) and lib.vx_get_last_error() == -2
+    assert not lib.vx_server_create(0, 0, 80, 11) and lib.vx_get_last_error() == -2
+    assert lib.vx_server_inject_fault(None, 0, 0) == -1 and lib.vx_get_last_error() == -2
     assert lib.vx_get_last_error() == 0  # reading clears it

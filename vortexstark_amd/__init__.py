@@ -400,10 +400,22 @@ class Server:
         if lib.vx_server_set_graphs(self.handle, 1 if on else 0):
             check("vx_server_set_graphs")
 
-    def wait_all(self) -> None:
-        if lib.vx_server_wait_all(self.handle):
+    def wait_all(self, raise_on_failure: bool = True) -> bool:
+        """Serves until every job has finished; a failed job does not stop the
+        others. Returns True when all succeeded; raises HipError when one failed
+        unless raise_on_failure is False."""
+        ok = lib.vx_server_wait_all(self.handle) == 0
+        if not ok and raise_on_failure:
             check("vx_server_wait_all")
-            raise HipError("vx_server_wait_all failed")
+            raise HipError("vx_server_wait_all: a job failed")
+        return ok
+
+    def inject_fault(self, job: int, col: int = 0) -> None:
+        """Test hook: the proof of column col of a queued job is reported failed
+        through the device-side failure flag (status -3)."""
+        if lib.vx_server_inject_fault(self.handle, job, col):
+            check("vx_server_inject_fault")
+            raise HipError("vx_server_inject_fault failed")
 
     def status(self, job: int) -> int:
         return lib.vx_server_job_status(self.handle, job)

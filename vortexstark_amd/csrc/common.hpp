@@ -19,6 +19,10 @@ namespace vx {
 
 void set_error(int code, const char* what);
 int peek_error();  // current thread's recorded error, not cleared
+// Moves the calling thread's recorded error out (0 when none) and clears it.
+int take_error(std::string& msg);
+// Records code and message as given, unless an error is already recorded.
+void put_error(int code, const std::string& msg);
 hipStream_t current_stream();
 
 // Records launch/runtime failures after a kernel launch or runtime call.

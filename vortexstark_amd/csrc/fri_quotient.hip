@@ -49,8 +49,9 @@ __global__ void quotient_kernel(const uint32_t* __restrict__ t, uint32_t* __rest
     uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= chunk_n) return;
     uint32_t i = offset + li;
-    uint32_t i1 = i + 1 >= n ? i + 1 - n : i + 1;
-    uint32_t i2 = i + 2 >= n ? i + 2 - n : i + 2;
+    // a full remainder, as the reference takes, so every n >= 1 wraps correctly
+    uint32_t i1 = (i + 1) % n;
+    uint32_t i2 = (i + 2) % n;
     uint32_t c = msub(t[i2], madd(t[i1], t[i]));
     QM31 r = qmul_m31(alpha, c);
     o0[li] = r.v[0];
@@ -106,6 +107,10 @@ void cuda_fibonacci_quotient_chunk_stream(const uint32_t* trace, uint32_t* out0,
                                           uint32_t* out3, const uint32_t* alpha, uint32_t offset, uint32_t chunk_n,
                                           uint32_t global_n, void* stream) {
     if (!chunk_n) return;
+    if (!global_n || (uint64_t)offset + chunk_n > global_n) {
+        set_error(-2, "cuda_fibonacci_quotient: need offset + chunk_n <= global_n");
+        return;
+    }
     quotient_kernel<<<ceil_div(chunk_n, 256), 256, 0, (hipStream_t)stream>>>(
         trace, out0, out1, out2, out3, q_from_host(alpha), offset, chunk_n, global_n);
     check_launch("cuda_fibonacci_quotient");

@@ -16,6 +16,7 @@
 // static words (magic, counts, path depths) is written once when the prover is
 // created and every proof overwrites all the data words.
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -309,6 +310,8 @@ __global__ void proofs_to_host_kernel(const uint32_t* src, uint64_t R, uint64_t 
         for (uint64_t y = tid; y < count; y += nth) failed_dst[y] = failed_src[y * R];
     __threadfence_system();
 }
+
+__global__ void raise_flag_kernel(uint32_t* flag) { *flag = 1; }
 
 }  // namespace vx
 
@@ -629,7 +632,9 @@ int32_t run_batch(vx_prover* p, uint32_t count) {
 bool prover_params(uint32_t log_n, uint32_t& n_queries, uint32_t& log_last_layer, uint32_t capacity) {
     if (n_queries == 0) n_queries = 80;           // prover.rs:90
     if (log_last_layer == 0) log_last_layer = 3;  // fri.rs:265
-    return !(log_n < 2 || log_n + 2 > 30 || log_last_layer > log_n + 1 || log_last_layer > 10 || n_queries > 4096 ||
+    // log_n + 2 <= 31: the largest evaluation domain is the whole circle group
+    // (circle.rs:102 asserts log_size <= 31)
+    return !(log_n < 2 || log_n + 2 > 31 || log_last_layer > log_n + 1 || log_last_layer > 10 || n_queries > 4096 ||
              capacity < 1 || capacity > 65535);
 }
 
@@ -668,6 +673,17 @@ void* device_view_of_host(void* p) {
 
 }  // namespace
 
+namespace vx {
+// Used by the proof server to report one proof of a batch failed through the
+// same flag a transcript failure raises (its fault-injection hook).
+bool prover_flag_failed(vx_prover* p, uint32_t index) {
+    if (!p || index >= p->capacity) return false;
+    uint32_t* flag = (uint32_t*)p->ch() + (uint64_t)index * p->R + offsetof(DevChannel, failed) / 4;
+    raise_flag_kernel<<<1, 1, 0, p->stream>>>(flag);
+    return check_launch("raise_flag_kernel");
+}
+}  // namespace vx
+
 extern "C" {
 
 vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer, uint32_t capacity) {
@@ -696,9 +712,11 @@ vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t l
         delete p;
         return nullptr;
     }
+    // evaluation domain half_coset(log_e), or the whole group subgroup(31) when
+    // log_e = 31, where no half coset exists (prover.rs:626-630)
     HostPt ti, ts, ei, es;
     host_coset(1, log_n, ti, ts);
-    host_coset(1, p->log_e, ei, es);
+    host_coset(p->log_e <= 30 ? 1 : 0, p->log_e, ei, es);
     build_ntt_tables(to_cpt(ti), to_cpt(ts), log_n, t_line, t_circle, t_iline, t_icircle, p->trace_tab.offs, s);
     build_ntt_tables(to_cpt(ei), to_cpt(es), p->log_e, e_line, e_circle, nullptr, nullptr, p->eval_tab.offs, s);
     p->trace_tab.line = t_iline;
@@ -708,7 +726,9 @@ vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t l
     p->eval_tab.circle = e_circle;
     p->eval_tab.log_n = p->log_e;
 
-    // FRI fold twiddles (fri.rs:143-160): [0] circle on half_coset(log_e), y;
+    // FRI fold twiddles (fri.rs:143-160): [0] circle on half_coset(log_e), y
+    // (subgroup(31) when log_e = 31, prover.rs:754-757: its points (1, 0) and
+    // (-1, 0) have y = 0, whose inverse the batch inverse takes as 0);
     // [j] line on half_odds(log_e - j), x.
     uint32_t* tw_scratch;
     ok = p->alloc((void**)&tw_scratch, 4 * (E / 2));
@@ -718,7 +738,7 @@ vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t l
         ok = p->alloc((void**)&p->fri_tw[j], 4 * ((1ull << k) / 2));
         if (!ok) break;
         HostPt di, dstp;
-        host_coset(j == 0 ? 1 : 2, k, di, dstp);
+        host_coset(j == 0 ? (k <= 30 ? 1 : 0) : 2, k, di, dstp);
         build_fold_twiddles(to_cpt(di), to_cpt(dstp), k, j == 0, p->fri_tw[j], tw_scratch, s);
     }
 
@@ -1029,13 +1049,9 @@ void vx_merkle_stage_plan_reset(void) {
 }
 
 uint64_t vx_krak_proof_size(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer) {
-    if (log_n < 2 || log_n + 2 > 30 || log_last_layer > log_n + 1) return 0;
+    if (!prover_params(log_n, n_queries, log_last_layer, 1)) return 0;
     vx_prover shape;  // layout only: no device resources
-    shape.log_n = log_n;
-    shape.log_e = log_n + 2;
-    shape.n_q = n_queries ? n_queries : 80;
-    shape.log_last = log_last_layer ? log_last_layer : 3;
-    shape.J = shape.log_e - shape.log_last;
+    set_sizes(&shape, log_n, n_queries, log_last_layer, 1);
     Layout L;
     build_layout(&shape, L);
     return 4ull * L.words.size();

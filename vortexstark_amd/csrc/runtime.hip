@@ -27,6 +27,18 @@ void set_error(int code, const char* what) {
     }
 }
 hipStream_t current_stream() { return t_stream; }
+int take_error(std::string& msg) {
+    const int e = t_err;
+    msg = e ? t_msg : std::string();
+    t_err = 0;
+    return e;
+}
+void put_error(int code, const std::string& msg) {
+    if (t_err == 0 && code) {
+        t_err = code;
+        t_msg = msg;
+    }
+}
 int peek_error() { return t_err; }
 
 __global__ void m31_add_kernel(const uint32_t* a, const uint32_t* b, uint32_t* out, uint32_t n) {

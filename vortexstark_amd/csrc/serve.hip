@@ -15,16 +15,20 @@
 //    budget (or ahead of traffic by vx_server_prewarm); idle provers of
 //    classes without work are released (least recently used first) to make
 //    room;
-//  * finished proofs go by one strided DMA copy per batch into a pinned host
-//    ring, where they stay until the job is released; a batch waits while the
-//    ring has no room for it;
-//  * a finished batch is noticed through an event query on its stream.
+//  * finished proofs are written by the device into one region of a pinned
+//    host ring, where they stay until the job is released; a batch waits while
+//    the ring has no room for it;
+//  * a finished batch is noticed through an event query on its stream;
+//  * a failure is confined to the jobs it touches: their status turns
+//    negative and every other job still completes (vx_server_wait_all returns
+//    once all jobs are finished, -1 if a job failed since the previous wait).
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -56,6 +60,7 @@ struct Job {
     double arrival = 0, done = -1;
     int32_t error = 0;
     bool released = false;
+    std::vector<uint8_t> inject;  // per column: report this proof failed (vx_server_inject_fault)
     uint64_t proof_size = 0;
     std::vector<int64_t> region;  // per column: ring region holding its proof
     std::vector<uint8_t*> where;  // per column: address of its proof in the ring
@@ -87,6 +92,12 @@ struct Class {
 
 }  // namespace
 
+namespace vx {
+// Raises the transcript failure flag of proof `index` of the prover's last
+// launch sequence, on the prover's stream (defined with the prover).
+bool prover_flag_failed(vx_prover* p, uint32_t index);
+}  // namespace vx
+
 struct vx_server {
     uint64_t budget = 0, used = 0;
     uint32_t n_q = 80, log_last = 3;
@@ -95,6 +106,32 @@ struct vx_server {
     std::map<uint32_t, Class> classes;
     std::vector<Job> jobs;
     uint64_t completed = 0, batches = 0, proofs_done = 0, peak_used = 0;
+    uint64_t failed_jobs = 0;
+    // First job failure since the last wait: reported to the waiting caller
+    // through the thread's error slot, which the server keeps clear meanwhile
+    // (a recorded error would make every later launch look failed).
+    int32_t fail_code = 0;
+    std::string fail_msg;
+    uint64_t failed_reported = 0;
+    void note_failure(int32_t code, const char* what) {
+        std::string cause;
+        const int e = vx::take_error(cause);
+        if (!fail_code) {
+            fail_code = code;
+            fail_msg = what;
+            if (e && !cause.empty()) fail_msg += " (" + cause + ")";
+        }
+    }
+    // 0 when no job failed since the previous call, else -1 with the first
+    // failure recorded as the thread's error
+    int32_t report_failures() {
+        if (failed_jobs == failed_reported) return 0;
+        failed_reported = failed_jobs;
+        vx::put_error(fail_code ? fail_code : -1, fail_msg.empty() ? "vx_server: a job failed" : fail_msg);
+        fail_code = 0;
+        fail_msg.clear();
+        return -1;
+    }
 
     ~vx_server() {
         for (auto& kv : classes)
@@ -131,25 +168,29 @@ struct vx_server {
     uint64_t estimate_bytes(uint32_t log_n) const {
         return vx_prover_estimate_bytes(log_n, n_q, log_last, class_capacity(log_n));
     }
-    bool reserve_room(uint64_t need, uint32_t for_class) {
-        while (used + need > budget) {
-            // release the idle prover used least recently, in a class without pending work
-            Slot* victim = nullptr;
-            uint32_t victim_class = 0;
-            for (auto& kv : classes) {
-                if (kv.first == for_class || !kv.second.pending.empty()) continue;
-                for (auto& s : kv.second.slots)
-                    if (s->prover && !s->busy() && (!victim || s->last_used < victim->last_used)) {
-                        victim = s.get();
-                        victim_class = kv.first;
-                    }
-            }
-            if (!victim) return false;
-            release(*victim);
-            auto& v = classes[victim_class].slots;
-            v.erase(std::remove_if(v.begin(), v.end(), [](const std::unique_ptr<Slot>& x) { return !x->prover; }),
-                    v.end());
+    // Releases the idle prover used least recently in a class other than
+    // for_class that has no pending work; false when there is none.
+    bool release_idle(uint32_t for_class) {
+        Slot* victim = nullptr;
+        uint32_t victim_class = 0;
+        for (auto& kv : classes) {
+            if (kv.first == for_class || !kv.second.pending.empty()) continue;
+            for (auto& s : kv.second.slots)
+                if (s->prover && !s->busy() && (!victim || s->last_used < victim->last_used)) {
+                    victim = s.get();
+                    victim_class = kv.first;
+                }
         }
+        if (!victim) return false;
+        release(*victim);
+        auto& v = classes[victim_class].slots;
+        v.erase(std::remove_if(v.begin(), v.end(), [](const std::unique_ptr<Slot>& x) { return !x->prover; }),
+                v.end());
+        return true;
+    }
+    bool reserve_room(uint64_t need, uint32_t for_class) {
+        while (used + need > budget)
+            if (!release_idle(for_class)) return false;
         return true;
     }
     // Launches up to capacity pending proofs of the class on an idle prover.
@@ -170,12 +211,23 @@ struct vx_server {
         }
         hipStream_t st = (hipStream_t)vx_prover_stream(s.prover);
         bool ok = vx_prove_fibonacci_batch(s.prover, ab.data(), count) == 0;
+        // an injected fault takes the path of a device-side failure: the proof's
+        // transcript flag is raised after its proving and read back with the proofs
+        for (uint32_t i = 0; ok && i < count; i++) {
+            const Job& j = jobs[s.running[i].job];
+            if (j.inject[s.running[i].col]) ok = vx::prover_flag_failed(s.prover, i);
+        }
         // proofs (into the ring) and transcript flags, on the same stream
         ok = ok && vx_prover_enqueue_copy(s.prover, count, ring.at(region), s.h_failed, st) == 0;
         ok = ok && vx::check(hipEventRecord(s.done, st), "hipEventRecord");
         s.last_used = now_s();
         batches++;
-        if (!ok) fail_running(s, -1);
+        if (!ok) {
+            // work already queued may still write the ring region: let it drain first
+            (void)hipStreamSynchronize(st);
+            note_failure(-1, "vx_server: a batch could not be launched");
+            fail_running(s, -1);
+        }
         return true;
     }
     void fail_running(Slot& s, int32_t code) {
@@ -191,6 +243,7 @@ struct vx_server {
     void finish(Job& j) {
         j.done = now_s();
         completed++;
+        if (j.error) failed_jobs++;
     }
     void retire(Slot& s) {
         uint8_t* base = ring.at(s.region);
@@ -199,7 +252,10 @@ struct vx_server {
             Job& j = jobs[it.job];
             j.region[it.col] = s.region;
             j.where[it.col] = base + (uint64_t)i * s.proof_size;
-            if (s.h_failed[i]) j.error = -3;
+            if (s.h_failed[i]) {
+                j.error = -3;
+                note_failure(-3, "vx_server: a proof's device-side failure flag was raised");
+            }
             proofs_done++;
             if (--j.remaining == 0) finish(j);
         }
@@ -221,8 +277,21 @@ struct vx_server {
         }
         c.pending.clear();
     }
-    // one scheduler step; returns the number of batches launched or retired
+    // One scheduler step; returns the number of batches launched or retired.
+    // An error the caller had recorded before the step is set aside during it
+    // and restored afterwards.
     int32_t poll() {
+        std::string held_msg;
+        const int held = vx::take_error(held_msg);
+        const int32_t acted = poll_step();
+        if (held) {
+            std::string ignore;
+            (void)vx::take_error(ignore);
+            vx::put_error(held, held_msg);
+        }
+        return acted;
+    }
+    int32_t poll_step() {
         int32_t acted = 0;
         for (auto& kv : classes) {
             for (auto& s : kv.second.slots) {
@@ -233,6 +302,7 @@ struct vx_server {
                     acted++;
                 } else if (q != hipErrorNotReady) {
                     vx::check(q, "vx_server: batch failed");
+                    note_failure((int32_t)q, "vx_server: batch failed");
                     fail_running(*s, (int32_t)q);
                     acted++;
                 }
@@ -262,12 +332,21 @@ struct vx_server {
                             acted++;
                         else
                             ring_full = true;
+                    } else if (c.slots.empty() && !any_busy() && !release_idle(kv.first)) {
+                        // the prover cannot be made and nothing running or idle can
+                        // free memory for it: waiting cannot help
+                        note_failure(-6, "vx_server: could not create a prover for this size class");
+                        fail_pending(c, -6);
+                        acted++;
                     } else {
-                        (void)vx_get_last_error();  // out of device memory: retry when something frees up
+                        // out of device memory now: retry once a batch finishes or
+                        // an idle prover has been released
+                        (void)vx_get_last_error();
+                        acted += c.slots.empty() && !any_busy() ? 1 : 0;  // released one: poll again at once
                     }
                 } else if (c.slots.empty() && !any_busy()) {
                     // nothing will free memory: this class can never run under the budget
-                    vx::set_error(-4, "vx_server: a proof of this size does not fit the HBM budget");
+                    note_failure(-4, "vx_server: a proof of this size does not fit the HBM budget");
                     fail_pending(c, -4);
                     acted++;
                 }
@@ -275,7 +354,7 @@ struct vx_server {
         }
         if (ring_full && !acted && !any_busy()) {
             // nothing runs and no finished job has been released: waiting cannot help
-            vx::set_error(-5, "vx_server: result ring full; release finished jobs");
+            note_failure(-5, "vx_server: result ring full; release finished jobs");
             for (auto& kv : classes) fail_pending(kv.second, -5);
         }
         return acted;
@@ -286,6 +365,13 @@ extern "C" {
 
 vx_server* vx_server_create(uint64_t budget_bytes, uint64_t result_bytes, uint32_t n_queries,
                             uint32_t log_last_layer) {
+    // the smallest trace a last layer of 2^log_last_layer allows must make a
+    // prover: otherwise no job could ever be served
+    const uint32_t log_last = log_last_layer ? log_last_layer : 3;
+    if (vx_prover_estimate_bytes(std::max<uint32_t>(2, log_last - 1), n_queries, log_last_layer, 1) == 0) {
+        vx::set_error(-2, "vx_server_create: unsupported n_queries or log_last_layer");
+        return nullptr;
+    }
     size_t free_b = 0, total_b = 0;
     if (!vx::check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo")) return nullptr;
     vx_server* s = new vx_server();
@@ -304,7 +390,7 @@ vx_server* vx_server_create(uint64_t budget_bytes, uint64_t result_bytes, uint32
 void vx_server_destroy(vx_server* s) { delete s; }
 
 int64_t vx_server_submit(vx_server* s, uint32_t log_n, uint32_t n_cols, const uint32_t* ab) {
-    if (!s || !ab || n_cols == 0 || log_n < 2 || log_n + 2 > 30 || s->log_last > log_n + 1) {
+    if (!s || !ab || n_cols == 0 || s->estimate_bytes(log_n) == 0) {
         vx::set_error(-2, "vx_server_submit: bad job");
         return -1;
     }
@@ -317,6 +403,7 @@ int64_t vx_server_submit(vx_server* s, uint32_t log_n, uint32_t n_cols, const ui
     j.proof_size = vx_krak_proof_size(log_n, s->n_q, s->log_last);
     j.region.assign(n_cols, -1);
     j.where.assign(n_cols, nullptr);
+    j.inject.assign(n_cols, 0);
     s->jobs.push_back(std::move(j));
     Class& c = s->classes[log_n];
     for (uint32_t k = 0; k < n_cols; k++) c.pending.push_back(Item{id, k, ab[2 * k] % M31_P, ab[2 * k + 1] % M31_P});
@@ -326,7 +413,7 @@ int64_t vx_server_submit(vx_server* s, uint32_t log_n, uint32_t n_cols, const ui
 int32_t vx_server_poll(vx_server* s) { return s ? s->poll() : -1; }
 
 int32_t vx_server_prewarm(vx_server* s, uint32_t log_n, uint32_t n_provers) {
-    if (!s || log_n < 2 || log_n + 2 > 30 || s->log_last > log_n + 1) {
+    if (!s || s->estimate_bytes(log_n) == 0) {
         vx::set_error(-2, "vx_server_prewarm: bad size class");
         return -1;
     }
@@ -358,10 +445,31 @@ uint64_t vx_server_jobs(const vx_server* s) { return s ? s->jobs.size() : 0; }
 
 int32_t vx_server_wait_all(vx_server* s) {
     if (!s) return -1;
-    while (s->completed < s->jobs.size()) {
+    // every pass either acts or has a batch in flight: poll fails the jobs of a
+    // class that can make no progress, so this loop ends
+    while (s->completed < s->jobs.size())
         if (s->poll() == 0) std::this_thread::sleep_for(std::chrono::microseconds(20));
-        if (vx::peek_error()) return -1;
+    return s->report_failures();
+}
+
+int32_t vx_server_inject_fault(vx_server* s, int64_t job, uint32_t col) {
+    if (!s || job < 0 || job >= (int64_t)s->jobs.size() || col >= s->jobs[job].n_cols) {
+        vx::set_error(-2, "vx_server_inject_fault: no such job column");
+        return -1;
     }
+    Job& j = s->jobs[job];
+    for (const auto& kv : s->classes)
+        for (const auto& sl : kv.second.slots)
+            for (const Item& it : sl->running)
+                if (it.job == job && it.col == col) {
+                    vx::set_error(-2, "vx_server_inject_fault: that proof is already on the device");
+                    return -1;
+                }
+    if (j.done >= 0) {
+        vx::set_error(-2, "vx_server_inject_fault: the job has finished");
+        return -1;
+    }
+    j.inject[col] = 1;
     return 0;
 }
 
@@ -415,11 +523,10 @@ int32_t vx_server_run_trace(vx_server* s, uint32_t n_jobs, const double* arrival
             next++;
         }
         if (s->poll() == 0) std::this_thread::sleep_for(std::chrono::microseconds(20));
-        if (vx::peek_error()) return -1;
     }
     if (latency_ms)
         for (uint32_t i = 0; i < n_jobs; i++) latency_ms[i] = vx_server_job_latency_ms(s, first + i);
-    return 0;
+    return s->report_failures();
 }
 
 void vx_server_stats(const vx_server* s, uint64_t out[6]) {
