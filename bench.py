@@ -126,11 +126,50 @@ def valu_roofline(kernel: str, compressions: int, ms: float) -> dict:
 
 
 def cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it: model name, family/model/stepping,
+    logical CPUs listed."""
     try:
         with open("/proc/cpuinfo") as f:
-            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+            text = f.read()
     except OSError:
         return ""
+    first = {}
+    for ln in text.splitlines():
+        if ":" in ln:
+            k, v = (x.strip() for x in ln.split(":", 1))
+            first.setdefault(k, v)
+    n = sum(1 for ln in text.splitlines() if ln.startswith("processor"))
+    return (f"{first.get('model name', '?')} (vendor {first.get('vendor_id', '?')}, family {first.get('cpu family', '?')}, "
+            f"model {first.get('model', '?')}, stepping {first.get('stepping', '?')}, {n} logical CPUs listed)")
+
+
+def host_threads() -> tuple:
+    """Threads for the CPU baseline: every core this process may use (its
+    affinity mask, capped by a cgroup CPU quota and by OMP_NUM_THREADS when
+    either is set: a shared host grants each GPU a share of its cores), with how
+    the count was found."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    threads = affinity if quota is None else max(1, min(affinity, int(quota + 0.5)))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    info = {"affinity_cpus": affinity, "cgroup_cpu_quota": quota, "omp_num_threads": omp or None,
+            "os_cpu_count": os.cpu_count(), "threads_used": threads, "cpu": cpu_model()}
+    return threads, info
+
+
+HOST_THREADS, HOST_INFO = host_threads()
 
 
 def quantiles(xs):
@@ -256,24 +295,49 @@ def bench_plain(vx, args) -> dict:
     if not args.no_cpu:
         import oracle_lib
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = HOST_THREADS
         L = oracle_lib.lib()
         L.orc_set_threads(threads)
         cl = args.cpu_log_n
-        log(f"cpu baseline: oracle prove at log_n={cl}, {args.cpu_reps} reps, {threads} threads")
-        secs = L.orc_time_prove_fibonacci(a % vx.P, b % vx.P, cl, nq, last, args.cpu_reps)
-        cpu_rate = (1 << cl) * args.cpu_reps / secs / 1e6
+        log(f"cpu baseline: oracle prove at log_n={cl}, 1 warm-up + {args.cpu_reps} timed, {threads} threads")
+        each = np.zeros(args.cpu_reps, np.float64)
+        L.orc_time_prove_each(a % vx.P, b % vx.P, cl, nq, last, 1, args.cpu_reps, each.ctypes.data)
+        secs = float(np.median(each))
+        cpu_rate = (1 << cl) / secs / 1e6
         cpu = {"value": round(cpu_rate, 4), "unit": "Melem/s", "cores": threads, "kind": "port",
-               "sample": f"oracle (C++ restatement) full proof of a 2^{cl}-row Fibonacci column, {nq} queries, "
-                         f"last layer 2^{last}, {args.cpu_reps} reps after twiddle precompute, {secs:.2f}s; "
-                         f"{cpu_model()}"}
-        log(f"cpu baseline: {cpu_rate:.3f} Melem/s ({secs:.2f}s)")
+               "sample": f"oracle (C++ restatement) full proof of the same 2^{cl}-row Fibonacci column, {nq} queries, "
+                         f"last layer 2^{last}: median of {args.cpu_reps} proofs after one warm-up "
+                         f"({', '.join(f'{x:.2f}' for x in each)} s), twiddles precomputed",
+               "host": HOST_INFO}
+        log(f"cpu baseline: {cpu_rate:.3f} Melem/s (median {secs:.2f}s)")
+        # the reference's CPU-runnable case: 2^16 rows, 64 queries, FRI to 2^4
+        c1 = np.zeros(7, np.float64)
+        L.orc_time_prove_each(a % vx.P, b % vx.P, 16, 64, 4, 1, len(c1), c1.ctypes.data)
+        c1_ms = 1e3 * float(np.median(c1))
+        gp = vx.Prover(16, 64, 4)
+        gp.prove(a, b)
+        g_ms = []
+        for _ in range(7):
+            t = time.perf_counter()
+            gp.prove(a, b)
+            g_ms.append(1e3 * (time.perf_counter() - t))
+        gp.close()
+        cpu["config_2p16"] = {"workload": "one Fibonacci column of 2^16 rows, blowup 4 (2^18), FRI to 2^4, "
+                                          "64 queries drawn by Fiat-Shamir",
+                              "cpu_ms_median": round(c1_ms, 3), "cpu_runs": 7, "cpu_threads": threads,
+                              "cpu_melem_s": round((1 << 16) / c1_ms / 1e3, 4),
+                              "gpu_ms_median": round(float(np.median(g_ms)), 4),
+                              "gpu_note": "vx_prove_fibonacci: device trace generation, proof, copy of the proof "
+                                          "bytes to the host, one call at a time"}
+        log(f"config 2^16: cpu median {c1_ms:.2f} ms, gpu median {np.median(g_ms):.3f} ms")
         # one thread, on a smaller trace so the run stays short
         l1 = min(cl, 18)
         L.orc_set_threads(1)
-        secs1 = L.orc_time_prove_fibonacci(a % vx.P, b % vx.P, l1, nq, last, 1)
+        one = np.zeros(3, np.float64)
+        L.orc_time_prove_each(a % vx.P, b % vx.P, l1, nq, last, 0, len(one), one.ctypes.data)
+        secs1 = float(np.median(one))
         cpu["single_thread"] = {"value": round((1 << l1) / secs1 / 1e6, 4), "unit": "Melem/s", "cores": 1,
-                                "sample": f"one oracle proof of a 2^{l1}-row column, {secs1:.2f}s"}
+                                "sample": f"oracle proof of a 2^{l1}-row column, median of 3, {secs1:.2f}s"}
         L.orc_set_threads(threads)
         log(f"cpu baseline, 1 thread: {(1 << l1) / secs1 / 1e6:.3f} Melem/s at 2^{l1} rows ({secs1:.2f}s)")
     prover.close()
@@ -324,7 +388,7 @@ def bench_wide(vx, args) -> dict:
     if not args.no_cpu:
         import oracle_lib
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = HOST_THREADS
         L = oracle_lib.lib()
         L.orc_set_threads(threads)
         ptrs = (C.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
@@ -377,17 +441,20 @@ def bench_batch(vx, args) -> dict:
     rows = sum((1 << l) * len(ab) for l, ab in jobs)
     nq, last = 80, 3
     # one job at a time on the plain path: one prover per size, each proof to host bytes
-    solo = {l: vx.Prover(l, nq, last) for l in sorted({l for l, _ in jobs})}
-    for l, p in solo.items():
-        p.prove(1, 1)
-    t = time.perf_counter()
-    solo_out = []
-    for l, ab in jobs:
-        solo_out.append([solo[l].prove(a, b) for a, b in ab])
-    t_solo = time.perf_counter() - t
-    for p in solo.values():
-        p.close()
-    log(f"batch: {n_proofs} proofs one at a time: {t_solo:.3f}s ({n_proofs / t_solo:.1f} proofs/s)")
+    # (--no-serial skips it, for profiles of the batched pass alone)
+    solo_out, t_solo = None, None
+    if not args.no_serial:
+        solo = {l: vx.Prover(l, nq, last) for l in sorted({l for l, _ in jobs})}
+        for l, p in solo.items():
+            p.prove(1, 1)
+        t = time.perf_counter()
+        solo_out = []
+        for l, ab in jobs:
+            solo_out.append([solo[l].prove(a, b) for a, b in ab])
+        t_solo = time.perf_counter() - t
+        for p in solo.values():
+            p.close()
+        log(f"batch: {n_proofs} proofs one at a time: {t_solo:.3f}s ({n_proofs / t_solo:.1f} proofs/s)")
     # batched: all jobs submitted at once; provers made ahead (vx_server_prewarm, two per
     # size class) and graphs captured by a first pass, so the timed pass sets nothing up
     srv = vx.Server(0, nq, last, result_bytes=4 << 30)
@@ -403,17 +470,19 @@ def bench_batch(vx, args) -> dict:
     srv.wait_all()
     t_batch = time.perf_counter() - t
     lat = [srv.latency_ms(i) for i in ids]
-    same = all(srv.proof(i, c, l) == solo_out[k][c] for k, (i, (l, ab)) in enumerate(zip(ids, jobs))
-               for c in range(len(ab)))
+    same = None if solo_out is None else all(
+        srv.proof(i, c, l) == solo_out[k][c] for k, (i, (l, ab)) in enumerate(zip(ids, jobs)) for c in range(len(ab)))
     st = srv.stats()
     srv.close()
-    log(f"batch: batched {t_batch:.3f}s ({n_proofs / t_batch:.1f} proofs/s), {t_solo / t_batch:.2f}x, "
-        f"identical to one-at-a-time: {same}")
+    log(f"batch: batched {t_batch:.3f}s ({n_proofs / t_batch:.1f} proofs/s), "
+        f"{(t_solo or 0) / t_batch:.2f}x, identical to one-at-a-time: {same}")
+    # the VALU bound of the job list: its Blake2s compressions at the VALU peak
+    bound_s = sum(proof_compressions(l, last) * len(ab) for l, ab in jobs) * OPS_PER_COMPRESSION / (VALU_PEAK_TOPS * 1e12)
     cpu = None
     if not args.no_cpu:
         import oracle_lib
 
-        threads = min(16, os.cpu_count() or 1)
+        threads = HOST_THREADS
         oracle_lib.lib().orc_set_threads(threads)
         sample = [(l, a, b) for l, ab in jobs[:12] for a, b in ab][:32]
         t = time.perf_counter()
@@ -425,32 +494,64 @@ def bench_batch(vx, args) -> dict:
     return {"workload": "256 jobs of 1-8 Fibonacci columns, log_n uniform in [10,18] (splitmix64 seed 4), "
                         "blowup 4, 80 queries, last layer 2^3; every column one KRAK proof",
             "n_proofs": n_proofs, "rows": rows, "batched_proofs_per_s": round(n_proofs / t_batch, 2),
-            "one_at_a_time_proofs_per_s": round(n_proofs / t_solo, 2),
-            "speedup_vs_one_at_a_time": round(t_solo / t_batch, 3), "batched_s": round(t_batch, 4),
-            "one_at_a_time_s": round(t_solo, 4), "batched_melem_s": round(rows / t_batch / 1e6, 2),
+            "one_at_a_time_proofs_per_s": round(n_proofs / t_solo, 2) if t_solo else None,
+            "speedup_vs_one_at_a_time": round(t_solo / t_batch, 3) if t_solo else None,
+            "batched_s": round(t_batch, 4), "one_at_a_time_s": round(t_solo, 4) if t_solo else None,
+            "batched_melem_s": round(rows / t_batch / 1e6, 2),
+            "valu_bound_s": round(bound_s, 4), "batched_frac_of_valu_bound": round(bound_s / t_batch, 4),
             "job_latency_ms": quantiles(lat), "identical_to_one_at_a_time": same,
             "batches": st["batches"], "peak_device_bytes": st["peak_device_bytes"], "cpu_baseline": cpu}
 
 
+def proof_compressions(log_n: int, log_last: int) -> int:
+    """Blake2s compressions of one proof's trees: trace and quotient trees of E
+    leaves (2E - 1 each) and one tree per FRI layer of E/2, E/4, ... values down to
+    2^(log_last + 1) (2n - 1 each); transcript hashes are negligible."""
+    E = 1 << (log_n + 2)
+    c = 2 * (2 * E - 1)
+    for j in range(log_n + 2 - log_last):
+        n = E >> (j + 1)
+        c += 2 * n - 1
+    return c
+
+
 def bench_serve(vx, args) -> dict:
-    """Two replays of the same job list: at the offered rate (latency) and with
-    every job arriving within a fraction of a second (throughput at saturation)."""
+    """The job list one at a time (a capacity-1 prover per size, every proof
+    copied to the host and hashed), then two replays of the same list through
+    the server: at the offered rate (latency) and with every job arriving
+    within a fraction of a second (throughput at saturation). Every served
+    proof is compared with its one-at-a-time bytes."""
     import inputs
 
     jobs = batch_jobs(5, args.serve_jobs, 12, 26, 34)
     u = (inputs.splitmix64(6, len(jobs)) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
     gaps = -np.log1p(-u)
     nq, last = 80, 3
+    n_proofs = sum(len(ab) for _, ab in jobs)
+    rows = sum((1 << l) * len(ab) for l, ab in jobs)
+    # one at a time: the whole job list in arrival order, each proof to host bytes
+    sizes = sorted({l for l, _ in jobs})
+    solo = {}
+    for l in sizes:
+        solo[l] = vx.Prover(l, nq, last)
+        solo[l].prove(1, 1)
+    t = time.perf_counter()
+    digests = []
+    for l, ab in jobs:
+        digests.append([hashlib.sha256(solo[l].prove(a, b)).digest() for a, b in ab])
+    t_serial = time.perf_counter() - t
+    for p in solo.values():
+        p.close()
+    log(f"serve: {n_proofs} proofs one at a time: {t_serial:.3f}s")
     srv = vx.Server(288 << 30, nq, last, result_bytes=8 << 30)
     # warm-up: one single-column job per size creates the provers (outside the replays)
-    for l in sorted({l for l, _ in jobs}):
+    for l in sizes:
         srv.submit(l, [(1, 1)])
     srv.wait_all()
     for i in range(srv.stats()["jobs_done"]):
         srv.release(i)
-    n_proofs = sum(len(ab) for _, ab in jobs)
-    rows = sum((1 << l) * len(ab) for l, ab in jobs)
     runs = {}
+    identical = True
     for name, rate in (("offered", args.serve_rate), ("saturated", args.serve_rate * 50)):
         arrivals = np.cumsum(gaps / rate)
         arrivals -= arrivals[0]
@@ -460,31 +561,36 @@ def bench_serve(vx, args) -> dict:
                             [v for _, ab in jobs for pair in ab for v in pair])
         span = time.perf_counter() - t
         ok = all(srv.status(first + i) == 1 for i in range(len(jobs)))
+        same = ok and all(hashlib.sha256(srv.proof(first + i, c, l)).digest() == digests[i][c]
+                          for i, (l, ab) in enumerate(jobs) for c in range(len(ab)))
+        identical = identical and same
         for i in range(len(jobs)):
             srv.release(first + i)
         runs[name] = {"rate_jobs_per_s": rate, "makespan_s": round(span, 4), "proofs_per_s": round(n_proofs / span, 2),
-                      "melem_s": round(rows / span / 1e6, 2), "latency_ms": quantiles(list(lat)), "all_done": ok}
+                      "melem_s": round(rows / span / 1e6, 2), "latency_ms": quantiles(list(lat)), "all_done": ok,
+                      "identical_to_one_at_a_time": same}
         log(f"serve ({name}, {rate:.0f} jobs/s): {n_proofs} proofs in {span:.3f}s, {rows / span / 1e6:.0f} Melem/s, "
-            f"p50 {np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms")
+            f"p50 {np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms, same bytes: {same}")
     st = srv.stats()
     srv.close()
-    # the same proofs one at a time on the plain path (time per size measured once, then summed)
-    per = {}
-    for l in sorted({l for l, _ in jobs}):
-        p = vx.Prover(l, nq, last)
-        p.prove(1, 1)
-        t = time.perf_counter()
-        p.prove(2, 3)
-        per[l] = time.perf_counter() - t
-        p.close()
-    t_serial = sum(per[l] * len(ab) for l, ab in jobs)
     sat = runs["saturated"]["makespan_s"]
-    log(f"serve: one at a time {t_serial:.3f}s -> {t_serial / sat:.2f}x at saturation")
+    # the VALU bound of the job list: its Blake2s compressions at the VALU peak
+    comp = sum(proof_compressions(l, last) * len(ab) for l, ab in jobs)
+    bound_s = comp * OPS_PER_COMPRESSION / (VALU_PEAK_TOPS * 1e12)
+    log(f"serve: one at a time {t_serial:.3f}s -> {t_serial / sat:.2f}x at saturation; VALU bound {bound_s:.3f}s "
+        f"(saturated at {bound_s / sat:.2f}, one at a time at {bound_s / t_serial:.2f} of it)")
     return {"workload": f"{len(jobs)} jobs, Poisson arrivals, 1-34 Fibonacci columns, log_n uniform in [12,26] "
                         "(splitmix64 seeds 5, 6), 80 queries, last layer 2^3, HBM budget 288 GB",
             "n_proofs": n_proofs, "rows": rows, **runs,
-            "one_at_a_time_s": round(t_serial, 4), "speedup_vs_one_at_a_time": round(t_serial / sat, 3),
-            "one_at_a_time_ms_per_proof": {str(l): round(1e3 * v, 3) for l, v in per.items()},
+            "one_at_a_time_s": round(t_serial, 4),
+            "one_at_a_time": "measured: the same job list proved in arrival order by a capacity-1 prover per size, "
+                             "each proof copied to the host",
+            "speedup_vs_one_at_a_time": round(t_serial / sat, 3),
+            "identical_to_one_at_a_time": identical,
+            "valu_bound_s": round(bound_s, 4), "compressions": comp,
+            "saturated_frac_of_valu_bound": round(bound_s / sat, 4),
+            "one_at_a_time_frac_of_valu_bound": round(bound_s / t_serial, 4),
+            "max_speedup_at_valu_bound": round(t_serial / bound_s, 3),
             "peak_device_bytes": st["peak_device_bytes"], "batches": st["batches"]}
 
 
@@ -768,8 +874,10 @@ def main() -> int:
     ap.add_argument("--log-last", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-pipelined", action="store_true", help="skip the multi-stream plain measurement")
+    ap.add_argument("--no-serial", action="store_true",
+                    help="skip the one-at-a-time pass of the batch workload (profiles of the batched pass alone)")
     ap.add_argument("--cpu-log-n", type=int, default=24)
-    ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--skip", default="", help="comma list of workloads to skip: plain,wide,batch,serve,host,deep,grind "
                                                "(skipping plain leaves value null: for profiling runs)")
     ap.add_argument("--serve-jobs", type=int, default=48)
