@@ -200,6 +200,19 @@ int32_t vx_deep_quotient(vx_deep* d, const uint32_t* const* cols, const uint32_t
                          uint32_t lifting_log_size, const uint32_t* domain_xs, const uint32_t* domain_ys,
                          uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3, void* stream);
 void vx_deep_destroy(vx_deep* d);
+/* The same quotient with no allocation at all (no reference counterpart; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops). Same arguments and words as vx_deep_quotient without
+ * the vx_deep object: the launch metadata (16 n_points + 4 n_cols +
+ * 4 n_cols n_points words, at most 8192: e.g. 680 columns at 2 points) is
+ * staged through kernel arguments into a fixed 32 KiB device array of the
+ * library, one per GPU; larger requests are rejected (-1, error -2). Calls on
+ * one GPU share that array, so calls that overlap on one GPU (different
+ * streams) race; calls on one stream are ordered. Asynchronous on stream. */
+int32_t vx_deep_quotient_noalloc(const uint32_t* const* cols, const uint32_t* col_log_sizes, uint32_t n_cols,
+                                 uint32_t n_points, const uint32_t* points, const uint32_t* line_coeffs,
+                                 uint32_t lifting_log_size, const uint32_t* domain_xs, const uint32_t* domain_ys,
+                                 uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3, void* stream);
 /* Points of CanonicCoset(log_size).circle_domain() at bit_reverse_index(row) for
  * every row, computed on the device (what quotient_ops.rs:140-147 builds on the host). */
 int32_t vx_circle_domain_points(uint32_t log_size, uint32_t* xs, uint32_t* ys, void* stream);

@@ -59,6 +59,7 @@ SIGNATURES = {
     "cuda_accumulate_lift": (None, [vp, vp, u32, u32]),
     "vx_deep_create": (vp, [u32, u32]),
     "vx_deep_quotient": (i32, [vp, vp, vp, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp, vp]),
+    "vx_deep_quotient_noalloc": (i32, [vp, vp, u32, u32, vp, vp, u32, vp, vp, vp, vp, vp, vp, vp]),
     "vx_deep_destroy": (None, [vp]),
     "vx_circle_domain_points": (i32, [u32, vp, vp, vp]),
     "cuda_grind_pow": (None, [vp, vp, u32, u64, u32]),

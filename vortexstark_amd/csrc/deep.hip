@@ -18,6 +18,11 @@
 // each thread takes COMBINE_ROWS rows (wave-strided, so every load stays
 // coalesced) and inverts their norms |den|^2 with one M31 inversion per
 // accumulation (Montgomery's trick) instead of one per row.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+
 #include "common.hpp"
 #include "field.hpp"
 #include "kernels.hpp"
@@ -417,6 +422,91 @@ using namespace vx;
 // Metadata goes through two pinned staging / device slots used in turn: a call
 // waits (host) only for the copy of the call two back, and its stream waits for
 // that call's kernel before overwriting the device slot.
+// Metadata of one fused DEEP launch, written to h (deep_meta_words words):
+// per point 16 words (x, y, then the sums A = sum_c a_pc and B = sum_c b_pc),
+// per column 4 words (pointer, lifting shift, 0), then per (column, point) the
+// 4 words of c_pc.
+static uint64_t deep_meta_words(uint32_t n_cols, uint32_t n_points) {
+    return 16ull * n_points + 4ull * n_cols + 4ull * n_cols * n_points;
+}
+static void deep_meta(uint32_t* h, const uint32_t* const* cols, const uint32_t* col_log_sizes, uint32_t n_cols,
+                      uint32_t n_points, const uint32_t* points, const uint32_t* line_coeffs,
+                      uint32_t lifting_log_size) {
+    const uint32_t NP = n_points;
+    for (uint32_t p = 0; p < NP; p++) {
+        uint32_t* pp = h + 16 * p;
+        for (int k = 0; k < 8; k++) pp[k] = points[8 * p + k] % P;
+        for (int k = 0; k < 8; k++) pp[8 + k] = 0;
+        for (uint32_t c = 0; c < n_cols; c++) {
+            const uint32_t* lc = line_coeffs + 12ull * ((uint64_t)p * n_cols + c);
+            for (int k = 0; k < 4; k++) {
+                pp[8 + k] = hadd31(pp[8 + k], lc[k] % P);       // A += a
+                pp[12 + k] = hadd31(pp[12 + k], lc[4 + k] % P);  // B += b
+            }
+        }
+    }
+    uint32_t* cm = h + 16 * NP;
+    uint32_t* cf = cm + 4 * n_cols;
+    for (uint32_t c = 0; c < n_cols; c++) {
+        const uint64_t ptr = (uint64_t)(uintptr_t)cols[c];
+        cm[4 * c] = (uint32_t)ptr;
+        cm[4 * c + 1] = (uint32_t)(ptr >> 32);
+        cm[4 * c + 2] = lifting_log_size - col_log_sizes[c] + 1;
+        cm[4 * c + 3] = 0;
+        for (uint32_t p = 0; p < NP; p++)
+            for (int k = 0; k < 4; k++)
+                cf[(c * NP + p) * 4 + k] = line_coeffs[12ull * ((uint64_t)p * n_cols + c) + 8 + k] % P;
+    }
+}
+
+// Argument checks shared by the fused DEEP entry points.
+static bool deep_args_ok(const char* who, const uint32_t* const* cols, const uint32_t* col_log_sizes, uint32_t n_cols,
+                         uint32_t max_cols, uint32_t n_points, uint32_t max_points, const uint32_t* points,
+                         const uint32_t* line_coeffs, uint32_t lifting_log_size, const uint32_t* domain_xs,
+                         const uint32_t* domain_ys, const uint32_t* out0, const uint32_t* out1, const uint32_t* out2,
+                         const uint32_t* out3) {
+    std::string msg;
+    if (!cols || !col_log_sizes || !points || !line_coeffs || !domain_xs || !domain_ys || !out0 || !out1 || !out2 ||
+        !out3) {
+        msg = std::string(who) + ": null argument";
+        set_error(hipErrorInvalidValue, msg.c_str());
+        return false;
+    }
+    if (!n_cols || n_cols > max_cols || !n_points || n_points > max_points || lifting_log_size < 1 ||
+        lifting_log_size > 30) {
+        msg = std::string(who) + ": column or point count beyond the limit, or lifting log outside 1..30";
+        set_error(-2, msg.c_str());
+        return false;
+    }
+    for (uint32_t c = 0; c < n_cols; c++)
+        if (col_log_sizes[c] < 1 || col_log_sizes[c] > lifting_log_size || !cols[c]) {
+            msg = std::string(who) + ": column log sizes must be in 1..lifting_log_size";
+            set_error(-2, msg.c_str());
+            return false;
+        }
+    return true;
+}
+
+// One fused DEEP launch over 2^lifting_log_size rows with its metadata at meta (device).
+static bool launch_deep_fused(const uint32_t* meta, uint32_t NP, uint32_t n_cols, uint32_t lifting_log_size,
+                              const uint32_t* domain_xs, const uint32_t* domain_ys, uint32_t* out0, uint32_t* out1,
+                              uint32_t* out2, uint32_t* out3, hipStream_t s) {
+    const uint32_t n_rows = 1u << lifting_log_size;
+    switch (NP) {
+#define VX_DEEP_LAUNCH(np, r)                                                                                  \
+    case np:                                                                                                   \
+        deep_fused_kernel<np, r><<<ceil_div(n_rows, (uint64_t)DEEP_THREADS * r), DEEP_THREADS, 0, s>>>(        \
+            meta, n_cols, domain_xs, domain_ys, n_rows, out0, out1, out2, out3);                               \
+        break;
+        VX_DEEP_LAUNCH(1, 4)
+        VX_DEEP_LAUNCH(2, 4)
+        VX_DEEP_LAUNCH(3, 2)
+        VX_DEEP_LAUNCH(4, 2)
+#undef VX_DEEP_LAUNCH
+    }
+    return check_launch("deep_fused_kernel");
+}
+
 struct vx_deep {
     uint32_t max_cols = 0, max_points = 0;
     size_t bytes = 0;
@@ -426,6 +516,51 @@ struct vx_deep {
     bool pending[2] = {false, false};
     uint32_t next = 0;
 };
+
+// ---------------------------------------------------------------- allocation-free form
+// vx_deep_quotient_noalloc keeps its launch metadata in this file-scope device
+// array instead of memory the library allocates: each GPU holds its own copy
+// until the process ends, and every call on a GPU reuses it (calls that overlap
+// on one GPU race on it). Host code must never pass the array's name to a
+// kernel or a copy (on the host it is only a stand-in for the symbol); it uses
+// the device address deep_scratch() looked up with hipGetSymbolAddress.
+constexpr uint32_t DEEP_SCRATCH_WORDS = 8192;  // 32 KiB
+static __device__ __align__(256) uint32_t scratch[DEEP_SCRATCH_WORDS];
+
+// Metadata words per staging launch: they travel by value in the kernel
+// arguments (2 KiB), so staging needs no host buffer the device reads.
+constexpr uint32_t META_CHUNK_WORDS = 512;
+struct MetaChunk {
+    uint32_t w[META_CHUNK_WORDS];
+};
+__global__ void stage_meta_kernel(MetaChunk c, uint32_t* dst, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = c.w[i];
+}
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_addr[MAX_DEVICES] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device: looked up on the first
+// call made with that device current, then kept.
+static uint32_t* deep_scratch() {
+    int dev = 0;
+    if (!check(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    if (dev < 0 || dev >= MAX_DEVICES) {
+        set_error(-2, "vx_deep_quotient_noalloc: device index beyond 63");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        if (!check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)") || !p)
+            return nullptr;
+        g_scratch_addr[dev] = (uint32_t*)p;
+    }
+    return g_scratch_addr[dev];
+}
 
 extern "C" {
 
@@ -508,21 +643,13 @@ int32_t vx_deep_quotient(vx_deep* d, const uint32_t* const* cols, const uint32_t
                          uint32_t n_points, const uint32_t* points, const uint32_t* line_coeffs,
                          uint32_t lifting_log_size, const uint32_t* domain_xs, const uint32_t* domain_ys,
                          uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3, void* stream) {
-    if (!d || !cols || !col_log_sizes || !points || !line_coeffs || !domain_xs || !domain_ys || !out0 || !out1 ||
-        !out2 || !out3) {
+    if (!d) {
         set_error(hipErrorInvalidValue, "vx_deep_quotient: null argument");
         return -1;
     }
-    if (!n_cols || n_cols > d->max_cols || !n_points || n_points > d->max_points || lifting_log_size < 1 ||
-        lifting_log_size > 30) {
-        set_error(-2, "vx_deep_quotient: column or point count beyond the object's, or lifting log outside 1..30");
+    if (!deep_args_ok("vx_deep_quotient", cols, col_log_sizes, n_cols, d->max_cols, n_points, d->max_points, points,
+                      line_coeffs, lifting_log_size, domain_xs, domain_ys, out0, out1, out2, out3))
         return -1;
-    }
-    for (uint32_t c = 0; c < n_cols; c++)
-        if (col_log_sizes[c] < 1 || col_log_sizes[c] > lifting_log_size || !cols[c]) {
-            set_error(-2, "vx_deep_quotient: column log sizes must be in 1..lifting_log_size");
-            return -1;
-        }
     hipStream_t s = (hipStream_t)stream;
     // this slot may still feed the copy of the call two back, and its device copy that call's kernel
     const uint32_t slot = d->next;
@@ -531,51 +658,49 @@ int32_t vx_deep_quotient(vx_deep* d, const uint32_t* const* cols, const uint32_t
         return -1;
     uint32_t* h = d->h_meta[slot];
     const uint32_t NP = n_points;
-    for (uint32_t p = 0; p < NP; p++) {
-        uint32_t* pp = h + 16 * p;
-        for (int k = 0; k < 8; k++) pp[k] = points[8 * p + k] % P;
-        for (int k = 0; k < 8; k++) pp[8 + k] = 0;
-        for (uint32_t c = 0; c < n_cols; c++) {
-            const uint32_t* lc = line_coeffs + 12ull * ((uint64_t)p * n_cols + c);
-            for (int k = 0; k < 4; k++) {
-                pp[8 + k] = hadd31(pp[8 + k], lc[k] % P);       // A += a
-                pp[12 + k] = hadd31(pp[12 + k], lc[4 + k] % P);  // B += b
-            }
-        }
-    }
-    uint32_t* cm = h + 16 * NP;
-    uint32_t* cf = cm + 4 * n_cols;
-    for (uint32_t c = 0; c < n_cols; c++) {
-        const uint64_t ptr = (uint64_t)(uintptr_t)cols[c];
-        cm[4 * c] = (uint32_t)ptr;
-        cm[4 * c + 1] = (uint32_t)(ptr >> 32);
-        cm[4 * c + 2] = lifting_log_size - col_log_sizes[c] + 1;
-        cm[4 * c + 3] = 0;
-        for (uint32_t p = 0; p < NP; p++)
-            for (int k = 0; k < 4; k++)
-                cf[(c * NP + p) * 4 + k] = line_coeffs[12ull * ((uint64_t)p * n_cols + c) + 8 + k] % P;
-    }
-    const size_t bytes = 4ull * (16ull * NP + 4ull * n_cols + 4ull * n_cols * NP);
+    deep_meta(h, cols, col_log_sizes, n_cols, n_points, points, line_coeffs, lifting_log_size);
+    const size_t bytes = 4ull * deep_meta_words(n_cols, NP);
     if (!check(hipMemcpyAsync(d->d_meta[slot], h, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync") ||
         !check(hipEventRecord(d->copied[slot], s), "hipEventRecord"))
         return -1;
-    const uint32_t n_rows = 1u << lifting_log_size;
-    switch (NP) {
-#define VX_DEEP_LAUNCH(np, r)                                                                                  \
-    case np:                                                                                                   \
-        deep_fused_kernel<np, r><<<ceil_div(n_rows, (uint64_t)DEEP_THREADS * r), DEEP_THREADS, 0, s>>>(        \
-            d->d_meta[slot], n_cols, domain_xs, domain_ys, n_rows, out0, out1, out2, out3);                          \
-        break;
-        VX_DEEP_LAUNCH(1, 4)
-        VX_DEEP_LAUNCH(2, 4)
-        VX_DEEP_LAUNCH(3, 2)
-        VX_DEEP_LAUNCH(4, 2)
-#undef VX_DEEP_LAUNCH
-    }
-    if (!check_launch("deep_fused_kernel") || !check(hipEventRecord(d->done[slot], s), "hipEventRecord")) return -1;
+    if (!launch_deep_fused(d->d_meta[slot], NP, n_cols, lifting_log_size, domain_xs, domain_ys, out0, out1, out2,
+                           out3, s))
+        return -1;
+    if (!check(hipEventRecord(d->done[slot], s), "hipEventRecord")) return -1;
     d->pending[slot] = true;
     d->next ^= 1;
     return 0;
+}
+
+int32_t vx_deep_quotient_noalloc(const uint32_t* const* cols, const uint32_t* col_log_sizes, uint32_t n_cols,
+                                 uint32_t n_points, const uint32_t* points, const uint32_t* line_coeffs,
+                                 uint32_t lifting_log_size, const uint32_t* domain_xs, const uint32_t* domain_ys,
+                                 uint32_t* out0, uint32_t* out1, uint32_t* out2, uint32_t* out3, void* stream) {
+    if (!deep_args_ok("vx_deep_quotient_noalloc", cols, col_log_sizes, n_cols, 1u << 20, n_points, DEEP_MAX_POINTS,
+                      points, line_coeffs, lifting_log_size, domain_xs, domain_ys, out0, out1, out2, out3))
+        return -1;
+    const uint64_t words = deep_meta_words(n_cols, n_points);
+    if (words > DEEP_SCRATCH_WORDS) {
+        set_error(-2, "vx_deep_quotient_noalloc: metadata of this many columns and points exceeds the 8192-word "
+                      "device scratch (16 n_points + 4 n_cols + 4 n_cols n_points words)");
+        return -1;
+    }
+    uint32_t* dst = deep_scratch();
+    if (!dst) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    // metadata travels as kernel arguments into the scratch array, chunk by chunk
+    MetaChunk c;
+    uint32_t h[DEEP_SCRATCH_WORDS];
+    deep_meta(h, cols, col_log_sizes, n_cols, n_points, points, line_coeffs, lifting_log_size);
+    for (uint64_t off = 0; off < words; off += META_CHUNK_WORDS) {
+        const uint32_t n = (uint32_t)std::min<uint64_t>(META_CHUNK_WORDS, words - off);
+        std::memcpy(c.w, h + off, 4ull * n);
+        stage_meta_kernel<<<1, 256, 0, s>>>(c, dst + off, n);
+        if (!check_launch("stage_meta_kernel")) return -1;
+    }
+    return launch_deep_fused(dst, n_points, n_cols, lifting_log_size, domain_xs, domain_ys, out0, out1, out2, out3, s)
+               ? 0
+               : -1;
 }
 
 int32_t vx_circle_domain_points(uint32_t log_size, uint32_t* xs, uint32_t* ys, void* stream) {
