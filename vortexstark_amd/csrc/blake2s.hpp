@@ -121,26 +121,18 @@ __device__ __forceinline__ void b2s_leaf4(uint32_t a, uint32_t b, uint32_t c, ui
 // diagonal step rotates rows 1..3 by 1..3 lanes with DPP quad permutes and
 // rotates them back afterwards, so a compression is ~400 instructions per lane
 // instead of ~980: used where a tree level is too narrow to fill a wave, cutting
-// its latency by ~2.4x. The message lives in LDS (16 words per quad); lane l
-// reads the words its two G functions of each round use.
+// its latency. The message lives in LDS (16 words per quad); every lane of the
+// quad holds all 16 words in registers and selects the ones its two G
+// functions of each round use.
 
-// For round r and quad lane l: sigma[r][2l] | sigma[r][2l+1] << 8 |
-// sigma[r][8+2l] << 16 | sigma[r][9+2l] << 24.
-__device__ __forceinline__ uint32_t b2s_quad_sigma(int r, uint32_t l) {
-    constexpr uint8_t S[10][16] = {
-        {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15}, {14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3},
-        {11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4}, {7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8},
-        {9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13}, {2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9},
-        {12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11}, {13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10},
-        {6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5}, {10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0},
-    };
-    uint32_t v[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-        v[q] = (uint32_t)S[r][2 * q] | ((uint32_t)S[r][2 * q + 1] << 8) | ((uint32_t)S[r][8 + 2 * q] << 16) |
-               ((uint32_t)S[r][9 + 2 * q] << 24);
-    return l == 0 ? v[0] : l == 1 ? v[1] : l == 2 ? v[2] : v[3];
-}
+// Message schedule of round r (RFC 7693, section 2.7).
+constexpr uint8_t B2S_SIGMA[10][16] = {
+    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15}, {14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3},
+    {11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4}, {7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8},
+    {9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13}, {2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9},
+    {12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11}, {13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10},
+    {6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5}, {10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0},
+};
 
 // value from lane (l + k) % 4 of the same quad
 __device__ __forceinline__ uint32_t quad_rot1(uint32_t x) {
@@ -153,28 +145,53 @@ __device__ __forceinline__ uint32_t quad_rot3(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x93, 0xF, 0xF, false);
 }
 
-// Lane l: in hl = h[l], hh = h[4+l]; out the same words of the new chaining value.
-// msg: the quad's 16 message words (LDS). All four lanes of the quad must be active.
-__device__ __forceinline__ void b2s_compress_quad(const uint32_t* msg, uint32_t t, uint32_t f, uint32_t& hl,
-                                                  uint32_t& hh) {
-    const uint32_t l = threadIdx.x & 3;
-    uint32_t a = hl, b = hh;
-    uint32_t c = l == 0 ? B2S_IV0 : l == 1 ? B2S_IV1 : l == 2 ? B2S_IV2 : B2S_IV3;
-    uint32_t d = l == 0 ? (B2S_IV4 ^ t) : l == 1 ? B2S_IV5 : l == 2 ? (B2S_IV6 ^ f) : B2S_IV7;
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        const uint32_t sg = b2s_quad_sigma(r, l);
-        const uint32_t m0 = msg[sg & 0xFF], m1 = msg[(sg >> 8) & 0xFF];
-        const uint32_t m2 = msg[(sg >> 16) & 0xFF], m3 = msg[sg >> 24];
-        VX_G(a, b, c, d, m0, m1);
+// mask ? a : b for a mask of all ones or all zeros: one v_bfi_b32
+__device__ __forceinline__ uint32_t b2s_pick(uint32_t mask, uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_bitop3_b32(mask, a, b, 0xCA);
+}
+
+// Message word sigma[R][2l + J] of quad lane l, chosen among the four
+// candidates of the quad with the lane's bit masks (m0: bit 0 of l set, m1:
+// bit 1): three selects on registers, independent of the round's state.
+template <int R, int J>
+__device__ __forceinline__ uint32_t b2s_quad_word(const uint32_t m[16], uint32_t m0, uint32_t m1) {
+    constexpr int w0 = B2S_SIGMA[R][J], w1 = B2S_SIGMA[R][2 + J], w2 = B2S_SIGMA[R][4 + J], w3 = B2S_SIGMA[R][6 + J];
+    return b2s_pick(m1, b2s_pick(m0, m[w3], m[w2]), b2s_pick(m0, m[w1], m[w0]));
+}
+
+template <int R>
+__device__ __forceinline__ void b2s_quad_round(const uint32_t m[16], uint32_t m0, uint32_t m1, uint32_t& a,
+                                               uint32_t& b, uint32_t& c, uint32_t& d) {
+    if constexpr (R < 10) {
+        // column step: lane l runs G on column l; diagonal step: on diagonal l
+        VX_G(a, b, c, d, (b2s_quad_word<R, 0>(m, m0, m1)), (b2s_quad_word<R, 1>(m, m0, m1)));
         b = quad_rot1(b);
         c = quad_rot2(c);
         d = quad_rot3(d);
-        VX_G(a, b, c, d, m2, m3);
+        VX_G(a, b, c, d, (b2s_quad_word<R, 8>(m, m0, m1)), (b2s_quad_word<R, 9>(m, m0, m1)));
         b = quad_rot3(b);
         c = quad_rot2(c);
         d = quad_rot1(d);
+        b2s_quad_round<R + 1>(m, m0, m1, a, b, c, d);
     }
+}
+
+// Lane l: in hl = h[l], hh = h[4+l]; out the same words of the new chaining value.
+// msg: the quad's 16 message words (LDS). All four lanes of the quad must be active.
+// Every lane reads the whole message into registers before the first round and
+// picks the words of its G functions by selects, so no LDS access waits inside
+// the dependent chain of the rounds.
+__device__ __forceinline__ void b2s_compress_quad(const uint32_t* msg, uint32_t t, uint32_t f, uint32_t& hl,
+                                                  uint32_t& hh) {
+    const uint32_t l = threadIdx.x & 3;
+    const uint32_t m0 = 0u - (l & 1u), m1 = 0u - ((l >> 1) & 1u);
+    uint32_t m[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) m[i] = msg[i];
+    uint32_t a = hl, b = hh;
+    uint32_t c = b2s_pick(m1, b2s_pick(m0, B2S_IV3, B2S_IV2), b2s_pick(m0, B2S_IV1, B2S_IV0));
+    uint32_t d = b2s_pick(m1, b2s_pick(m0, B2S_IV7, B2S_IV6 ^ f), b2s_pick(m0, B2S_IV5, B2S_IV4 ^ t));
+    b2s_quad_round<0>(m, m0, m1, a, b, c, d);
     hl = VX_XOR3(hl, a, c);
     hh = VX_XOR3(hh, b, d);
 }

@@ -35,46 +35,6 @@ struct FriTailArgs {
     DevChannel* channel;
 };
 
-// Block-wide form of narrow_reduce: n (a power of two) nodes stored node-major
-// in LDS buffer a are reduced to one, every quad of the block taking one
-// compression per level (ping-ponging with b), so a level of up to
-// THREADS / 4 compressions costs one quad-cooperative compression of latency.
-// All threads of the block must call it; returns the buffer holding the root.
-template <uint32_t THREADS>
-__device__ __forceinline__ uint32_t* block_quad_reduce(uint32_t* a, uint32_t* b, uint32_t n, uint32_t lvl,
-                                                       const LevelStore& st) {
-    constexpr uint32_t QUADS = THREADS / 4;
-    const uint32_t quad = threadIdx.x >> 2, l = threadIdx.x & 3;
-    const uint32_t wave_q0 = (threadIdx.x >> 6) * 16;  // first quad of this wave
-    while (n > 1) {
-        const uint32_t half = n / 2;
-        for (uint32_t q0 = 0; q0 < half; q0 += QUADS) {
-            if (q0 + wave_q0 >= half) continue;  // wave-uniform: no quad of this wave has a node
-            const uint32_t q = q0 + quad;
-            const uint32_t qq = q < half ? q : half - 1;  // whole quads stay active for the DPP permutes
-            uint32_t hl, hh;
-            b2s_quad_init(hl, hh);
-            b2s_compress_quad(a + 16 * qq, 64u, 0xFFFFFFFFu, hl, hh);
-            if (q < half) {
-                b[8 * q + l] = hl;
-                b[8 * q + 4 + l] = hh;
-                if (st.want(lvl + 1)) {
-                    uint32_t* dst = st.at(lvl + 1, q);
-                    dst[l] = hl;
-                    dst[4 + l] = hh;
-                }
-            }
-        }
-        __syncthreads();
-        uint32_t* t = a;
-        a = b;
-        b = t;
-        n = half;
-        lvl++;
-    }
-    return a;
-}
-
 static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTailArgs args, uint64_t bs) {
     constexpr uint32_t W = FRI_TAIL_MAX_ITEMS / 2;  // nodes of the widest level
     constexpr uint32_t QUADS = FRI_TAIL_THREADS / 4;
@@ -143,7 +103,7 @@ static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTa
             m = hm;
             lvl++;
         }
-        const uint32_t* root = block_quad_reduce<FRI_TAIL_THREADS>(cur, nxt, m, lvl, st);
+        const uint32_t* root = block_quad_reduce<FRI_TAIL_THREADS>(cur, nxt, m, lvl, 0, st);
         if (tid < 64) {  // one wave: root and transcript
             if (tid < 8) L.root_dst[boff + tid] = root[tid];
             uint32_t* adst = L.alpha_dst ? L.alpha_dst + boff : nullptr;

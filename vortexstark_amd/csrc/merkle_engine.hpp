@@ -485,6 +485,54 @@ __device__ __forceinline__ uint32_t* narrow_reduce(uint32_t* a, uint32_t* b, uin
     return a;
 }
 
+// Block-wide form of narrow_reduce: n (a power of two) nodes stored node-major
+// in LDS buffer a are reduced to one, every quad of the block taking one
+// compression per level (ping-ponging with b), so a level of up to
+// THREADS / 4 compressions costs one quad-cooperative compression of latency.
+// The nodes are at tree level lvl, node 0 having index idx0 there; it stops at
+// `stop` nodes (a power of two >= 1). All threads of the block must call it;
+// returns the buffer holding the remaining nodes.
+template <uint32_t THREADS>
+__device__ __forceinline__ uint32_t* block_quad_reduce(uint32_t* a, uint32_t* b, uint32_t& n, uint32_t& lvl,
+                                                       uint64_t& idx0, const LevelStore& st, uint32_t stop) {
+    constexpr uint32_t QUADS = THREADS / 4;
+    const uint32_t quad = threadIdx.x >> 2, l = threadIdx.x & 3;
+    const uint32_t wave_q0 = (threadIdx.x >> 6) * 16;  // first quad of this wave
+    while (n > stop) {
+        const uint32_t half = n / 2;
+        for (uint32_t q0 = 0; q0 < half; q0 += QUADS) {
+            if (q0 + wave_q0 >= half) continue;  // wave-uniform: no quad of this wave has a node
+            const uint32_t q = q0 + quad;
+            const uint32_t qq = q < half ? q : half - 1;  // whole quads stay active for the DPP permutes
+            uint32_t hl, hh;
+            b2s_quad_init(hl, hh);
+            b2s_compress_quad(a + 16 * qq, 64u, 0xFFFFFFFFu, hl, hh);
+            if (q < half) {
+                b[8 * q + l] = hl;
+                b[8 * q + 4 + l] = hh;
+                if (st.want(lvl + 1)) {
+                    uint32_t* dst = st.at(lvl + 1, (idx0 >> 1) + q);
+                    dst[l] = hl;
+                    dst[4 + l] = hh;
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t* t = a;
+        a = b;
+        b = t;
+        n = half;
+        lvl++;
+        idx0 >>= 1;
+    }
+    return a;
+}
+template <uint32_t THREADS>
+__device__ __forceinline__ uint32_t* block_quad_reduce(uint32_t* a, uint32_t* b, uint32_t n, uint32_t lvl,
+                                                       uint64_t idx0, const LevelStore& st) {
+    return block_quad_reduce<THREADS>(a, b, n, lvl, idx0, st, 1u);
+}
+
 // Fiat-Shamir step after a commitment, by one whole wave (quad 0 is the one whose
 // results are kept): optional reset, mix_digest(root) and n_felts draw_felt
 // calls into alpha_dst[4k..4k+4) (channel.rs:30-39, 67-86). root and msg are
@@ -634,6 +682,7 @@ template <class Src>
 __global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src0, uint32_t item_level, uint32_t n_items,
                                                                   LevelStore st0, TailOut out, uint64_t bs) {
     constexpr uint32_t W = MERKLE_TAIL_MAX / 2;
+    constexpr uint32_t QUADS = TAIL_THREADS / 4;
     const uint64_t boff = (uint64_t)blockIdx.y * bs;
     const Src src = src0.at(boff);
     const LevelStore st = st0.shifted(boff);
@@ -643,6 +692,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src0, uin
         if (out.alpha_dst) out.alpha_dst += boff;
     }
     __shared__ uint32_t nodes[8 * W];  // wide levels, word-major: word q of node j at q * W + j
+    __shared__ uint4 qa[4 * QUADS];    // quad levels, node-major: up to 2 * QUADS nodes
+    __shared__ uint4 qb[2 * QUADS];
     __shared__ uint32_t na[8 * 64], nb[8 * 64];
     __shared__ uint32_t msg[16];
     const uint32_t tid = threadIdx.x;
@@ -657,7 +708,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src0, uin
         for (int q = 0; q < 8; q++) nodes[q * W + j] = h[q];
     }
     __syncthreads();
-    while (n > 64) {
+    // wide levels (more compressions than the block has quads): one per thread
+    while (n > 2 * QUADS) {
         const uint32_t half = n / 2;
         uint32_t l[8], r[8];
         if (tid < half) {
@@ -679,11 +731,28 @@ __global__ __launch_bounds__(TAIL_THREADS) void merkle_tail_kernel(Src src0, uin
         __syncthreads();
         n = half;
     }
+    // levels of 32 to 256 compressions: quad-cooperative across the whole block
+    uint64_t idx0 = item_base >> (lvl - item_level);
+    uint32_t* cur = na;
+    if (n > 64) {
+        uint32_t* qaw = (uint32_t*)qa;
+        for (uint32_t j = tid; j < n; j += TAIL_THREADS)
+#pragma unroll
+            for (int q = 0; q < 8; q++) qaw[8 * j + q] = nodes[q * W + j];
+        __syncthreads();
+        cur = block_quad_reduce<TAIL_THREADS>(qaw, (uint32_t*)qb, n, lvl, idx0, st, 64u);
+    } else {
+        for (uint32_t j = tid; j < n; j += TAIL_THREADS)
+#pragma unroll
+            for (int q = 0; q < 8; q++) na[8 * j + q] = nodes[q * W + j];
+        __syncthreads();
+    }
     if (tid >= 64) return;  // one wave finishes: narrow levels and transcript
-    if (tid < n)
-        for (int q = 0; q < 8; q++) na[tid * 8 + q] = nodes[q * W + tid];
-    VX_WAVE_SYNC();
-    uint32_t* root = narrow_reduce(na, nb, n, lvl, item_base >> (lvl - item_level), st);
+    if (cur != na) {
+        for (uint32_t k = tid; k < 8 * n; k += 64) na[k] = cur[k];
+        VX_WAVE_SYNC();
+    }
+    uint32_t* root = narrow_reduce(na, nb, n, lvl, idx0, st);
     if (out.root_dst && tid < 8) out.root_dst[8ull * blockIdx.x + tid] = root[tid];
     if (out.channel) quad_transcript(out.channel, root, msg, out.init_channel != 0, out.alpha_dst, out.n_felts);
 }
