@@ -453,6 +453,10 @@ int32_t vx_server_poll(vx_server* s);
 int32_t vx_server_prewarm(vx_server* s, uint32_t log_n, uint32_t n_provers);
 /* Batches replay captured HIP graphs (one per prover and batch count). */
 int32_t vx_server_set_graphs(vx_server* s, int32_t enable);
+/* Order in which a scheduling step launches the size classes: smallest proofs
+ * first (1, the default: short, latency-bound batches are dispatched ahead of
+ * the long ones that share their hardware queue) or largest first (0). */
+int32_t vx_server_set_order(vx_server* s, int32_t smallest_first);
 /* Polls until every job has finished, successfully or not; returns 0, or -1
  * when a job failed since the previous wait (the others still complete; the
  * first such failure is the thread's recorded error). */

@@ -98,3 +98,15 @@ def test_argument_errors_are_reported_without_a_device():
     assert not lib.vx_server_create(0, 0, 80, 11) and lib.vx_get_last_error() == -2
     assert lib.vx_server_inject_fault(None, 0, 0) == -1 and lib.vx_get_last_error() == -2
     assert lib.vx_get_last_error() == 0  # reading clears it
+
+
+def test_krak_proof_size_counts_the_layout():
+    """vx_krak_proof_size (host arithmetic over the KRAK v1 layout,
+    stark_cli.rs:948-975) equals the length of the oracle's proof bytes."""
+    import vortexstark_amd as vx
+    import oracle_lib as O
+
+    for log_n, nq, last in [(2, 80, 3), (4, 80, 3), (5, 1, 1), (8, 64, 4), (10, 100, 4), (12, 4096, 10), (13, 80, 3)]:
+        assert vx.lib.vx_krak_proof_size(log_n, nq, last) == len(O.prove(5, 7, log_n, nq, last)), (log_n, nq, last)
+    assert vx.lib.vx_krak_proof_size(12, 0, 0) == vx.lib.vx_krak_proof_size(12, 80, 3)
+    assert vx.lib.vx_krak_proof_size(30, 80, 3) == 0 and vx.lib.vx_krak_proof_size(29, 80, 3) > 0

@@ -400,6 +400,11 @@ class Server:
         if lib.vx_server_set_graphs(self.handle, 1 if on else 0):
             check("vx_server_set_graphs")
 
+    def set_order(self, smallest_first: bool) -> None:
+        """Launch order of size classes in a scheduling step."""
+        if lib.vx_server_set_order(self.handle, 1 if smallest_first else 0):
+            check("vx_server_set_order")
+
     def wait_all(self, raise_on_failure: bool = True) -> bool:
         """Serves until every job has finished; a failed job does not stop the
         others. Returns True when all succeeded; raises HipError when one failed

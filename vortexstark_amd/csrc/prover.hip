@@ -468,6 +468,16 @@ void build_layout(vx_prover* p, Layout& L) {
     for (uint32_t j = 0; j < p->J; j++) decom(4, p->log_e - 1 - j, L.trees[2 + j]);
 }
 
+// Words of the KRAK v1 image build_layout makes, counted without building it.
+uint64_t layout_words(const vx_prover* p) {
+    const uint64_t nq = p->n_q;
+    auto decom = [&](uint64_t width, uint64_t depth) { return 1 + 2 * nq * width + 2 * nq * (1 + 8 * depth); };
+    uint64_t w = 5 + 16 + 1 + 8ull * p->J + 1 + (4ull << p->log_last) + 1 + nq;
+    w += decom(1, p->log_e) + decom(4, p->log_e) + 1;
+    for (uint32_t j = 0; j < p->J; j++) w += decom(4, p->log_e - 1 - j);
+    return w;
+}
+
 CPt to_cpt(HostPt h) { return {h.x, h.y}; }
 
 // Word offsets of one arena (each region 256-byte aligned).
@@ -745,6 +755,11 @@ vx_prover* vx_prover_create_batch(uint32_t log_n, uint32_t n_queries, uint32_t l
     Layout L;
     build_layout(p, L);
     p->proof_words = L.words.size();
+    if (p->proof_words != layout_words(p)) {  // the counted and the built layout must agree
+        set_error(-1, "vx_prover_create: proof layout size mismatch");
+        delete p;
+        return nullptr;
+    }
     p->off_fri_roots = L.fri_roots;
     p->off_last = L.last;
     p->off_queries = L.queries;
@@ -830,9 +845,7 @@ uint64_t vx_prover_estimate_bytes(uint32_t log_n, uint32_t n_queries, uint32_t l
     if (!prover_params(log_n, n_queries, log_last_layer, capacity)) return 0;
     vx_prover p;
     set_sizes(&p, log_n, n_queries, log_last_layer, capacity);
-    Layout L;
-    build_layout(&p, L);
-    p.proof_words = L.words.size();
+    p.proof_words = layout_words(&p);
     plan_arena(&p);
     return 4 * p.R * capacity + shared_bytes(&p);
 }
@@ -1050,11 +1063,9 @@ void vx_merkle_stage_plan_reset(void) {
 
 uint64_t vx_krak_proof_size(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer) {
     if (!prover_params(log_n, n_queries, log_last_layer, 1)) return 0;
-    vx_prover shape;  // layout only: no device resources
+    vx_prover shape;  // sizes only: no device resources
     set_sizes(&shape, log_n, n_queries, log_last_layer, 1);
-    Layout L;
-    build_layout(&shape, L);
-    return 4ull * L.words.size();
+    return 4ull * layout_words(&shape);
 }
 
 int32_t vx_prover_set_graph(vx_prover* p, int32_t enable) {

@@ -102,6 +102,11 @@ struct vx_server {
     uint64_t budget = 0, used = 0;
     uint32_t n_q = 80, log_last = 3;
     bool graphs = false;  // batches replay captured graphs
+    // Launch order within a scheduling step. Smallest first (the default): a
+    // small class's batch is short and latency-bound, so it is dispatched ahead
+    // of the long batches that share its hardware queue instead of waiting for
+    // them to finish.
+    bool smallest_first = true;
     vx::ResultRing ring;
     std::map<uint32_t, Class> classes;
     std::vector<Job> jobs;
@@ -309,8 +314,11 @@ struct vx_server {
             }
         }
         bool ring_full = false;
-        // largest proofs first: their batches are the longest, so they start first
-        for (auto it = classes.rbegin(); it != classes.rend(); ++it) {
+        // classes in the server's order (smallest or largest proofs first)
+        std::vector<std::pair<const uint32_t, Class>*> order;
+        for (auto& kv : classes) order.push_back(&kv);
+        if (!smallest_first) std::reverse(order.begin(), order.end());
+        for (auto* it : order) {
             auto& kv = *it;
             Class& c = kv.second;
             if (c.pending.empty()) continue;
@@ -429,6 +437,12 @@ int32_t vx_server_prewarm(vx_server* s, uint32_t log_n, uint32_t n_provers) {
         made++;
     }
     return made;
+}
+
+int32_t vx_server_set_order(vx_server* s, int32_t smallest_first) {
+    if (!s) return -1;
+    s->smallest_first = smallest_first != 0;
+    return 0;
 }
 
 int32_t vx_server_set_graphs(vx_server* s, int32_t enable) {

@@ -181,6 +181,12 @@ struct vx_wide {
     };
     std::vector<Group> groups;
     hipStream_t stream = nullptr;
+    // The LDE of each size group after the first runs on its own side stream,
+    // forked from and joined back into `stream` with events, so the small
+    // groups' latency-bound passes overlap the largest group's.
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_done;
+    hipEvent_t fork = nullptr;
     std::vector<void*> allocs;
     uint32_t* tw_fwd = nullptr;  // stwo flat buffer of the 2^S domain
     uint32_t* tw_inv = nullptr;
@@ -212,6 +218,10 @@ struct vx_wide {
     }
     ~vx_wide() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipStream_t x : side) (void)hipStreamSynchronize(x);
+        for (hipStream_t x : side) (void)hipStreamDestroy(x);
+        for (hipEvent_t e : side_done) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
         for (void* p : allocs) (void)hipFree(p);
         if (h_src_ptrs) (void)hipHostFree((void*)h_src_ptrs);
         if (stream) (void)hipStreamDestroy(stream);
@@ -319,6 +329,17 @@ vx_wide* vx_wide_create(uint32_t n_cols, const uint32_t* log_sizes, uint32_t log
         w->groups.back().count++;
     }
     bool ok = check(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking), "hipStreamCreate");
+    // side streams for the LDE of the smaller size groups (at most three)
+    const size_t n_side = std::min<size_t>(3, w->groups.size() - 1);
+    ok = ok && (n_side == 0 || check(hipEventCreateWithFlags(&w->fork, hipEventDisableTiming), "hipEventCreate"));
+    for (size_t k = 0; ok && k < n_side; k++) {
+        hipStream_t x = nullptr;
+        hipEvent_t e = nullptr;
+        ok = check(hipStreamCreateWithFlags(&x, hipStreamNonBlocking), "hipStreamCreate");
+        if (ok) w->side.push_back(x);
+        ok = ok && check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        if (ok) w->side_done.push_back(e);
+    }
     hipStream_t s = w->stream;
     const uint64_t half = 1ull << (S - 1);
     ok = ok && w->alloc((void**)&w->tw_fwd, 4 * half) && w->alloc((void**)&w->tw_inv, 4 * half) &&
@@ -413,17 +434,28 @@ int32_t vx_wide_commit_device(vx_wide* w, const uint32_t* const* h_cols) {
         w->last_src.assign(h_cols, h_cols + w->n_cols);
     }
     const uint64_t tc = 1ull << (S - 1);
-    // LDE: a suffix of the 2^S twiddle buffer is the buffer of every smaller domain
-    for (const auto& g : w->groups) {
+    // LDE of every size group: interpolate, then evaluate on the 2^B-times
+    // larger domain (a suffix of the 2^S twiddle buffer is the buffer of every
+    // smaller domain). The largest group runs on the main stream, every other
+    // group on a side stream of its own, forked after the column pointers are
+    // uploaded and joined before the commitment.
+    const size_t ng = w->groups.size(), ns = std::min(w->side.size(), ng - 1);
+    if (ns && !check(hipEventRecord(w->fork, s), "hipEventRecord")) return -1;
+    for (size_t k = 0; k < ns; k++)
+        if (!check(hipStreamWaitEvent(w->side[k], w->fork, 0), "hipStreamWaitEvent")) return -1;
+    for (size_t gi = 0; gi < ng; gi++) {
+        const auto& g = w->groups[ng - 1 - gi];  // groups ascend in size: the largest first
+        hipStream_t gs = gi == 0 || !ns ? s : w->side[(gi - 1) % ns];
         const NttTables ti = stwo_tables(w->tw_inv + (tc - (1ull << (g.log - 1))), g.log);
-        ntt_inverse_batch(w->d_lde_ptrs + g.first, g.count, ti, s, w->d_src_ptrs + g.first, w->src_aligned);
-    }
-    w->mark("lde:interpolate");
-    for (const auto& g : w->groups) {
+        ntt_inverse_batch(w->d_lde_ptrs + g.first, g.count, ti, gs, w->d_src_ptrs + g.first, w->src_aligned);
         const NttTables te = stwo_tables(w->tw_fwd + (tc - (1ull << (g.log + B - 1))), g.log + B);
-        ntt_forward_batch(w->d_lde_ptrs + g.first, g.count, te, s, 1ull << g.log, true);
+        ntt_forward_batch(w->d_lde_ptrs + g.first, g.count, te, gs, 1ull << g.log, true);
     }
-    w->mark("lde:evaluate");
+    for (size_t k = 0; k < ns; k++)
+        if (!check(hipEventRecord(w->side_done[k], w->side[k]), "hipEventRecord") ||
+            !check(hipStreamWaitEvent(s, w->side_done[k], 0), "hipStreamWaitEvent"))
+            return -1;
+    w->mark("lde");
     std::string phase = "lifted_commit";
     auto mk = [&](const char* what) { w->mark(phase + ":" + what); };
     const LevelStore none = {nullptr, 0, 0};
