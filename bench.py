@@ -445,6 +445,9 @@ def vx_p() -> int:
 
 def bench_batch(vx, args) -> dict:
     jobs = batch_jobs(4, 256, 10, 18, 8)
+    if args.batch_classes:  # a subset of the size classes (diagnostic runs)
+        lo, hi = (int(x) for x in args.batch_classes.split(","))
+        jobs = [j for j in jobs if lo <= j[0] <= hi]
     n_proofs = sum(len(ab) for _, ab in jobs)
     rows = sum((1 << l) * len(ab) for l, ab in jobs)
     nq, last = 80, 3
@@ -468,6 +471,7 @@ def bench_batch(vx, args) -> dict:
     srv = vx.Server(0, nq, last, result_bytes=4 << 30)
     srv.set_graphs(True)
     srv.set_order(args.batch_order == "smallest")
+    srv.set_big_batches(args.big_batches, 23)
     for l in sorted({l for l, _ in jobs}):
         srv.prewarm(l, 2)
     warm = [srv.submit(l, ab) for l, ab in jobs]
@@ -497,6 +501,11 @@ def bench_batch(vx, args) -> dict:
     log(f"batch: passes {[round(x, 4) for x in all_s]} s; host time to submit {1e3 * t_submit:.2f} ms, "
         f"first scheduling step {1e3 * t_first:.2f} ms")
     lat = [srv.latency_ms(i) for i in ids]
+    by_class = {}
+    for (l, _), x in zip(jobs, lat):
+        by_class.setdefault(l, []).append(x)
+    class_done = {str(l): round(max(v), 2) for l, v in sorted(by_class.items())}
+    log(f"batch: last job of each size class done at (ms) {class_done}")
     same = None if solo_out is None else same_all and all(
         srv.proof(i, c, l) == solo_out[k][c] for k, (i, (l, ab)) in enumerate(zip(last_ids, jobs))
         for c in range(len(ab)))
@@ -527,7 +536,7 @@ def bench_batch(vx, args) -> dict:
             "batched_s": round(t_batch, 4), "one_at_a_time_s": round(t_solo, 4) if t_solo else None,
             "batched_melem_s": round(rows / t_batch / 1e6, 2),
             "host_submit_ms": round(1e3 * t_submit, 3), "host_first_step_ms": round(1e3 * t_first, 3),
-            "batched_passes_s": [round(x, 4) for x in all_s],
+            "batched_passes_s": [round(x, 4) for x in all_s], "class_done_ms": class_done,
             "valu_bound_s": round(bound_s, 4), "batched_frac_of_valu_bound": round(bound_s / t_batch, 4),
             "job_latency_ms": quantiles(lat), "identical_to_one_at_a_time": same,
             "batches": st["batches"], "peak_device_bytes": st["peak_device_bytes"], "cpu_baseline": cpu}
@@ -905,6 +914,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-pipelined", action="store_true", help="skip the multi-stream plain measurement")
     ap.add_argument("--batch-reps", type=int, default=3, help="timed passes of the batch workload (median reported)")
+    ap.add_argument("--big-batches", type=int, default=0,
+                    help="batches of >= 2^23 rows in flight at once in the batch workload (0: no limit)")
+    ap.add_argument("--batch-classes", default="", help="lo,hi: only the jobs of log_n in [lo, hi] (diagnostics)")
     ap.add_argument("--batch-order", choices=["smallest", "largest"], default="smallest",
                     help="launch order of size classes in the server's scheduling steps")
     ap.add_argument("--no-serial", action="store_true",

@@ -457,6 +457,10 @@ int32_t vx_server_set_graphs(vx_server* s, int32_t enable);
  * first (1, the default: short, latency-bound batches are dispatched ahead of
  * the long ones that share their hardware queue) or largest first (0). */
 int32_t vx_server_set_order(vx_server* s, int32_t smallest_first);
+/* Batches of at least 2^log_rows trace rows (proofs x rows per proof) run at
+ * most max_in_flight at a time (0, the default: no limit); smaller batches are
+ * not limited. Returns 0 or -1. */
+int32_t vx_server_set_big_batches(vx_server* s, uint32_t max_in_flight, uint32_t log_rows);
 /* Polls until every job has finished, successfully or not; returns 0, or -1
  * when a job failed since the previous wait (the others still complete; the
  * first such failure is the thread's recorded error). */

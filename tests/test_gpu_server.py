@@ -197,3 +197,22 @@ def test_unsupported_protocol_is_refused_at_creation(vx):
     with pytest.raises(vx.HipError):
         srv.submit(1, [(1, 2)])
     srv.close()
+
+
+@pytest.mark.parametrize("smallest_first", [True, False])
+def test_big_batch_cap_and_order_keep_bytes(vx, smallest_first):
+    """With at most one batch of >= 2^12 rows in flight and either class
+    order, every proof still equals the oracle's and every job completes."""
+    spec = jobs_spec(21, 12, [4, 6, 8], 4)
+    srv = vx.Server(0, 80, 3)
+    srv.set_order(smallest_first)
+    srv.set_big_batches(1, 12)
+    ids = [srv.submit(log_n, ab) for log_n, ab in spec]
+    assert srv.wait_all()
+    for jid, (log_n, ab) in zip(ids, spec):
+        assert srv.status(jid) == 1
+        for c, (a, b) in enumerate(ab):
+            assert srv.proof(jid, c, log_n) == O.prove(a, b, log_n, 80, 3), (jid, c)
+    with pytest.raises(vx.HipError):
+        srv.set_big_batches(1, 41)
+    srv.close()

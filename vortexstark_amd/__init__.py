@@ -400,6 +400,12 @@ class Server:
         if lib.vx_server_set_graphs(self.handle, 1 if on else 0):
             check("vx_server_set_graphs")
 
+    def set_big_batches(self, max_in_flight: int, log_rows: int = 23) -> None:
+        """Run batches of at least 2^log_rows rows at most max_in_flight at a time (0: no limit)."""
+        if lib.vx_server_set_big_batches(self.handle, max_in_flight, log_rows):
+            check("vx_server_set_big_batches")
+            raise HipError("vx_server_set_big_batches failed")
+
     def set_order(self, smallest_first: bool) -> None:
         """Launch order of size classes in a scheduling step."""
         if lib.vx_server_set_order(self.handle, 1 if smallest_first else 0):

@@ -82,6 +82,7 @@ struct Slot {
     int64_t region = -1;
     std::vector<Item> running;
     double last_used = 0;
+    bool big = false;  // the running batch counts against the server's big-batch cap
     bool busy() const { return !running.empty(); }
 };
 
@@ -107,6 +108,17 @@ struct vx_server {
     // of the long batches that share its hardware queue instead of waiting for
     // them to finish.
     bool smallest_first = true;
+    // Batches of at least 2^big_log_rows rows (proofs x rows) in flight at once;
+    // 0: no limit. Large batches are VALU-bound on their own; run side by side
+    // they slow each other down more than they fill each other's idle time.
+    uint32_t big_cap = 0, big_log_rows = 23, big_running = 0;
+    bool is_big(const Class& c, const Slot& s, uint32_t log_n) const {
+        const uint64_t count = std::min<uint64_t>(s.capacity, c.pending.size());
+        return big_cap && (count << log_n) >= (1ull << big_log_rows);
+    }
+    bool may_launch(const Class& c, const Slot& s, uint32_t log_n) const {
+        return !is_big(c, s, log_n) || big_running < big_cap;
+    }
     vx::ResultRing ring;
     std::map<uint32_t, Class> classes;
     std::vector<Job> jobs;
@@ -200,7 +212,8 @@ struct vx_server {
     }
     // Launches up to capacity pending proofs of the class on an idle prover.
     // Returns false when the result ring has no room for even one proof.
-    bool launch(Class& c, Slot& s) {
+    bool launch(Class& c, Slot& s, uint32_t log_n) {
+        const bool big = is_big(c, s, log_n);
         uint32_t count = (uint32_t)std::min<size_t>(s.capacity, c.pending.size());
         int64_t region = -1;
         while (count && (region = ring.alloc(count * s.proof_size, count)) < 0) count /= 2;
@@ -208,6 +221,8 @@ struct vx_server {
         std::vector<uint32_t> ab(2 * count);
         s.running.clear();
         s.region = region;
+        s.big = big;
+        if (big) big_running++;
         for (uint32_t i = 0; i < count; i++) {
             s.running.push_back(c.pending.front());
             c.pending.pop_front();
@@ -235,7 +250,12 @@ struct vx_server {
         }
         return true;
     }
+    void end_batch(Slot& s) {
+        if (s.big) big_running--;
+        s.big = false;
+    }
     void fail_running(Slot& s, int32_t code) {
+        end_batch(s);
         for (const Item& it : s.running) {
             Job& j = jobs[it.job];
             j.error = code;
@@ -251,6 +271,7 @@ struct vx_server {
         if (j.error) failed_jobs++;
     }
     void retire(Slot& s) {
+        end_batch(s);
         uint8_t* base = ring.at(s.region);
         for (uint32_t i = 0; i < s.running.size(); i++) {
             const Item& it = s.running[i];
@@ -322,21 +343,28 @@ struct vx_server {
             auto& kv = *it;
             Class& c = kv.second;
             if (c.pending.empty()) continue;
+            bool held = false;  // a big batch of this class waits for the big-batch cap
             for (auto& s : c.slots)
                 if (!s->busy() && !c.pending.empty()) {
-                    if (launch(c, *s))
+                    if (!may_launch(c, *s, kv.first)) {
+                        held = true;
+                        continue;
+                    }
+                    if (launch(c, *s, kv.first))
                         acted++;
                     else
                         ring_full = true;
                 }
             // more proofs waiting than the idle provers take: add a prover if the budget allows
-            if (!c.pending.empty() && !ring_full && c.slots.size() < MAX_PROVERS_PER_CLASS) {
+            if (!c.pending.empty() && !ring_full && !held && c.slots.size() < MAX_PROVERS_PER_CLASS) {
                 const uint64_t need = estimate_bytes(kv.first);
                 if (reserve_room(need, kv.first)) {
                     auto s = std::make_unique<Slot>();
                     if (make_slot(kv.first, *s)) {
                         c.slots.push_back(std::move(s));
-                        if (launch(c, *c.slots.back()))
+                        if (!may_launch(c, *c.slots.back(), kv.first))
+                            ;  // made ahead: it takes the class's next batch once the cap allows
+                        else if (launch(c, *c.slots.back(), kv.first))
                             acted++;
                         else
                             ring_full = true;
@@ -442,6 +470,16 @@ int32_t vx_server_prewarm(vx_server* s, uint32_t log_n, uint32_t n_provers) {
 int32_t vx_server_set_order(vx_server* s, int32_t smallest_first) {
     if (!s) return -1;
     s->smallest_first = smallest_first != 0;
+    return 0;
+}
+
+int32_t vx_server_set_big_batches(vx_server* s, uint32_t max_in_flight, uint32_t log_rows) {
+    if (!s || log_rows > 40) {
+        vx::set_error(-2, "vx_server_set_big_batches: bad arguments");
+        return -1;
+    }
+    s->big_cap = max_in_flight;
+    s->big_log_rows = log_rows;
     return 0;
 }
 
