@@ -152,6 +152,10 @@ static void test_stage_plan() {
     CHECK(pick_logc(1ull << 20) == 2);
     CHECK(pick_logc(1ull << 14, 5, 64) == 2);  // 2^20 items in the launch
     CHECK(pick_logc(1ull << 20, 1) == 1);
+    // memory-heavy sources ignore the table: the default rule's depth 2 (4096 waves)
+    stage_plan_table()[21].store(5 + 1);
+    CHECK(pick_logc(1ull << 21) == 5);
+    CHECK(pick_logc(1ull << 21, 4, 1, false) == 2);
     stage_plan_table()[9].store(5 + 1);
     CHECK(pick_logc(1ull << 9) == 1);  // 2^9 items: depth 1 is the deepest with two waves
     for (uint32_t i = 0; i < STAGE_PLAN_LOGS; i++) stage_plan_table()[i].store(0);

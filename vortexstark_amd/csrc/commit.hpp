@@ -28,7 +28,7 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
         mark("tail");
         return;
     }
-    uint32_t logc = pick_logc(n_leaves, StageLogcMax<Src>::value, batch);
+    uint32_t logc = pick_logc(n_leaves, StageLogcMax<Src>::value, batch, stage_uses_plan<Src>());
     uint64_t waves = n_leaves >> (7 + logc);
     merkle_stage_kernel<Src><<<dim3(ceil_div(waves, MERKLE_STAGE_WAVES), batch), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
         src, 0, n_leaves, logc, sc.a, st, bs);

@@ -594,6 +594,14 @@ template <class S>
 struct StageLogcMax<S, std::void_t<decltype(S::STAGE_LOGC_MAX)>> {
     static constexpr uint32_t value = S::STAGE_LOGC_MAX;
 };
+// The tuned depth table is timed on single-column leaves, whose stage is
+// compute-bound; a source that caps its depth is memory-heavy and keeps the
+// occupancy rule of pick_logc (a tuned depth of 4-5 leaves it one wave per
+// SIMD, with its leaf loads exposed).
+template <class S>
+constexpr bool stage_uses_plan() {
+    return StageLogcMax<S>::value == MERKLE_LOGC_MAX;
+}
 template <class S>
 constexpr uint32_t stage_wave_lds() {
     return (StageLogcMax<S>::value + 1) * 8 * 64;

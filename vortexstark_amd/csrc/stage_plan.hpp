@@ -24,12 +24,12 @@ inline std::atomic<uint8_t>* stage_plan_table() {
 inline uint32_t floor_log2(uint64_t x) { return x ? 63u - (uint32_t)__builtin_clzll(x) : 0u; }
 
 // Depth of a stage over n items per proof for `batch` proofs: the tuned depth
-// for n * batch items when there is one, else the largest c that still leaves
-// STAGE_MIN_WAVES waves in the launch; never above logc_max, and always at
-// least two waves per proof.
-inline uint32_t pick_logc(uint64_t n, uint32_t logc_max = MERKLE_LOGC_MAX, uint32_t batch = 1) {
+// for n * batch items when there is one and use_plan is set, else the largest
+// c that still leaves STAGE_MIN_WAVES waves in the launch; never above
+// logc_max, and always at least two waves per proof.
+inline uint32_t pick_logc(uint64_t n, uint32_t logc_max = MERKLE_LOGC_MAX, uint32_t batch = 1, bool use_plan = true) {
     const uint32_t lt = floor_log2(n * batch);
-    int c = lt < STAGE_PLAN_LOGS ? (int)stage_plan_table()[lt].load(std::memory_order_relaxed) - 1 : -1;
+    int c = use_plan && lt < STAGE_PLAN_LOGS ? (int)stage_plan_table()[lt].load(std::memory_order_relaxed) - 1 : -1;
     if (c < 0) {
         c = 0;
         for (int k = (int)logc_max; k > 0; k--) {
