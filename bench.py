@@ -472,6 +472,7 @@ def bench_batch(vx, args) -> dict:
     srv.set_graphs(True)
     srv.set_order(args.batch_order == "smallest")
     srv.set_big_batches(args.big_batches, 23)
+    srv.set_stagger(not args.no_stagger)
     for l in sorted({l for l, _ in jobs}):
         srv.prewarm(l, 2)
     warm = [srv.submit(l, ab) for l, ab in jobs]
@@ -916,6 +917,9 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=3, help="timed passes of the batch workload (median reported)")
     ap.add_argument("--big-batches", type=int, default=0,
                     help="batches of >= 2^23 rows in flight at once in the batch workload (0: no limit)")
+    ap.add_argument("--no-stagger", action="store_true",
+                    help="big batches of the batch workload start at once instead of after the heavy part "
+                         "of the big batch before them")
     ap.add_argument("--batch-classes", default="", help="lo,hi: only the jobs of log_n in [lo, hi] (diagnostics)")
     ap.add_argument("--batch-order", choices=["smallest", "largest"], default="smallest",
                     help="launch order of size classes in the server's scheduling steps")

@@ -362,8 +362,14 @@ uint64_t vx_prover_proof_copy(vx_prover* p, void* host_out, uint64_t cap);
 uint64_t vx_prover_proof_copy_at(vx_prover* p, uint32_t index, void* host_out, uint64_t cap);
 /* Enqueues on `stream` (NULL: the prover's) copies of proofs 0 .. count-1 to
  * host_proofs (proof_size bytes each, back to back) and, when host_failed is
- * not NULL, each proof's transcript failure flag (non-zero: draw_felt gave up). */
+ * not NULL, each proof's transcript failure flag (non-zero: draw_felt gave up).
+ * The copies run on the DMA engines unless vx_prover_set_copy_kernel chose a
+ * copy kernel. */
 int32_t vx_prover_enqueue_copy(vx_prover* p, uint32_t count, void* host_proofs, uint32_t* host_failed, void* stream);
+/* enable = 1: vx_prover_enqueue_copy writes pinned (device-visible) host memory
+ * with a kernel on the stream instead of the DMA engines (0, the default); the
+ * bytes are the same. No reference counterpart. Returns 0 or -1. */
+int32_t vx_prover_set_copy_kernel(vx_prover* p, int32_t enable);
 /* Size in bytes of the KRAK v1 proof of a 2^log_n trace (no device needed). */
 uint64_t vx_krak_proof_size(uint32_t log_n, uint32_t n_queries, uint32_t log_last_layer);
 
@@ -457,10 +463,18 @@ int32_t vx_server_set_graphs(vx_server* s, int32_t enable);
  * first (1, the default: short, latency-bound batches are dispatched ahead of
  * the long ones that share their hardware queue) or largest first (0). */
 int32_t vx_server_set_order(vx_server* s, int32_t smallest_first);
-/* Batches of at least 2^log_rows trace rows (proofs x rows per proof) run at
- * most max_in_flight at a time (0, the default: no limit); smaller batches are
- * not limited. Returns 0 or -1. */
+/* Batches of at least 2^log_rows trace rows (proofs x rows per proof, default
+ * 2^23) are big: they run at most max_in_flight at a time (0, the default: no
+ * limit) and are staggered (vx_server_set_stagger); smaller batches are not
+ * limited. Returns 0 or -1. */
 int32_t vx_server_set_big_batches(vx_server* s, uint32_t max_in_flight, uint32_t log_rows);
+/* Staggered big batches (1, the default): a big batch's stream waits on the
+ * device until the big batch launched before it has finished its heavy part
+ * (LDE and the trace, quotient and circle-fold trees), so big batches overlap
+ * only in their latency- and PCIe-bound phases (FRI line folds, queries,
+ * decommitment, proof copy). 0: big batches start as soon as launched.
+ * No reference counterpart (scheduling of this server). Returns 0 or -1. */
+int32_t vx_server_set_stagger(vx_server* s, int32_t enable);
 /* Polls until every job has finished, successfully or not; returns 0, or -1
  * when a job failed since the previous wait (the others still complete; the
  * first such failure is the thread's recorded error). */

@@ -82,20 +82,23 @@ def test_device_bytes_equal_estimate(vx, log_n, nq, last, cap):
 
 
 @pytest.mark.gpu
-def test_enqueue_copy_pinned_and_pageable(vx):
+@pytest.mark.parametrize("copy_kernel", [False, True])
+def test_enqueue_copy_pinned_and_pageable(vx, copy_kernel):
     """vx_prover_enqueue_copy writes a batch's proofs and failure flags into
-    pinned memory with a device kernel and into other host memory with a DMA
-    copy; both give the bytes of proof_bytes."""
+    pinned memory (with the DMA engines, or with a device kernel once
+    vx_prover_set_copy_kernel chose it) and into other host memory with a DMA
+    copy; every way gives the bytes of proof_bytes."""
     import ctypes as C
 
     ab = pairs(11, 5)
     p = vx.Prover(9, 80, 3, capacity=8)
     try:
+        assert vx.lib.vx_prover_set_copy_kernel(p.handle, 1 if copy_kernel else 0) == 0
         p.prove_batch_async(ab)
         vx.synchronize_stream(p.stream)
         want = [p.proof_bytes(i) for i in range(len(ab))]
         size = p.proof_size * len(ab)
-        # pinned: device-written
+        # pinned
         hp, hf = C.c_void_p(), C.c_void_p()
         assert vx.lib.vx_malloc_host(C.byref(hp), size) == 0 and vx.lib.vx_malloc_host(C.byref(hf), 4 * len(ab)) == 0
         try:

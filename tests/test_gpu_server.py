@@ -216,3 +216,24 @@ def test_big_batch_cap_and_order_keep_bytes(vx, smallest_first):
     with pytest.raises(vx.HipError):
         srv.set_big_batches(1, 41)
     srv.close()
+
+
+@pytest.mark.parametrize("stagger", [True, False])
+def test_staggered_big_batches_keep_bytes(vx, stagger):
+    """Big batches (here >= 2^10 rows) wait for the heavy part of the big batch
+    launched before them, or start at once: proofs equal the oracle's either
+    way, over two passes (the second replays captured graphs)."""
+    spec = jobs_spec(22, 10, [5, 7, 9], 3)
+    srv = vx.Server(0, 80, 3)
+    srv.set_graphs(True)
+    srv.set_big_batches(0, 10)
+    srv.set_stagger(stagger)
+    for _ in range(2):
+        ids = [srv.submit(log_n, ab) for log_n, ab in spec]
+        assert srv.wait_all()
+        for jid, (log_n, ab) in zip(ids, spec):
+            assert srv.status(jid) == 1
+            for c, (a, b) in enumerate(ab):
+                assert srv.proof(jid, c, log_n) == O.prove(a, b, log_n, 80, 3), (jid, c)
+            srv.release(jid)
+    srv.close()

@@ -411,6 +411,13 @@ class Server:
         if lib.vx_server_set_order(self.handle, 1 if smallest_first else 0):
             check("vx_server_set_order")
 
+    def set_stagger(self, on: bool) -> None:
+        """Big batches wait on the device for the heavy part of the big batch
+        launched before them (the default), or start at once."""
+        if lib.vx_server_set_stagger(self.handle, 1 if on else 0):
+            check("vx_server_set_stagger")
+            raise HipError("vx_server_set_stagger failed")
+
     def wait_all(self, raise_on_failure: bool = True) -> bool:
         """Serves until every job has finished; a failed job does not stop the
         others. Returns True when all succeeded; raises HipError when one failed

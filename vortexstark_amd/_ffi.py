@@ -116,6 +116,7 @@ SIGNATURES = {
     "vx_prover_proof_device_at": (vp, [vp, u32]),
     "vx_prover_proof_copy_at": (u64, [vp, u32, vp, u64]),
     "vx_prover_enqueue_copy": (i32, [vp, u32, vp, vp, vp]),
+    "vx_prover_set_copy_kernel": (i32, [vp, i32]),
     "vx_krak_proof_size": (u64, [u32, u32, u32]),
     "vx_prover_destroy": (None, [vp]),
     "vx_prover_stream": (vp, [vp]),
@@ -155,6 +156,7 @@ SIGNATURES = {
     "vx_server_set_graphs": (i32, [vp, i32]),
     "vx_server_set_order": (i32, [vp, i32]),
     "vx_server_set_big_batches": (i32, [vp, u32, u32]),
+    "vx_server_set_stagger": (i32, [vp, i32]),
     "vx_server_wait_all": (i32, [vp]),
     "vx_server_inject_fault": (i32, [vp, C.c_int64, u32]),
     "vx_server_jobs": (u64, [vp]),

@@ -292,10 +292,11 @@ __global__ void replicate_kernel(const uint32_t* src, uint64_t words, uint64_t s
 }
 
 // Proofs 0 .. count-1 (words each, R words apart) and their transcript failure
-// flags, written by the device straight into pinned host memory: the copy
-// stays on the prover's compute queue, with no copy-engine hand-off. The grid
-// is small (PCIe, not the CUs, sets the pace) so that the copy of one batch
-// leaves wave slots to the kernels of other batches.
+// flags, written by the device straight into pinned host memory (opt-in, see
+// vx_prover_set_copy_kernel): the copy stays on the prover's compute queue,
+// with no copy-engine hand-off. The grid is small (PCIe, not the CUs, sets the
+// pace), but its waves still hold CU slots for the whole transfer; with many
+// batches in flight the copy engines (the default) finish the mix sooner.
 constexpr uint32_t P2H_BLOCKS = 64;
 __global__ void proofs_to_host_kernel(const uint32_t* src, uint64_t R, uint64_t words, uint32_t count, uint32_t* dst,
                                       const uint32_t* failed_src, uint32_t* failed_dst) {
@@ -352,14 +353,23 @@ struct vx_prover {
     // graph replay of the launch sequence: one instantiated graph per batch
     // count and trace alignment, at most GRAPH_CACHE of them
     static constexpr size_t GRAPH_CACHE = 32;
+    // The sequence is captured as two graphs, its heavy part (LDE and the
+    // trees over the LDE and the first FRI layer, whose stage kernels fill
+    // the device) and its light part (the FRI line folds, queries and
+    // decommitment), with heavy_done recorded between them.
     struct Captured {
         uint32_t count;
         bool aligned;  // the captured launches move trace tiles as uint4
-        hipGraph_t graph;
-        hipGraphExec_t exec;
+        hipGraph_t graph[2];
+        hipGraphExec_t exec[2];
     };
     bool use_graph = false;
     std::vector<Captured> graphs;  // most recently used last
+    // Recorded on the stream after the heavy part of every launch sequence.
+    hipEvent_t heavy_done = nullptr;
+    // vx_prover_enqueue_copy into device-visible host memory uses
+    // proofs_to_host_kernel instead of the copy engines.
+    bool copy_kernel = false;
     // Host-trace calls (vx_prover_prove_batch_host): two device slots of
     // `capacity` traces each, filled over PCIe on their own stream, so one
     // call's upload overlaps the proving of the call before it; a slot is
@@ -397,8 +407,10 @@ struct vx_prover {
         return true;
     }
     static void destroy(Captured& c) {
-        if (c.exec) (void)hipGraphExecDestroy(c.exec);
-        if (c.graph) (void)hipGraphDestroy(c.graph);
+        for (int k = 0; k < 2; k++) {
+            if (c.exec[k]) (void)hipGraphExecDestroy(c.exec[k]);
+            if (c.graph[k]) (void)hipGraphDestroy(c.graph[k]);
+        }
     }
     void drop_graphs() {
         for (Captured& c : graphs) destroy(c);
@@ -415,6 +427,7 @@ struct vx_prover {
         drop_graphs();
         for (void* p : allocs) (void)hipFree(p);
         if (call_copied) (void)hipEventDestroy(call_copied);
+        if (heavy_done) (void)hipEventDestroy(heavy_done);
         if (h_call) (void)hipHostFree(h_call);
         if (h_setup) (void)hipHostFree(h_setup);
         if (stream) (void)hipStreamDestroy(stream);
@@ -509,9 +522,10 @@ void plan_arena(vx_prover* p) {
     p->R = o;
 }
 
-// Enqueues the whole proof sequence for proofs 0 .. count-1 (trace pointers and
-// (a, b) already in d_call).
-void enqueue_prove(vx_prover* p, uint32_t count) {
+// Enqueues the heavy part of the proof sequence for proofs 0 .. count-1 (trace
+// pointers and (a, b) already in d_call): the LDE, the trace and quotient
+// trees and the circle fold with its tree.
+void enqueue_heavy(vx_prover* p, uint32_t count) {
     hipStream_t s = p->stream;
     const uint64_t N = p->N, E = p->E, R = p->R;
     p->mark("start");
@@ -538,6 +552,17 @@ void enqueue_prove(vx_prover* p, uint32_t count) {
         uint32_t* nxt = p->J > 1 ? p->alpha_f(1) : nullptr;
         commit_tree(src, E / 2, p->stores[2], TailOut{pr + p->off_fri_roots, p->ch(), nxt, 0}, sc, s, mk, count, R);
     }
+}
+
+// Enqueues the light part: the FRI line folds with their trees, the queries
+// and the decommitment.
+void enqueue_light(vx_prover* p, uint32_t count) {
+    hipStream_t s = p->stream;
+    const uint64_t E = p->E, R = p->R;
+    std::string phase;
+    auto mk = [&](const char* what) { p->mark(phase + ":" + what); };
+    Scratch sc = {p->at(p->o_scr_a), p->at(p->o_scr_b)};
+    uint32_t* pr = p->proof();
     // layers of at most FRI_TAIL_MAX_ITEMS values all go to one fri_tail_kernel launch
     uint32_t js = 1;
     while (js < p->J && (E >> (js + 1)) > FRI_TAIL_MAX_ITEMS) js++;
@@ -611,9 +636,13 @@ int32_t run_batch(vx_prover* p, uint32_t count) {
     // Graphs are kept until the prover is destroyed (destroying one waits for
     // the whole device); once GRAPH_CACHE counts are cached, other counts are
     // launched directly.
+    if (!p->heavy_done && !check(hipEventCreateWithFlags(&p->heavy_done, hipEventDisableTiming), "hipEventCreate"))
+        return -1;
     if (!p->use_graph || (it == G.end() && G.size() == vx_prover::GRAPH_CACHE)) {
         p->tl.n = 0;
-        enqueue_prove(p, count);
+        enqueue_heavy(p, count);
+        if (!check(hipEventRecord(p->heavy_done, p->stream), "hipEventRecord")) return -1;
+        enqueue_light(p, count);
         return peek_error() == 0 ? 0 : -1;
     }
     if (it != G.end()) {
@@ -621,20 +650,31 @@ int32_t run_batch(vx_prover* p, uint32_t count) {
     } else {
         const bool timing = p->tl.on;
         p->tl.on = false;  // no event records inside a captured graph
-        if (!check(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
-            return -1;
-        enqueue_prove(p, count);
-        vx_prover::Captured c = {count, p->traces_aligned, nullptr, nullptr};
-        const bool ok = check(hipStreamEndCapture(p->stream, &c.graph), "hipStreamEndCapture");
+        vx_prover::Captured c = {count, p->traces_aligned, {nullptr, nullptr}, {nullptr, nullptr}};
+        bool ok = true;
+        for (int k = 0; ok && k < 2; k++) {
+            ok = check(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            if (!ok) break;
+            if (k == 0)
+                enqueue_heavy(p, count);
+            else
+                enqueue_light(p, count);
+            ok = check(hipStreamEndCapture(p->stream, &c.graph[k]), "hipStreamEndCapture") && !peek_error() &&
+                 check(hipGraphInstantiate(&c.exec[k], c.graph[k], nullptr, nullptr, 0), "hipGraphInstantiate");
+        }
         p->tl.on = timing;
-        if (!ok || peek_error() || !check(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0),
-                                          "hipGraphInstantiate")) {
+        if (!ok) {
             vx_prover::destroy(c);
             return -1;
         }
         G.push_back(c);
     }
-    return check(hipGraphLaunch(G.back().exec, p->stream), "hipGraphLaunch") ? 0 : -1;
+    const vx_prover::Captured& c = G.back();
+    return check(hipGraphLaunch(c.exec[0], p->stream), "hipGraphLaunch") &&
+                   check(hipEventRecord(p->heavy_done, p->stream), "hipEventRecord") &&
+                   check(hipGraphLaunch(c.exec[1], p->stream), "hipGraphLaunch")
+               ? 0
+               : -1;
 }
 
 // Applies the reference defaults (80 queries, last layer 2^3) for zero
@@ -692,6 +732,10 @@ bool prover_flag_failed(vx_prover* p, uint32_t index) {
     raise_flag_kernel<<<1, 1, 0, p->stream>>>(flag);
     return check_launch("raise_flag_kernel");
 }
+// The event recorded on the prover's stream after the heavy part of its latest
+// launch sequence (the LDE and the trees of the trace, the quotient and the
+// circle fold); null before the first.
+hipEvent_t prover_heavy_done(vx_prover* p) { return p ? p->heavy_done : nullptr; }
 }  // namespace vx
 
 extern "C" {
@@ -982,12 +1026,15 @@ int32_t vx_prover_enqueue_copy(vx_prover* p, uint32_t count, void* host_proofs, 
     const uint64_t pb = 4 * p->proof_words, pitch = 4 * p->R;
     uint32_t* dev_proofs = (uint32_t*)device_view_of_host(host_proofs);
     uint32_t* dev_failed = host_failed ? (uint32_t*)device_view_of_host(host_failed) : nullptr;
-    if (dev_proofs && ((uintptr_t)dev_proofs & 3) == 0 && (!host_failed || (dev_failed && ((uintptr_t)dev_failed & 3) == 0))) {
+    if (p->copy_kernel && dev_proofs && ((uintptr_t)dev_proofs & 3) == 0 &&
+        (!host_failed || (dev_failed && ((uintptr_t)dev_failed & 3) == 0))) {
         proofs_to_host_kernel<<<P2H_BLOCKS, 256, 0, s>>>(p->proof(0), p->R, p->proof_words, count, dev_proofs,
                                                          &p->ch()->failed, dev_failed);
         return check_launch("proofs_to_host_kernel") ? 0 : -1;
     }
-    // other host memory: one strided DMA copy per field (proof i sits i * R words after proof 0)
+    // the default, and any host memory: one strided DMA copy per field (proof i
+    // sits i * R words after proof 0) on the copy engines, which leaves the CUs
+    // to the kernels of other batches
     if (!check(hipMemcpy2DAsync(host_proofs, pb, p->proof(0), pitch, pb, count, hipMemcpyDeviceToHost, s), "D2H"))
         return -1;
     if (host_failed &&
@@ -1073,6 +1120,15 @@ int32_t vx_prover_set_graph(vx_prover* p, int32_t enable) {
     if (!check(hipStreamSynchronize(p->stream), "vx_prover_set_graph")) return -1;
     p->use_graph = enable != 0;
     if (!p->use_graph) p->drop_graphs();
+    return 0;
+}
+
+int32_t vx_prover_set_copy_kernel(vx_prover* p, int32_t enable) {
+    if (!p) {
+        set_error(-2, "vx_prover_set_copy_kernel: null prover");
+        return -1;
+    }
+    p->copy_kernel = enable != 0;
     return 0;
 }
 

@@ -97,6 +97,9 @@ namespace vx {
 // Raises the transcript failure flag of proof `index` of the prover's last
 // launch sequence, on the prover's stream (defined with the prover).
 bool prover_flag_failed(vx_prover* p, uint32_t index);
+// The event recorded after the heavy part (LDE, trace, quotient and circle-fold
+// trees) of the prover's latest launch sequence (defined with the prover).
+hipEvent_t prover_heavy_done(vx_prover* p);
 }  // namespace vx
 
 struct vx_server {
@@ -111,6 +114,7 @@ struct vx_server {
     // Batches of at least 2^big_log_rows rows (proofs x rows) in flight at once;
     // 0: no limit. Large batches are VALU-bound on their own; run side by side
     // they slow each other down more than they fill each other's idle time.
+    // The same threshold marks the batches that are staggered (below).
     uint32_t big_cap = 0, big_log_rows = 23, big_running = 0;
     bool is_big(const Class& c, const Slot& s, uint32_t log_n) const {
         const uint64_t count = std::min<uint64_t>(s.capacity, c.pending.size());
@@ -119,6 +123,17 @@ struct vx_server {
     bool may_launch(const Class& c, const Slot& s, uint32_t log_n) const {
         return !is_big(c, s, log_n) || big_running < big_cap;
     }
+    // Staggered big batches (on by default): the stream of a batch of at least
+    // 2^big_log_rows rows first waits, on the device, for the heavy part of the
+    // big batch launched before it. Their VALU-bound stage kernels then run one
+    // batch after another at full speed, and consecutive big batches overlap
+    // only where one of them is latency- or PCIe-bound (its FRI line folds,
+    // queries, decommitment and proof copy), instead of all of them reaching
+    // those phases together at the end of a pass. Waits follow launch order,
+    // so they cannot form a cycle.
+    bool stagger = true;
+    hipEvent_t last_heavy = nullptr;  // owned by the prover of the latest big batch
+    vx_prover* last_heavy_prover = nullptr;
     vx::ResultRing ring;
     std::map<uint32_t, Class> classes;
     std::vector<Job> jobs;
@@ -157,6 +172,10 @@ struct vx_server {
     }
     void release(Slot& s) {
         if (s.prover) {
+            if (s.prover == last_heavy_prover) {
+                last_heavy = nullptr;
+                last_heavy_prover = nullptr;
+            }
             (void)hipStreamSynchronize((hipStream_t)vx_prover_stream(s.prover));
             vx_prover_destroy(s.prover);
             used -= s.bytes;
@@ -230,7 +249,14 @@ struct vx_server {
             ab[2 * i + 1] = s.running[i].b;
         }
         hipStream_t st = (hipStream_t)vx_prover_stream(s.prover);
-        bool ok = vx_prove_fibonacci_batch(s.prover, ab.data(), count) == 0;
+        const bool heavy = stagger && ((uint64_t)count << log_n) >= (1ull << big_log_rows);
+        bool ok = true;
+        if (heavy && last_heavy) ok = vx::check(hipStreamWaitEvent(st, last_heavy, 0), "hipStreamWaitEvent");
+        ok = ok && vx_prove_fibonacci_batch(s.prover, ab.data(), count) == 0;
+        if (heavy && ok) {
+            last_heavy = vx::prover_heavy_done(s.prover);
+            last_heavy_prover = s.prover;
+        }
         // an injected fault takes the path of a device-side failure: the proof's
         // transcript flag is raised after its proving and read back with the proofs
         for (uint32_t i = 0; ok && i < count; i++) {
@@ -480,6 +506,19 @@ int32_t vx_server_set_big_batches(vx_server* s, uint32_t max_in_flight, uint32_t
     }
     s->big_cap = max_in_flight;
     s->big_log_rows = log_rows;
+    return 0;
+}
+
+int32_t vx_server_set_stagger(vx_server* s, int32_t enable) {
+    if (!s) {
+        vx::set_error(-2, "vx_server_set_stagger: null server");
+        return -1;
+    }
+    s->stagger = enable != 0;
+    if (!s->stagger) {
+        s->last_heavy = nullptr;
+        s->last_heavy_prover = nullptr;
+    }
     return 0;
 }
 
