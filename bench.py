@@ -467,12 +467,15 @@ def bench_batch(vx, args) -> dict:
             p.close()
         log(f"batch: {n_proofs} proofs one at a time: {t_solo:.3f}s ({n_proofs / t_solo:.1f} proofs/s)")
     # batched: all jobs submitted at once; provers made ahead (vx_server_prewarm, two per
-    # size class) and graphs captured by a first pass, so the timed pass sets nothing up
+    # size class) and graphs captured by a first pass, so the timed pass sets nothing up;
+    # big batches staggered (the whole list arrives at once, so they would otherwise all
+    # reach their FRI chains and proof copies together at the end)
     srv = vx.Server(0, nq, last, result_bytes=4 << 30)
     srv.set_graphs(True)
     srv.set_order(args.batch_order == "smallest")
     srv.set_big_batches(args.big_batches, 23)
     srv.set_stagger(not args.no_stagger)
+    srv.set_copy_kernel(args.copy_kernel)
     for l in sorted({l for l, _ in jobs}):
         srv.prewarm(l, 2)
     warm = [srv.submit(l, ab) for l, ab in jobs]
@@ -540,7 +543,10 @@ def bench_batch(vx, args) -> dict:
             "batched_passes_s": [round(x, 4) for x in all_s], "class_done_ms": class_done,
             "valu_bound_s": round(bound_s, 4), "batched_frac_of_valu_bound": round(bound_s / t_batch, 4),
             "job_latency_ms": quantiles(lat), "identical_to_one_at_a_time": same,
-            "batches": st["batches"], "peak_device_bytes": st["peak_device_bytes"], "cpu_baseline": cpu}
+            "batches": st["batches"], "peak_device_bytes": st["peak_device_bytes"],
+            "scheduling": {"order": args.batch_order, "staggered_big_batches": not args.no_stagger,
+                           "proof_copies": "kernel" if args.copy_kernel else "dma", "graphs": True},
+            "cpu_baseline": cpu}
 
 
 def proof_compressions(log_n: int, log_last: int) -> int:
@@ -584,6 +590,8 @@ def bench_serve(vx, args) -> dict:
         p.close()
     log(f"serve: {n_proofs} proofs one at a time: {t_serial:.3f}s")
     srv = vx.Server(288 << 30, nq, last, result_bytes=8 << 30)
+    srv.set_stagger(args.serve_stagger)
+    srv.set_copy_kernel(args.copy_kernel)
     # warm-up: one single-column job per size creates the provers (outside the replays)
     for l in sizes:
         srv.submit(l, [(1, 1)])
@@ -918,8 +926,12 @@ def main() -> int:
     ap.add_argument("--big-batches", type=int, default=0,
                     help="batches of >= 2^23 rows in flight at once in the batch workload (0: no limit)")
     ap.add_argument("--no-stagger", action="store_true",
-                    help="big batches of the batch workload start at once instead of after the heavy part "
-                         "of the big batch before them")
+                    help="big batches of the batch workload (all jobs submitted at once) start at once instead "
+                         "of after the heavy part of the big batch before them")
+    ap.add_argument("--serve-stagger", action="store_true",
+                    help="stagger the big batches of the serve workload too (off: arrivals already spread them)")
+    ap.add_argument("--copy-kernel", action="store_true",
+                    help="the batch and serve workloads copy proofs with a kernel instead of the DMA engines")
     ap.add_argument("--batch-classes", default="", help="lo,hi: only the jobs of log_n in [lo, hi] (diagnostics)")
     ap.add_argument("--batch-order", choices=["smallest", "largest"], default="smallest",
                     help="launch order of size classes in the server's scheduling steps")

@@ -468,13 +468,18 @@ int32_t vx_server_set_order(vx_server* s, int32_t smallest_first);
  * limit) and are staggered (vx_server_set_stagger); smaller batches are not
  * limited. Returns 0 or -1. */
 int32_t vx_server_set_big_batches(vx_server* s, uint32_t max_in_flight, uint32_t log_rows);
-/* Staggered big batches (1, the default): a big batch's stream waits on the
- * device until the big batch launched before it has finished its heavy part
- * (LDE and the trace, quotient and circle-fold trees), so big batches overlap
- * only in their latency- and PCIe-bound phases (FRI line folds, queries,
- * decommitment, proof copy). 0: big batches start as soon as launched.
- * No reference counterpart (scheduling of this server). Returns 0 or -1. */
+/* Staggered big batches (1): a big batch's stream waits on the device until
+ * the big batch launched before it has finished its heavy part (LDE and the
+ * trace, quotient and circle-fold trees), so big batches overlap only in their
+ * latency- and PCIe-bound phases (FRI line folds, queries, decommitment, proof
+ * copy); this suits a job list submitted at once. 0 (the default): big batches
+ * start as soon as launched. No reference counterpart (scheduling of this
+ * server). Returns 0 or -1. */
 int32_t vx_server_set_stagger(vx_server* s, int32_t enable);
+/* Proofs reach the result ring through a copy kernel on each batch's stream
+ * (1) or the DMA engines (0, the default); see vx_prover_set_copy_kernel. No
+ * reference counterpart. Returns 0 or -1. */
+int32_t vx_server_set_copy_kernel(vx_server* s, int32_t enable);
 /* Polls until every job has finished, successfully or not; returns 0, or -1
  * when a job failed since the previous wait (the others still complete; the
  * first such failure is the thread's recorded error). */

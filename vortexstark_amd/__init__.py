@@ -418,6 +418,13 @@ class Server:
             check("vx_server_set_stagger")
             raise HipError("vx_server_set_stagger failed")
 
+    def set_copy_kernel(self, on: bool) -> None:
+        """Proofs reach the result ring through a copy kernel (True) or the DMA
+        engines (False, the default)."""
+        if lib.vx_server_set_copy_kernel(self.handle, 1 if on else 0):
+            check("vx_server_set_copy_kernel")
+            raise HipError("vx_server_set_copy_kernel failed")
+
     def wait_all(self, raise_on_failure: bool = True) -> bool:
         """Serves until every job has finished; a failed job does not stop the
         others. Returns True when all succeeded; raises HipError when one failed

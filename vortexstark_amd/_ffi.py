@@ -157,6 +157,7 @@ SIGNATURES = {
     "vx_server_set_order": (i32, [vp, i32]),
     "vx_server_set_big_batches": (i32, [vp, u32, u32]),
     "vx_server_set_stagger": (i32, [vp, i32]),
+    "vx_server_set_copy_kernel": (i32, [vp, i32]),
     "vx_server_wait_all": (i32, [vp]),
     "vx_server_inject_fault": (i32, [vp, C.c_int64, u32]),
     "vx_server_jobs": (u64, [vp]),

@@ -106,6 +106,7 @@ struct vx_server {
     uint64_t budget = 0, used = 0;
     uint32_t n_q = 80, log_last = 3;
     bool graphs = false;  // batches replay captured graphs
+    bool copy_kernel = false;  // proofs reach the ring through a kernel instead of the DMA engines
     // Launch order within a scheduling step. Smallest first (the default): a
     // small class's batch is short and latency-bound, so it is dispatched ahead
     // of the long batches that share its hardware queue instead of waiting for
@@ -123,15 +124,18 @@ struct vx_server {
     bool may_launch(const Class& c, const Slot& s, uint32_t log_n) const {
         return !is_big(c, s, log_n) || big_running < big_cap;
     }
-    // Staggered big batches (on by default): the stream of a batch of at least
+    // Staggered big batches (off by default): the stream of a batch of at least
     // 2^big_log_rows rows first waits, on the device, for the heavy part of the
     // big batch launched before it. Their VALU-bound stage kernels then run one
     // batch after another at full speed, and consecutive big batches overlap
     // only where one of them is latency- or PCIe-bound (its FRI line folds,
     // queries, decommitment and proof copy), instead of all of them reaching
     // those phases together at the end of a pass. Waits follow launch order,
-    // so they cannot form a cycle.
-    bool stagger = true;
+    // so they cannot form a cycle. This pays when a whole job list is submitted
+    // at once (the big batches would otherwise finish together); under a
+    // stream of arrivals, where big batches start at different times anyway,
+    // the waits only delay them.
+    bool stagger = false;
     hipEvent_t last_heavy = nullptr;  // owned by the prover of the latest big batch
     vx_prover* last_heavy_prover = nullptr;
     vx::ResultRing ring;
@@ -191,6 +195,7 @@ struct vx_server {
         s.prover = p;
         s.capacity = cap;
         if (graphs) vx_prover_set_graph(p, 1);
+        if (copy_kernel) vx_prover_set_copy_kernel(p, 1);
         s.bytes = vx_prover_device_bytes(p);
         s.proof_size = vx_prover_proof_size(p);
         used += s.bytes;
@@ -519,6 +524,18 @@ int32_t vx_server_set_stagger(vx_server* s, int32_t enable) {
         s->last_heavy = nullptr;
         s->last_heavy_prover = nullptr;
     }
+    return 0;
+}
+
+int32_t vx_server_set_copy_kernel(vx_server* s, int32_t enable) {
+    if (!s) {
+        vx::set_error(-2, "vx_server_set_copy_kernel: null server");
+        return -1;
+    }
+    s->copy_kernel = enable != 0;
+    for (auto& kv : s->classes)
+        for (auto& sl : kv.second.slots)
+            if (sl->prover && vx_prover_set_copy_kernel(sl->prover, enable)) return -1;
     return 0;
 }
 
