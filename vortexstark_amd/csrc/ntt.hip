@@ -149,9 +149,37 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
             if (FWD && R == 3 && t >= 1 && zero_top) continue;
             const uint32_t rb = J + t;                 // row bit of the layer
             const uint32_t* T = a.tw[rb];
-            const bool circle0 = STWO && a.s + rb == 0;
+            // only the contiguous pass (stride bits from 0) has a stwo variant,
+            // so the circle layer is row bit 0, known at compile time
+            const bool circle0 = STWO && rb == 0;
             // h(d) = (hi << (M - rb - 1)) + (row >> (rb + 1)) = hb + (d >> (t + 1))
             const uint32_t hb = (uint32_t)(hi << (M - rb - 1)) + (row_hi << (R - t - 1));
+            if (STWO && R == 3 && J == 0 && t == 0) {
+                // the circle layer in a radix-8 round: its four butterflies
+                // (d = 0, 2, 4, 6) take h = hb .. hb + 3 with hb a multiple of
+                // 4, i.e. [y, -y, -x, x] of the pair at hb / 2; a negated
+                // twiddle swaps the two outputs (forward) or negates the
+                // difference (inverse), so no negation is computed
+                const uint2 xy = *(const uint2*)(T + (hb >> 1));
+#pragma unroll
+                for (int d = 0; d < 8; d += 2) {
+                    const uint32_t w = d < 4 ? xy.y : xy.x;
+                    const bool neg = d == 2 || d == 4;
+                    uint32_t& x0 = v[d];
+                    uint32_t& x1 = v[d + 1];
+                    if (FWD) {
+                        const uint32_t m = mmul(x1, w);
+                        const uint32_t s0 = madd(x0, m), s1 = msub(x0, m);
+                        x0 = neg ? s1 : s0;
+                        x1 = neg ? s0 : s1;
+                    } else {
+                        const uint32_t s0 = madd(x0, x1);
+                        x1 = mmul(neg ? msub(x1, x0) : msub(x0, x1), w);
+                        x0 = s0;
+                    }
+                }
+                continue;
+            }
 #pragma unroll
             for (int d = 0; d < (1 << R); d++) {
                 if (d & (1 << t)) continue;
@@ -202,6 +230,7 @@ __device__ __forceinline__ void inv_rounds(uint32_t* tile, const NttPassArgs& a,
 // words, or the contiguous tile of the first pass) when the buffers allow it.
 template <bool FWD, int M, int C, bool STWO>
 __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
+    static_assert(!STWO || C == 0, "the stwo variant is the contiguous pass (stride bits from 0)");
     __shared__ uint32_t tile[(1u << (M + C)) + (1u << (M + C)) / 8];
     uint32_t* dst = a.cols ? a.cols[blockIdx.y] : a.dst;
     const uint32_t* src = a.cols ? (a.srcs ? a.srcs[blockIdx.y] : a.cols[blockIdx.y]) : a.src;
