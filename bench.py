@@ -57,9 +57,10 @@ VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 # VALU ops of one Blake2s compression on gfx950: 10 rounds x 8 G x 12 ops (v_add3,
 # v_add, v_xor, v_alignbit) + 8 three-input xors of the feed-forward (v_bitop3)
 OPS_PER_COMPRESSION = 10 * 8 * 12 + 8
-# one radix-2 M31 butterfly: product (v_mad_u64_u32, alignbit, and, add, sub,
-# min), modular add (add, sub, min) and modular subtract (sub, add, sub, min)
-OPS_PER_BUTTERFLY = 13
+# one radix-2 M31 butterfly in the pass kernel's gfx950 ISA: product
+# (v_mad_u64_u32, alignbit, and, add, add of -p, min), modular add (add, add of
+# -p, min) and modular subtract (sub, add of p, min)
+OPS_PER_BUTTERFLY = 12
 # VALU ops of one nonce of the proof-of-work search for pow_bits <= 32 (the loop
 # body of grind_kernel in its gfx950 ISA: 279 v_xor, 279 v_alignbit, 280 adds
 # and the check): the uniform part of round 0 runs once per wave and only digest
