@@ -66,6 +66,9 @@ static void test_plans() {
     CHECK(plan_valid(16, make({{4, 0}, {6, 4}, {6, 5}})));
     CHECK(!plan_valid(16, make({{2, 0}, {7, 4}, {7, 5}})));  // c = 4 pass starting at stride 2^2
     CHECK(!plan_valid(16, make({{8, 0}, {8, 3}})));          // no kernel for 8-word segments
+    CHECK(plan_valid(16, make({{8, 0}, {8, 6}})));           // 256-byte segments, 8 layers
+    CHECK(!plan_valid(16, make({{7, 0}, {9, 6}})));          // strided pass too deep (c = 6)
+    CHECK(!plan_valid(16, make({{5, 0}, {5, 6}, {6, 5}})));  // c = 6 pass starting at stride 2^5
     CHECK(!plan_valid(16, make({{8, 0}, {0, 5}, {8, 5}})));  // empty pass
 }
 

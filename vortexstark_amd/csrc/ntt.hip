@@ -10,7 +10,7 @@
 // cut into a few passes, each applying several consecutive layers to tiles
 // held in LDS. The pass holding the low strides works on contiguous tiles of
 // up to 2^14 words (64 KB LDS); passes over higher strides gather 2^m rows of
-// 2^c contiguous words (c = 5: 128-byte segments) so every HBM access stays
+// 2^c contiguous words (c = 4, 5, 6: 64- to 256-byte segments) so every HBM access stays
 // coalesced. A 2^26-point evaluate is 3 passes instead of 26.
 //
 // The same pass kernel runs the stwo-basis transform (cuda/circle_ntt_stwo.cu):
@@ -328,6 +328,8 @@ static void launch_pass_dir(bool stwo, const NttPassArgs& a, uint32_t m, uint32_
             dispatch_m<FWD, 0, NTT_LOW_MAX, false>(m, a, k, n_cols, st);
     } else if (c == 4) {
         dispatch_m<FWD, 4, NTT_HIGH_MAX4, false>(m, a, k, n_cols, st);
+    } else if (c == 6) {
+        dispatch_m<FWD, 6, NTT_HIGH_MAX6, false>(m, a, k, n_cols, st);
     } else {
         dispatch_m<FWD, 5, NTT_HIGH_MAX5, false>(m, a, k, n_cols, st);
     }

@@ -11,9 +11,11 @@ namespace vx {
 
 // Pass shapes. The contiguous pass (c = 0) holds up to 2^14 words in LDS;
 // strided passes move 2^c contiguous words per row (c = 4: 64-byte, c = 5:
-// 128-byte segments) of up to 2^14-word tiles. Which split a size uses comes
-// from the plan table: the default below, or what the autotuner measured.
+// 128-byte, c = 6: 256-byte segments) of up to 2^14-word tiles. Which split a
+// size uses comes from the plan table: the default below, or what the
+// autotuner measured.
 constexpr uint32_t NTT_LOW_MAX = 14;   // most layers of the contiguous pass
+constexpr uint32_t NTT_HIGH_MAX6 = 8;  // most layers of a strided pass, c = 6
 constexpr uint32_t NTT_HIGH_MAX5 = 9;  // most layers of a strided pass, c = 5
 constexpr uint32_t NTT_HIGH_MAX4 = 10; // most layers of a strided pass, c = 4
 constexpr uint32_t NTT_LOW = 12;       // default split: contiguous pass
@@ -51,17 +53,18 @@ inline bool plan_valid(uint32_t k, const std::vector<Pass>& ps) {
     for (const Pass& p : ps) {
         if (p.s != s || p.m == 0) return false;
         if (p.c == 0 && (p.s != 0 || p.m > NTT_LOW_MAX)) return false;
+        if (p.c == 6 && (p.m > NTT_HIGH_MAX6 || p.s < 6)) return false;
         if (p.c == 5 && (p.m > NTT_HIGH_MAX5 || p.s < 5)) return false;
         if (p.c == 4 && (p.m > NTT_HIGH_MAX4 || p.s < 4)) return false;
-        if (p.c != 0 && p.c != 4 && p.c != 5) return false;
+        if (p.c != 0 && p.c != 4 && p.c != 5 && p.c != 6) return false;
         s += p.m;
     }
     return s == k;
 }
 
 // Candidate splits of a size-2^k transform for the autotuner: every contiguous
-// pass depth that leaves at most three strided passes, with 64- or 128-byte
-// row segments and the fewest (or one more) balanced strided passes.
+// pass depth that leaves at most three strided passes, with 64-, 128- or
+// 256-byte row segments and the fewest (or one more) balanced strided passes.
 inline std::vector<std::vector<Pass>> plan_candidates(uint32_t k) {
     std::vector<std::vector<Pass>> out;
     auto add = [&](const std::vector<Pass>& ps) {
@@ -80,8 +83,8 @@ inline std::vector<std::vector<Pass>> plan_candidates(uint32_t k) {
             add({{0, m0, 0}});
             continue;
         }
-        for (uint32_t c : {4u, 5u}) {
-            const uint32_t hmax = c == 4 ? NTT_HIGH_MAX4 : NTT_HIGH_MAX5;
+        for (uint32_t c : {4u, 5u, 6u}) {
+            const uint32_t hmax = c == 4 ? NTT_HIGH_MAX4 : c == 5 ? NTT_HIGH_MAX5 : NTT_HIGH_MAX6;
             const uint32_t npmin = (R + hmax - 1) / hmax;
             for (uint32_t np = npmin; np <= npmin + 1 && np <= 7 && np <= R; np++) {
                 std::vector<Pass> ps = {{0, m0, 0}};
