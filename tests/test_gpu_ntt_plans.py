@@ -11,10 +11,12 @@ pytestmark = pytest.mark.gpu
 P = (1 << 31) - 1
 
 PLANS = {
-    12: [[(12, 0)], [(8, 0), (4, 4)], [(6, 0), (6, 5)], [(4, 0), (4, 4), (4, 4)]],
-    16: [[(14, 0), (2, 4)], [(9, 0), (7, 5)], [(8, 0), (8, 4)], [(10, 0), (3, 5), (3, 5)]],
-    20: [[(14, 0), (6, 5)], [(10, 0), (10, 4)], [(11, 0), (9, 5)], [(8, 0), (6, 4), (6, 5)], [(13, 0), (7, 4)]],
-    23: [[(14, 0), (9, 5)], [(13, 0), (10, 4)], [(12, 0), (6, 5), (5, 4)], [(9, 0), (7, 5), (7, 5)]],
+    12: [[(12, 0)], [(8, 0), (4, 4)], [(6, 0), (6, 5)], [(4, 0), (4, 4), (4, 4)], [(6, 0), (6, 6)]],
+    16: [[(14, 0), (2, 4)], [(9, 0), (7, 5)], [(8, 0), (8, 4)], [(10, 0), (3, 5), (3, 5)], [(8, 0), (8, 6)]],
+    20: [[(14, 0), (6, 5)], [(10, 0), (10, 4)], [(11, 0), (9, 5)], [(8, 0), (6, 4), (6, 5)], [(13, 0), (7, 4)],
+         [(12, 0), (8, 6)], [(6, 0), (7, 6), (7, 5)]],
+    23: [[(14, 0), (9, 5)], [(13, 0), (10, 4)], [(12, 0), (6, 5), (5, 4)], [(9, 0), (7, 5), (7, 5)],
+         [(11, 0), (6, 6), (6, 6)]],
 }
 
 
