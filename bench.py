@@ -960,6 +960,9 @@ def main() -> int:
     skip = {x for x in args.skip.split(",") if x}
     MEASURED["valu_peak"] = vx.measure_valu_peak()
     log(f"measured int32 VALU peak: {MEASURED['valu_peak']:.2f} T lane-ops/s (nominal {VALU_PEAK_TOPS:.2f})")
+    MEASURED["hash_latency_ns"] = dict(zip(("quad", "lane"), (round(x, 1) for x in vx.measure_hash_latency())))
+    log(f"Merkle node hash latency on one wave: {MEASURED['hash_latency_ns']['quad']:.0f} ns quad-cooperative, "
+        f"{MEASURED['hash_latency_ns']['lane']:.0f} ns one lane")
     if not args.no_tune:
         t = time.time()
         MEASURED["ntt_plans"] = {str(k): {"ms": round(ms, 4), "passes": plan}
@@ -1021,6 +1024,7 @@ def main() -> int:
         "parity_vs_oracle": plain["parity"],
         "cpu_baseline": plain["cpu"],
         "valu_peak_measured": round(MEASURED["valu_peak"], 3),
+        "hash_latency_ns": MEASURED.get("hash_latency_ns"),
         "hip_hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
         "merkle_stage_plans": MEASURED.get("merkle_stage_plans"),
         "ntt_plans": MEASURED.get("ntt_plans"),

@@ -262,7 +262,7 @@ void* vx_get_stream(void);
 /* Pass plans of the circle NTT (native and stwo basis). A transform of 2^log_n
  * points runs as passes over LDS tiles: a contiguous pass of m[0] layers, then
  * strided passes of m[i] layers moving 2^c[i] contiguous words per row
- * (c in {4, 5}). vx_ntt_autotune times every candidate split of each size on
+ * (c in {4, 5, 6}). vx_ntt_autotune times every candidate split of each size on
  * the device and installs the fastest for this process; plan_set installs a
  * given one (n_passes = 0: the default split); plan_get reports the one in use
  * (returns the number of passes). Any plan gives bit-identical results. */
@@ -270,6 +270,12 @@ int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms);
 /* Measured int32 VALU issue rate of the current device in T lane-ops/s (a
  * grid of v_add3 / v_xor / v_alignbit chains, the instructions of Blake2s). */
 int32_t vx_measure_valu_peak(double* tops, double* ms);
+/* Measured latency of one Merkle node hash (a Blake2s compression of two
+ * children) in a chain of dependent nodes on a single wave, in ns: four lanes
+ * per compression as the narrow tree levels and the transcript run it
+ * (*ns_quad), or one lane per compression (*ns_lane). No reference
+ * counterpart (a device calibration). Returns 0 or -1. */
+int32_t vx_measure_hash_latency(double* ns_quad, double* ns_lane);
 int32_t vx_ntt_plan_set(uint32_t log_n, uint32_t n_passes, const uint32_t* m, const uint32_t* c);
 uint32_t vx_ntt_plan_get(uint32_t log_n, uint32_t* m, uint32_t* c);
 void vx_ntt_plan_reset(void);

@@ -619,3 +619,13 @@ def measure_valu_peak() -> float:
         check("vx_measure_valu_peak")
         raise HipError("vx_measure_valu_peak failed")
     return tops.value
+
+
+def measure_hash_latency() -> tuple:
+    """(quad-cooperative, one-lane) latency of one Merkle node hash in a chain
+    of dependent nodes on one wave, in ns (microbenchmark)."""
+    q, l = C.c_double(0.0), C.c_double(0.0)
+    if lib.vx_measure_hash_latency(C.byref(q), C.byref(l)):
+        check("vx_measure_hash_latency")
+        raise HipError("vx_measure_hash_latency failed")
+    return q.value, l.value

@@ -95,6 +95,7 @@ SIGNATURES = {
     "vx_event_destroy": (i32, [vp]),
     "vx_ntt_autotune": (i32, [u32, u32, vp]),
     "vx_measure_valu_peak": (i32, [vp, vp]),
+    "vx_measure_hash_latency": (i32, [vp, vp]),
     "vx_ntt_plan_set": (i32, [u32, u32, vp, vp]),
     "vx_ntt_plan_get": (u32, [u32, vp, vp]),
     "vx_ntt_plan_reset": (None, []),

@@ -134,15 +134,19 @@ constexpr uint8_t B2S_SIGMA[10][16] = {
     {6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5}, {10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0},
 };
 
-// value from lane (l + k) % 4 of the same quad
+// value from lane (l + k) % 4 of the same quad. A quad permute reads only lanes
+// of its own quad, so no lane ever reads an invalid source and bound_ctrl
+// changes nothing in the result; set, it lets the compiler fold the move into
+// the add or xor that consumes it (a DPP operand), saving about 40 of the ~450
+// instructions of a compression.
 __device__ __forceinline__ uint32_t quad_rot1(uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x39, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x39, 0xF, 0xF, true);
 }
 __device__ __forceinline__ uint32_t quad_rot2(uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, true);
 }
 __device__ __forceinline__ uint32_t quad_rot3(uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x93, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x93, 0xF, 0xF, true);
 }
 
 // mask ? a : b for a mask of all ones or all zeros: one v_bfi_b32
