@@ -284,7 +284,8 @@ def bench_plain(vx, args) -> dict:
         roofline_hbm = {"bound": "hbm", "kernel": "ntt_pass_kernel<true> (evaluate 2^%d, all passes)" % (log_n + 2),
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get("lde:evaluate"),
-                        "phase_ms": round(ntt_ms, 4), "passes": MEASURED.get("ntt_plans", {}).get(str(log_n + 2))}
+                        "phase_ms": round(ntt_ms, 4),
+                        "passes": (MEASURED.get("ntt_plans", {}).get(str(log_n + 2)) or {}).get("lde_passes")}
     # the same evaluate priced against the VALU: its butterflies are the work
     # (the top two layers of a zero-padded input only copy and are not counted)
     roofline_ntt = None
@@ -965,9 +966,11 @@ def main() -> int:
         f"{MEASURED['hash_latency_ns']['lane']:.0f} ns one lane")
     if not args.no_tune:
         t = time.time()
-        MEASURED["ntt_plans"] = {str(k): {"ms": round(ms, 4), "passes": plan}
+        MEASURED["ntt_plans"] = {str(k): {"ms": round(ms, 4), "passes": plan, "lde_passes": vx.ntt_lde_plan(k)}
                                  for k, (ms, plan) in vx.ntt_autotune(10, 28).items()}
-        log(f"ntt autotune 2^10..2^28: {time.time() - t:.1f}s")
+        log(f"ntt autotune 2^10..2^28: {time.time() - t:.1f}s; blowup-4 evaluation plans (layers, segment, "
+            f"replicating): " + ", ".join(f"2^{k}: {v['lde_passes']}" for k, v in MEASURED["ntt_plans"].items()
+                                          if int(k) in (16, 20, 22, 26)))
         t = time.time()
         MEASURED["merkle_stage_plans"] = {str(k): {"ms": round(ms, 4), "logc": c}
                                           for k, (ms, c) in vx.merkle_autotune(12, 27).items()}

@@ -265,7 +265,14 @@ void* vx_get_stream(void);
  * (c in {4, 5, 6}). vx_ntt_autotune times every candidate split of each size on
  * the device and installs the fastest for this process; plan_set installs a
  * given one (n_passes = 0: the default split); plan_get reports the one in use
- * (returns the number of passes). Any plan gives bit-identical results. */
+ * (returns the number of passes). Any plan gives bit-identical results.
+ * Evaluations whose input fills only the first quarter (the blowup-4 LDE of
+ * the provers) have a table of their own, tuned by the same call on such
+ * inputs: its plans may end in a replicating top pass (r[i] = 1), whose two
+ * top layers only copy the quarter into all four, so the pass holds 2^(m-2)
+ * rows in LDS and transforms them once per quarter; lde_plan_set with
+ * n_passes = 0 falls back to the general table, plan_reset clears both. No
+ * reference counterpart (tuning of this library's pass structure). */
 int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms);
 /* Measured int32 VALU issue rate of the current device in T lane-ops/s (a
  * grid of v_add3 / v_xor / v_alignbit chains, the instructions of Blake2s). */
@@ -279,6 +286,9 @@ int32_t vx_measure_hash_latency(double* ns_quad, double* ns_lane);
 int32_t vx_ntt_plan_set(uint32_t log_n, uint32_t n_passes, const uint32_t* m, const uint32_t* c);
 uint32_t vx_ntt_plan_get(uint32_t log_n, uint32_t* m, uint32_t* c);
 void vx_ntt_plan_reset(void);
+int32_t vx_ntt_lde_plan_set(uint32_t log_n, uint32_t n_passes, const uint32_t* m, const uint32_t* c,
+                            const uint32_t* r);
+uint32_t vx_ntt_lde_plan_get(uint32_t log_n, uint32_t* m, uint32_t* c, uint32_t* r);
 
 /* Chunk depth of the Merkle stage kernels (no reference counterpart): a wave
  * hashes 2^c chunks of 128 items before its 64 -> 1 reduction. The depth is

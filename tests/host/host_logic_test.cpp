@@ -70,6 +70,53 @@ static void test_plans() {
     CHECK(!plan_valid(16, make({{7, 0}, {9, 6}})));          // strided pass too deep (c = 6)
     CHECK(!plan_valid(16, make({{5, 0}, {5, 6}, {6, 5}})));  // c = 6 pass starting at stride 2^5
     CHECK(!plan_valid(16, make({{8, 0}, {0, 5}, {8, 5}})));  // empty pass
+
+    // blowup-4 evaluation plans: a replicating top pass holds m - 2 rows
+    auto repl_top = [](std::vector<Pass> ps) {
+        ps.back().r = 1;
+        return ps;
+    };
+    const auto p22 = repl_top(make({{12, 0}, {10, 5}}));  // 8 tile rows, c = 5: two passes for 2^22
+    CHECK(!plan_valid(22, p22));                        // only in an LDE plan
+    CHECK(plan_valid(22, p22, true));
+    CHECK(!plan_valid(22, make({{12, 0}, {10, 5}}), true));  // 10 rows at c = 5 without replication
+    CHECK(plan_valid(26, repl_top(make({{14, 0}, {12, 4}})), true));
+    CHECK(!plan_valid(26, repl_top(make({{14, 0}, {12, 5}})), true));  // 10 tile rows at c = 5
+    CHECK(plan_valid(16, repl_top(make({{16, 0}})), true));            // one contiguous pass of 2^14 rows
+    CHECK(!plan_valid(17, repl_top(make({{17, 0}})), true));           // 2^15 rows
+    CHECK(!plan_valid(16, repl_top(make({{14, 0}, {2, 5}})), true));   // no tile rows left
+    {
+        auto ps = make({{8, 0}, {4, 5}, {4, 5}});
+        ps[1].r = 1;  // replicating pass below the top
+        CHECK(!plan_valid(16, ps, true));
+    }
+    for (uint32_t k = 3; k <= 30; k++) {
+        const auto cands = lde_plan_candidates(k);
+        bool has_repl = false;
+        for (const auto& ps : cands) {
+            CHECK(plan_valid(k, ps, true));
+            uint32_t sum = 0;
+            for (size_t i = 0; i < ps.size(); i++) {
+                sum += ps[i].m;
+                CHECK(ps[i].r == 0 || i + 1 == ps.size());
+            }
+            CHECK(sum == k);
+            has_repl |= ps.back().r == 1;
+        }
+        CHECK(has_repl);
+        CHECK(cands.size() > plan_candidates(k).size());
+    }
+    // the two-pass LDE plans of the provers' largest sizes are among the candidates
+    for (auto [k, want] : {std::pair<uint32_t, std::vector<Pass>>{22, p22},
+                           {26, repl_top(make({{14, 0}, {12, 4}}))}}) {
+        bool found = false;
+        for (const auto& ps : lde_plan_candidates(k))
+            found |= ps.size() == want.size() &&
+                     std::equal(ps.begin(), ps.end(), want.begin(), [](const Pass& x, const Pass& y) {
+                         return x.s == y.s && x.m == y.m && x.c == y.c && x.r == y.r;
+                     });
+        CHECK(found);
+    }
 }
 
 static void test_ring() {

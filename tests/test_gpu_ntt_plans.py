@@ -1,10 +1,13 @@
 """Every pass plan of the NTT (contiguous depth, strided pass count and row
-segment) gives the oracle's results bit for bit, native and stwo basis; the
-autotuner only ever installs such plans. Runs on the MI355X."""
+segment) gives the oracle's results bit for bit, native and stwo basis; so
+does every plan of a blowup-4 evaluation, with or without a replicating top
+pass, in proofs (single and batched) and wide commitments; the autotuner only
+ever installs such plans. Runs on the MI355X."""
 
 import numpy as np
 import pytest
 
+import inputs
 import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
@@ -74,5 +77,78 @@ def test_autotune_installs_valid_plans_and_proofs_stay_exact(vx, reset_plans):
     res = vx.ntt_autotune(8, 18)
     for k, (ms, plan) in res.items():
         assert ms > 0 and sum(m for m, _ in plan) == k and plan[0][1] == 0
+        lde = vx.ntt_lde_plan(k)
+        assert sum(m for m, _, _ in lde) == k and lde[0][1] == 0
+        assert all(r == 0 for _, _, r in lde[:-1])
     # a proof after tuning is still the oracle's
     assert vx.prove(3, 4, 16, 80, 3) == O.prove(3, 4, 16, 80, 3)
+
+
+# plans of blowup-4 evaluations, k = log_n + 2: (layers, segment_log, replicating);
+# a replicating top pass holds layers - 2 rows of the first quarter in LDS
+LDE_PLANS = {
+    13: [[(13, 0, 1)], [(8, 0, 0), (5, 4, 1)], [(6, 0, 0), (7, 5, 1)], [(6, 0, 0), (7, 6, 1)],
+         [(4, 0, 0), (4, 4, 0), (5, 4, 1)], [(8, 0, 0), (5, 5, 0)]],
+    16: [[(16, 0, 1)], [(12, 0, 0), (4, 5, 1)], [(8, 0, 0), (8, 6, 1)], [(6, 0, 0), (10, 4, 1)],
+         [(14, 0, 0), (2, 4, 0)]],
+    20: [[(10, 0, 0), (10, 6, 1)], [(9, 0, 0), (11, 5, 1)], [(8, 0, 0), (12, 4, 1)], [(14, 0, 0), (6, 4, 1)],
+         [(8, 0, 0), (5, 5, 0), (7, 5, 1)], [(12, 0, 0), (8, 5, 0)]],
+}
+
+
+def test_lde_plan_set_and_reject(vx, reset_plans):
+    general = vx.ntt_plan(22)
+    vx.ntt_lde_plan_set(22, [(12, 0, 0), (10, 5, 1)])
+    assert vx.ntt_lde_plan(22) == [(12, 0, 0), (10, 5, 1)]
+    assert vx.ntt_plan(22) == general  # the general table is untouched
+    for k, plan in [(22, [(12, 0, 0), (10, 5, 0)]),   # 10 rows at c = 5
+                    (22, [(12, 0, 1), (10, 5, 0)]),   # replicating pass below the top
+                    (17, [(17, 0, 1)]),               # 2^15 rows
+                    (16, [(14, 0, 0), (2, 5, 1)])]:   # no rows left
+        with pytest.raises(vx.HipError):
+            vx.ntt_lde_plan_set(k, plan)
+    with pytest.raises(vx.HipError):
+        vx.ntt_plan_set(22, [(12, 0), (10, 5)])  # replication is for LDE plans only
+    vx.ntt_lde_plan_set(22, [])
+    assert vx.ntt_lde_plan(22) == [(k, c, 0) for k, c in vx.ntt_plan(22)]
+
+
+@pytest.mark.parametrize("k", sorted(LDE_PLANS))
+def test_every_lde_plan_is_bit_exact(vx, reset_plans, k):
+    log_n = k - 2
+    exp_proof = O.prove(5, 7, log_n, 80, 3)
+    exp_lde = O.lde(O.fibonacci_trace(5, 7, log_n), log_n)
+    pairs = [(5, 7), (11, 13), (17, 19)]
+    exp_batch = [O.prove(a, b, log_n, 80, 3) for a, b in pairs[1:]] if k <= 16 else None
+    for plan in LDE_PLANS[k]:
+        vx.ntt_lde_plan_set(k, plan)
+        assert vx.ntt_lde_plan(k) == plan
+        p = vx.Prover(log_n, 80, 3)
+        assert p.prove(5, 7) == exp_proof, plan
+        assert np.array_equal(p.eval_words(), exp_lde), plan
+        p.close()
+        if exp_batch is not None:  # grid.y = proof
+            bp = vx.Prover(log_n, 80, 3, capacity=4)
+            got = bp.prove_batch(pairs)
+            assert got[0] == exp_proof and got[1:] == exp_batch, plan
+            bp.close()
+
+
+def test_wide_commit_with_replicating_lde_plans(vx, reset_plans):
+    """stwo basis: the wide commitment's LDE of every size through replicating plans"""
+    logs, blowup, last = [8, 8, 11, 14], 2, 3
+    cols = inputs.m31_columns(31, logs)
+    root, fri, lastv, beta = O.wide_commit(cols, logs, blowup, last)
+    for plans in ({10: [(10, 0, 1)], 13: [(13, 0, 1)], 16: [(16, 0, 1)]},
+                  {10: [(4, 0, 0), (6, 4, 1)], 13: [(6, 0, 0), (7, 6, 1)], 16: [(8, 0, 0), (8, 5, 1)]}):
+        for k, plan in plans.items():
+            vx.ntt_lde_plan_set(k, plan)
+        w = vx.WideCommit(logs, blowup, last)
+        bufs = [vx.DeviceBuffer.from_host(c) for c in cols]
+        w.commit_device([b.addr for b in bufs])
+        g_root, g_fri, g_last, g_beta = w.results()
+        assert np.array_equal(g_root, root) and np.array_equal(g_fri, fri), plans
+        assert np.array_equal(g_last, lastv) and np.array_equal(g_beta, beta), plans
+        for c, l in enumerate(logs):
+            assert np.array_equal(w.lde(c), O.stwo_lde(cols[c], l, blowup)), (plans, c)
+        w.close()

@@ -515,6 +515,25 @@ def ntt_plan_set(log_n: int, passes) -> None:
         raise HipError("vx_ntt_plan_set failed")
 
 
+def ntt_lde_plan(log_n: int) -> list:
+    """Plan of a blowup-4 evaluation of 2^log_n points (input in the first
+    quarter): [(layers, segment_log, replicating), ...]."""
+    m, c, r = np.zeros(8, np.uint32), np.zeros(8, np.uint32), np.zeros(8, np.uint32)
+    n = lib.vx_ntt_lde_plan_get(log_n, m.ctypes.data, c.ctypes.data, r.ctypes.data)
+    return [(int(m[i]), int(c[i]), int(r[i])) for i in range(n)]
+
+
+def ntt_lde_plan_set(log_n: int, passes) -> None:
+    """passes: [(layers, segment_log, replicating), ...]; only the top pass may
+    replicate; [] = the general plan of the size."""
+    m = np.array([p[0] for p in passes] or [0], np.uint32)
+    c = np.array([p[1] for p in passes] or [0], np.uint32)
+    r = np.array([p[2] for p in passes] or [0], np.uint32)
+    if lib.vx_ntt_lde_plan_set(log_n, len(passes), m.ctypes.data, c.ctypes.data, r.ctypes.data):
+        check("vx_ntt_lde_plan_set")
+        raise HipError("vx_ntt_lde_plan_set failed")
+
+
 def merkle_autotune(log_min: int, log_max: int) -> dict:
     """Times whole tree commitments of 2^log_min .. 2^log_max items at every
     stage chunk depth and installs the fastest; returns {log: (ms, depth)}."""

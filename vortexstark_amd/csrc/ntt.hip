@@ -228,26 +228,21 @@ __device__ __forceinline__ void inv_rounds(uint32_t* tile, const NttPassArgs& a,
 // the tile of 2^M rows (stride 2^s) by 2^C contiguous columns owned by this block.
 // Tiles move between HBM and LDS 4 words per lane (rows of >= 4 contiguous
 // words, or the contiguous tile of the first pass) when the buffers allow it.
-template <bool FWD, int M, int C, bool STWO>
-__global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
-    static_assert(!STWO || C == 0, "the stwo variant is the contiguous pass (stride bits from 0)");
-    __shared__ uint32_t tile[(1u << (M + C)) + (1u << (M + C)) / 8];
-    uint32_t* dst = a.cols ? a.cols[blockIdx.y] : a.dst;
-    const uint32_t* src = a.cols ? (a.srcs ? a.srcs[blockIdx.y] : a.cols[blockIdx.y]) : a.src;
-    const uint32_t s = a.s;
+//
+// REPL (forward only): the pass is the top pass of an evaluation whose input is
+// zero past its first quarter, with two layers above the tile's M. Those two
+// only copy the first quarter into all four, so the block loads its tile from
+// the first quarter and, for each quarter q, runs the M layers with q as the
+// row bits above the tile and stores the result to quarter q. Quarter 0 is
+// done last: in place, it overwrites the input the other quarters reload.
+template <bool FWD, int M, int C>
+__device__ __forceinline__ void tile_load(uint32_t* tile, const uint32_t* src, uint64_t base, uint32_t s,
+                                          uint64_t src_n, bool vec) {
     constexpr uint32_t TILE = 1u << (M + C);
-    const uint32_t mid_bits = s - C;
-    const uint64_t blk = blockIdx.x;
-    const uint64_t mid = blk & ((1ull << mid_bits) - 1);
-    const uint64_t hi = blk >> mid_bits;
-    const uint64_t base = (hi << (s + M)) + (mid << C);
-    constexpr bool VEC_OK = (C >= 2 || C == 0) && TILE >= 4;
-    const bool vec = VEC_OK && a.vec;
-
     if (vec) {
         for (uint32_t e = 4 * threadIdx.x; e < TILE; e += 4 * blockDim.x) {
             const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
-            const uint4 x = g < a.src_n ? *(const uint4*)(src + g) : make_uint4(0, 0, 0, 0);
+            const uint4 x = g < src_n ? *(const uint4*)(src + g) : make_uint4(0, 0, 0, 0);
             const uint32_t p = lpad(e);  // e % 4 == 0: the four words stay within one padded group
             tile[p] = x.x;
             tile[p + 1] = x.y;
@@ -257,34 +252,73 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
     } else {
         for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
             const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
-            tile[lpad(e)] = g < a.src_n ? src[g] : 0u;
+            tile[lpad(e)] = g < src_n ? src[g] : 0u;
         }
     }
-    __syncthreads();
-    if constexpr (FWD)
-        fwd_rounds<FWD, M, C, STWO, M>(tile, a, hi);
-    else
-        inv_rounds<FWD, M, C, STWO, 0>(tile, a, hi);
+}
+template <bool FWD, int M, int C>
+__device__ __forceinline__ void tile_store(const uint32_t* tile, uint32_t* dst, uint64_t base, uint32_t s,
+                                           uint32_t scale, bool vec) {
+    constexpr uint32_t TILE = 1u << (M + C);
     if (vec) {
         for (uint32_t e = 4 * threadIdx.x; e < TILE; e += 4 * blockDim.x) {
             const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
             const uint32_t p = lpad(e);
             uint4 x = make_uint4(tile[p], tile[p + 1], tile[p + 2], tile[p + 3]);
-            if (a.scale) x = make_uint4(mmul(x.x, a.scale), mmul(x.y, a.scale), mmul(x.z, a.scale), mmul(x.w, a.scale));
+            if (scale) x = make_uint4(mmul(x.x, scale), mmul(x.y, scale), mmul(x.z, scale), mmul(x.w, scale));
             *(uint4*)(dst + g) = x;
         }
     } else {
         for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
             const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
-            uint32_t v = tile[lpad(e)];
-            dst[g] = a.scale ? mmul(v, a.scale) : v;
+            const uint32_t v = tile[lpad(e)];
+            dst[g] = scale ? mmul(v, scale) : v;
         }
+    }
+}
+
+template <bool FWD, int M, int C, bool STWO, bool REPL>
+__global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
+    static_assert(!STWO || C == 0, "the stwo variant is the contiguous pass (stride bits from 0)");
+    static_assert(!REPL || FWD, "only an evaluation replicates its input");
+    __shared__ uint32_t tile[(1u << (M + C)) + (1u << (M + C)) / 8];
+    uint32_t* dst = a.cols ? a.cols[blockIdx.y] : a.dst;
+    const uint32_t* src = a.cols ? (a.srcs ? a.srcs[blockIdx.y] : a.cols[blockIdx.y]) : a.src;
+    const uint32_t s = a.s;
+    constexpr uint32_t TILE = 1u << (M + C);
+    const uint32_t mid_bits = s - C;
+    const uint64_t blk = blockIdx.x;
+    const uint64_t mid = blk & ((1ull << mid_bits) - 1);
+    const uint64_t hi = blk >> mid_bits;  // REPL: the grid holds no row bits above the tile, hi = 0
+    const uint64_t base = (hi << (s + M)) + (mid << C);
+    constexpr bool VEC_OK = (C >= 2 || C == 0) && TILE >= 4;
+    const bool vec = VEC_OK && a.vec;
+
+    tile_load<FWD, M, C>(tile, src, base, s, a.src_n, vec);
+    __syncthreads();
+    if constexpr (REPL) {
+        for (int q = 3; q >= 0; q--) {
+            if (q != 3) {
+                tile_load<FWD, M, C>(tile, src, base, s, a.src_n, vec);
+                __syncthreads();
+            }
+            fwd_rounds<FWD, M, C, STWO, M>(tile, a, (uint64_t)q);
+            tile_store<FWD, M, C>(tile, dst, base + ((uint64_t)q << (s + M)), s, 0u, vec);
+            __syncthreads();  // the next quarter's load overwrites the tile
+        }
+    } else {
+        if constexpr (FWD)
+            fwd_rounds<FWD, M, C, STWO, M>(tile, a, hi);
+        else
+            inv_rounds<FWD, M, C, STWO, 0>(tile, a, hi);
+        tile_store<FWD, M, C>(tile, dst, base, s, a.scale, vec);
     }
 }
 
 namespace {
 std::mutex g_plan_mu;
-std::vector<Pass> g_plan[32];  // tuned or set plans; empty: default split
+std::vector<Pass> g_plan[32];      // tuned or set plans; empty: default split
+std::vector<Pass> g_lde_plan[32];  // the same for blowup-4 evaluations; empty: g_plan's
 }  // namespace
 
 static std::vector<Pass> plan_passes(uint32_t k) {
@@ -294,23 +328,31 @@ static std::vector<Pass> plan_passes(uint32_t k) {
     }
     return default_plan(k);
 }
-
-template <bool FWD, int M, int C, bool STWO>
-static void launch_tile(const NttPassArgs& a, uint32_t k, uint32_t n_cols, hipStream_t st) {
-    dim3 grid((uint32_t)(1ull << (k - M - C)), n_cols);
-    constexpr uint32_t groups = 1u << (M + C > 3 ? M + C - 3 : 0);
-    uint32_t T = groups >= 256 ? 256 : (groups < 64 ? 64 : groups);
-    ntt_pass_kernel<FWD, M, C, STWO><<<grid, T, 0, st>>>(a);
+static std::vector<Pass> lde_plan_passes(uint32_t k) {
+    {
+        std::lock_guard<std::mutex> g(g_plan_mu);
+        if (k < 32 && !g_lde_plan[k].empty()) return g_lde_plan[k];
+    }
+    return plan_passes(k);
 }
 
-template <bool FWD, int C, int M, bool STWO>
+template <bool FWD, int M, int C, bool STWO, bool REPL>
+static void launch_tile(const NttPassArgs& a, uint32_t k, uint32_t n_cols, hipStream_t st) {
+    dim3 grid((uint32_t)(1ull << (k - (REPL ? 2 : 0) - M - C)), n_cols);
+    constexpr uint32_t groups = 1u << (M + C > 3 ? M + C - 3 : 0);
+    uint32_t T = groups >= 256 ? 256 : (groups < 64 ? 64 : groups);
+    ntt_pass_kernel<FWD, M, C, STWO, REPL><<<grid, T, 0, st>>>(a);
+}
+
+// m: rows of the tile (a replicating pass's layers less its two copy layers)
+template <bool FWD, int C, int M, bool STWO, bool REPL>
 static void dispatch_m(uint32_t m, const NttPassArgs& a, uint32_t k, uint32_t n_cols, hipStream_t st) {
     if constexpr (M >= 1) {
         if (m == (uint32_t)M) {
-            launch_tile<FWD, M, C, STWO>(a, k, n_cols, st);
+            launch_tile<FWD, M, C, STWO, REPL>(a, k, n_cols, st);
             return;
         }
-        dispatch_m<FWD, C, M - 1, STWO>(m, a, k, n_cols, st);
+        dispatch_m<FWD, C, M - 1, STWO, REPL>(m, a, k, n_cols, st);
     } else {
         set_error(-2, "ntt: unsupported pass shape");
     }
@@ -318,28 +360,34 @@ static void dispatch_m(uint32_t m, const NttPassArgs& a, uint32_t k, uint32_t n_
 
 // Only the contiguous pass (c == 0) contains the circle layer, so only it has a
 // stwo variant; strided passes differ by their layer offsets alone.
-template <bool FWD>
+template <bool FWD, bool REPL>
 static void launch_pass_dir(bool stwo, const NttPassArgs& a, uint32_t m, uint32_t c, uint32_t k, uint32_t n_cols,
                             hipStream_t st) {
     if (c == 0) {
         if (stwo)
-            dispatch_m<FWD, 0, NTT_LOW_MAX, true>(m, a, k, n_cols, st);
+            dispatch_m<FWD, 0, NTT_LOW_MAX, true, REPL>(m, a, k, n_cols, st);
         else
-            dispatch_m<FWD, 0, NTT_LOW_MAX, false>(m, a, k, n_cols, st);
+            dispatch_m<FWD, 0, NTT_LOW_MAX, false, REPL>(m, a, k, n_cols, st);
     } else if (c == 4) {
-        dispatch_m<FWD, 4, NTT_HIGH_MAX4, false>(m, a, k, n_cols, st);
+        dispatch_m<FWD, 4, NTT_HIGH_MAX4, false, REPL>(m, a, k, n_cols, st);
     } else if (c == 6) {
-        dispatch_m<FWD, 6, NTT_HIGH_MAX6, false>(m, a, k, n_cols, st);
+        dispatch_m<FWD, 6, NTT_HIGH_MAX6, false, REPL>(m, a, k, n_cols, st);
     } else {
-        dispatch_m<FWD, 5, NTT_HIGH_MAX5, false>(m, a, k, n_cols, st);
+        dispatch_m<FWD, 5, NTT_HIGH_MAX5, false, REPL>(m, a, k, n_cols, st);
     }
 }
-static void launch_pass(bool fwd, bool stwo, const NttPassArgs& a, uint32_t m, uint32_t c, uint32_t k,
-                        uint32_t n_cols, hipStream_t st) {
-    if (fwd)
-        launch_pass_dir<true>(stwo, a, m, c, k, n_cols, st);
+static void launch_pass(bool fwd, bool stwo, const NttPassArgs& a, const Pass& p, uint32_t k, uint32_t n_cols,
+                        hipStream_t st) {
+    if (p.r && !fwd) {
+        set_error(-2, "ntt: a replicating pass is forward only");
+        return;
+    }
+    if (p.r)
+        launch_pass_dir<true, true>(stwo, a, pass_rows(p), p.c, k, n_cols, st);
+    else if (fwd)
+        launch_pass_dir<true, false>(stwo, a, p.m, p.c, k, n_cols, st);
     else
-        launch_pass_dir<false>(stwo, a, m, c, k, n_cols, st);
+        launch_pass_dir<false, false>(stwo, a, p.m, p.c, k, n_cols, st);
     check_launch("ntt_pass_kernel");
 }
 
@@ -368,9 +416,13 @@ static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t 
         a.vec = vec ? 1u : 0u;
         a.scale = (!fwd && i + 1 == ps.size() && inv_n != 1u) ? inv_n : 0u;
         // the top pass of an evaluation of n/4 coefficients: its two top layers copy
-        a.zero_top2 = (fwd && i == 0 && ps[pi].m >= 3 && ps[pi].s + ps[pi].m == k && k >= 2 && src_n == (1ull << k) / 4)
-                          ? 1u : 0u;
-        launch_pass(fwd, t.stwo, a, ps[pi].m, ps[pi].c, k, n_cols, st);
+        const bool quarter = fwd && i == 0 && ps[pi].s + ps[pi].m == k && k >= 2 && src_n == (1ull << k) / 4;
+        if (ps[pi].r && !quarter) {
+            set_error(-2, "ntt: a replicating pass needs an input that fills the first quarter");
+            return;
+        }
+        a.zero_top2 = (quarter && !ps[pi].r && ps[pi].m >= 3) ? 1u : 0u;
+        launch_pass(fwd, t.stwo, a, ps[pi], k, n_cols, st);
     }
     if (!fwd && ps.empty() && src != dst && src_n) {
         check(hipMemcpyAsync(dst, src, 4, hipMemcpyDeviceToDevice, st), "ntt copy");
@@ -378,7 +430,9 @@ static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t 
 }
 static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n, uint32_t* const* cols,
                     const uint32_t* const* srcs, uint32_t n_cols, const NttTables& t, hipStream_t st, bool vec) {
-    run_ntt_plan(fwd, dst, src, src_n, cols, srcs, n_cols, t, st, plan_passes(t.log_n), vec);
+    const bool lde = fwd && t.log_n >= 2 && src_n == (1ull << t.log_n) / 4;
+    run_ntt_plan(fwd, dst, src, src_n, cols, srcs, n_cols, t, st, lde ? lde_plan_passes(t.log_n) : plan_passes(t.log_n),
+                 vec);
 }
 
 void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTables& t, hipStream_t s) {
@@ -700,6 +754,39 @@ uint32_t vx_ntt_plan_get(uint32_t log_n, uint32_t* m, uint32_t* c) {
 void vx_ntt_plan_reset(void) {
     std::lock_guard<std::mutex> g(g_plan_mu);
     for (auto& p : g_plan) p.clear();
+    for (auto& p : g_lde_plan) p.clear();
+}
+
+int32_t vx_ntt_lde_plan_set(uint32_t log_n, uint32_t n_passes, const uint32_t* m, const uint32_t* c,
+                            const uint32_t* r) {
+    if (log_n >= 32 || n_passes > 8 || (n_passes && (!m || !c || !r))) {
+        set_error(-2, "vx_ntt_lde_plan_set: bad arguments");
+        return -1;
+    }
+    std::vector<Pass> ps;
+    uint32_t s0 = 0;
+    for (uint32_t i = 0; i < n_passes; i++) {
+        ps.push_back({s0, m[i], c[i], r[i]});
+        s0 += m[i];
+    }
+    if (n_passes && !plan_valid(log_n, ps, true)) {
+        set_error(-2, "vx_ntt_lde_plan_set: passes must cover every layer with supported shapes");
+        return -1;
+    }
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    g_lde_plan[log_n] = ps;  // empty: the general plan of the size
+    return 0;
+}
+
+uint32_t vx_ntt_lde_plan_get(uint32_t log_n, uint32_t* m, uint32_t* c, uint32_t* r) {
+    if (log_n >= 32) return 0;
+    const auto ps = lde_plan_passes(log_n);
+    for (size_t i = 0; i < ps.size() && i < 8; i++) {
+        if (m) m[i] = ps[i].m;
+        if (c) c[i] = ps[i].c;
+        if (r) r[i] = ps[i].r;
+    }
+    return (uint32_t)ps.size();
 }
 
 int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms) {
@@ -708,11 +795,17 @@ int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms) {
         return -1;
     }
     const uint64_t N = 1ull << log_max;
+    // blowup-4 evaluations are timed as the provers run them: a batch of columns
+    // (one per proof or per committed column) of about 2^26 output words
+    constexpr uint32_t LDE_LOG_WORDS = 26, LDE_MAX_COLS = 1024;
+    const uint64_t words = std::max<uint64_t>(N, 1ull << LDE_LOG_WORDS);
     hipStream_t st = nullptr;
     uint32_t *data = nullptr, *line = nullptr, *circle = nullptr;
+    uint32_t** d_cols = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool ok = check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate") &&
-              check(hipMalloc((void**)&data, 4 * N), "hipMalloc") &&
+              check(hipMalloc((void**)&data, 4 * words), "hipMalloc") &&
+              check(hipMalloc((void**)&d_cols, sizeof(uint32_t*) * LDE_MAX_COLS), "hipMalloc") &&
               check(hipMalloc((void**)&line, 4 * N), "hipMalloc") &&
               check(hipMalloc((void**)&circle, 4 * N / 2), "hipMalloc") && check(hipEventCreate(&e0), "hipEvent") &&
               check(hipEventCreate(&e1), "hipEvent");
@@ -724,34 +817,51 @@ int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms) {
         t.circle = circle;
         t.log_n = k;
         build_ntt_tables(CPt{hi.x, hi.y}, CPt{hs.x, hs.y}, k, line, circle, nullptr, nullptr, t.offs, st);
-        ok = check(hipMemsetAsync(data, 0x11, 4ull << k, st), "hipMemsetAsync");  // canonical M31 words
-        const auto cands = plan_candidates(k);
-        float best = 1e30f;
-        size_t bi = 0;
-        for (size_t ci = 0; ok && ci < cands.size(); ci++) {
-            run_ntt_plan(true, data, data, 1ull << k, nullptr, nullptr, 1, t, st, cands[ci], true);  // warm-up
-            const int reps = 3;
-            ok = check(hipEventRecord(e0, st), "hipEventRecord");
-            for (int r = 0; r < reps; r++)
-                run_ntt_plan(true, data, data, 1ull << k, nullptr, nullptr, 1, t, st, cands[ci], true);
-            ok = ok && check(hipEventRecord(e1, st), "hipEventRecord") && check(hipEventSynchronize(e1), "sync");
-            float ms = 0.f;
-            ok = ok && check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime") && peek_error() == 0;
-            if (ok && ms / reps < best) {
-                best = ms / reps;
-                bi = ci;
+        const uint32_t n_cols = k >= LDE_LOG_WORDS ? 1u : std::min<uint32_t>(LDE_MAX_COLS, 1u << (LDE_LOG_WORDS - k));
+        ok = check(hipMemsetAsync(data, 0x11, 4ull * n_cols << k, st), "hipMemsetAsync");  // canonical M31 words
+        std::vector<uint32_t*> h_cols(n_cols);
+        for (uint32_t c = 0; c < n_cols; c++) h_cols[c] = data + ((uint64_t)c << k);
+        ok = ok && check(hipMemcpyAsync(d_cols, h_cols.data(), sizeof(uint32_t*) * n_cols, hipMemcpyHostToDevice, st),
+                         "hipMemcpyAsync") && check(hipStreamSynchronize(st), "sync");
+        // the general table on one full input, then the blowup-4 table on a batch
+        // of inputs that fill their first quarter (the LDE evaluations)
+        for (int lde = 0; lde < 2 && ok; lde++) {
+            if (lde && k < 3) break;
+            const uint64_t src_n = lde ? (1ull << k) / 4 : 1ull << k;
+            const auto cands = lde ? lde_plan_candidates(k) : plan_candidates(k);
+            float best = 1e30f;
+            size_t bi = 0;
+            auto run = [&](const std::vector<Pass>& ps) {
+                if (lde)
+                    run_ntt_plan(true, nullptr, nullptr, src_n, d_cols, nullptr, n_cols, t, st, ps, true);
+                else
+                    run_ntt_plan(true, data, data, src_n, nullptr, nullptr, 1, t, st, ps, true);
+            };
+            for (size_t ci = 0; ok && ci < cands.size(); ci++) {
+                run(cands[ci]);  // warm-up
+                const int reps = 3;
+                ok = check(hipEventRecord(e0, st), "hipEventRecord");
+                for (int r = 0; r < reps; r++) run(cands[ci]);
+                ok = ok && check(hipEventRecord(e1, st), "hipEventRecord") && check(hipEventSynchronize(e1), "sync");
+                float ms = 0.f;
+                ok = ok && check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime") && peek_error() == 0;
+                if (ok && ms / reps < best) {
+                    best = ms / reps;
+                    bi = ci;
+                }
             }
+            if (ok) {
+                std::lock_guard<std::mutex> g(g_plan_mu);
+                (lde ? g_lde_plan : g_plan)[k] = cands[bi];
+            }
+            if (best_ms && !lde) best_ms[k - log_min] = best;
         }
-        if (ok) {
-            std::lock_guard<std::mutex> g(g_plan_mu);
-            g_plan[k] = cands[bi];
-        }
-        if (best_ms) best_ms[k - log_min] = best;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamSynchronize(st);
     if (data) (void)hipFree(data);
+    if (d_cols) (void)hipFree(d_cols);
     if (line) (void)hipFree(line);
     if (circle) (void)hipFree(circle);
     if (st) (void)hipStreamDestroy(st);
