@@ -5,8 +5,10 @@
 // itself. fri_tail_kernel runs every remaining line fold of a proof in one
 // workgroup of 1024 threads: fold (cuda/fri.cu:10-39) into the next layer,
 // hash it as an SoA4 tree (level-store nodes written as by the stage/tail
-// kernels; levels wider than the block's 256 quads with one compression per
-// thread, the rest quad-cooperatively across the whole block), reduce to the
+// kernels; a layer of up to 1024 values with one leaf per thread, the largest
+// with two leaves and their parent per thread; levels wider than the block's
+// 256 quads with one compression per thread, the rest quad-cooperatively
+// across the whole block), reduce to the
 // root, mix the root into the transcript and draw the next challenge, which
 // stays in LDS for the next layer (and is also written to device memory).
 // blockIdx.y selects the proof of a batch (buffers bs words apart).
@@ -58,8 +60,31 @@ static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTa
         const uint32_t* i3 = L.in[3] + boff;
         uint32_t* cur = (uint32_t*)bufa;
         uint32_t* nxt = (uint32_t*)bufb;
-        // level 1: parents of leaf pairs (2j, 2j+1), each leaf a fold of two values
-        for (uint32_t j = tid; j < half; j += FRI_TAIL_THREADS) {
+        uint32_t m = half, lvl = 1;
+        if (L.n <= FRI_TAIL_THREADS) {
+            // one leaf per thread, each a fold of two values; their parents are
+            // the first of the parallel levels below (a thread hashing two
+            // leaves and their parent in turn would leave most lanes idle)
+            const uint32_t k = tid;
+            if (k < L.n) {
+                const uint32_t x = 2 * k;
+                const uint2 a = *(const uint2*)(i0 + x), b = *(const uint2*)(i1 + x);
+                const uint2 c = *(const uint2*)(i2 + x), d = *(const uint2*)(i3 + x);
+                const QM31 f = fold_pair({{a.x, b.x, c.x, d.x}}, {{a.y, b.y, c.y, d.y}}, alpha, L.tw[k]);
+                L.out[0][boff + k] = f.v[0];
+                L.out[1][boff + k] = f.v[1];
+                L.out[2][boff + k] = f.v[2];
+                L.out[3][boff + k] = f.v[3];
+                uint32_t h[8];
+                b2s_leaf4(f.v[0], f.v[1], f.v[2], f.v[3], h);
+                bufa[2 * k] = make_uint4(h[0], h[1], h[2], h[3]);
+                bufa[2 * k + 1] = make_uint4(h[4], h[5], h[6], h[7]);
+            }
+            m = L.n;
+            lvl = 0;
+        }
+        // level 1 of the largest layers: parents of leaf pairs (2j, 2j+1)
+        for (uint32_t j = tid; L.n > FRI_TAIL_THREADS && j < half; j += FRI_TAIL_THREADS) {
             const uint32_t k = 2 * j;
             const uint32_t x = 2 * k;
             const uint4 a = *(const uint4*)(i0 + x), b = *(const uint4*)(i1 + x);
@@ -80,7 +105,6 @@ static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTa
             bufa[2 * j + 1] = make_uint4(h[4], h[5], h[6], h[7]);
         }
         __syncthreads();
-        uint32_t m = half, lvl = 1;
         // wide levels (more compressions than quads): one per thread, all lanes busy
         while (m > 2 * QUADS) {
             const uint32_t hm = m / 2;

@@ -93,12 +93,31 @@ struct NttPassArgs {
 // accesses below free of bank conflicts.
 __device__ __forceinline__ uint32_t lpad(uint32_t e) { return e + (e >> 3); }
 
+// Word h / word pair h of a twiddle table whose address is wave-uniform. A
+// layer's table holds at most 2^30 words (transforms of up to 2^31 points), so
+// the byte offset is a 32-bit value: the load takes the table in scalar
+// registers plus one 32-bit offset per lane instead of a 64-bit address
+// computed per lane.
+__device__ __forceinline__ uint32_t ldw(const uint32_t* T, uint32_t h) {
+    return *(const uint32_t*)((const char*)T + (h << 2));
+}
+__device__ __forceinline__ uint2 ldw2(const uint32_t* T, uint32_t h) {
+    return *(const uint2*)((const char*)T + (h << 3));
+}
+
+// Threads of a pass over 2^M x 2^C tiles: one radix-8 group per thread, 64 to 256.
+template <int M, int C>
+constexpr uint32_t ntt_threads() {
+    constexpr uint32_t groups = 1u << (M + C > 3 ? M + C - 3 : 0);
+    return groups >= 256 ? 256 : (groups < 64 ? 64 : groups);
+}
+
 // Twiddle h of a layer whose table is T; circle0: the layer is stride bit 0.
 template <bool STWO>
 __device__ __forceinline__ uint32_t ntt_twiddle(const uint32_t* T, bool circle0, uint32_t h) {
-    if (!STWO || !circle0) return T[h];
-    const uint32_t* pr = T + 2 * (h >> 2);
-    const uint32_t x = pr[0], y = pr[1];
+    if (!STWO || !circle0) return ldw(T, h);
+    const uint2 pr = ldw2(T, h >> 2);
+    const uint32_t x = pr.x, y = pr.y;
     switch (h & 3) {
         case 0: return y;
         case 1: return mneg(y);
@@ -121,12 +140,31 @@ template <bool FWD, int M, int C, int R, int J, bool STWO>
 __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi, bool zero_top = false) {
     constexpr uint32_t GROUPS = 1u << (M + C - R);
     constexpr uint32_t STEP = 1u << (J + C);  // tile index distance between rows d and d + 1
-    for (uint32_t g = threadIdx.x; g < GROUPS; g += blockDim.x) {
-        const uint32_t q = g & ((1u << C) - 1);
-        const uint32_t rest = g >> C;
-        const uint32_t row_lo = rest & ((1u << J) - 1);
-        const uint32_t row_hi = rest >> J;
-        const uint32_t e0 = ((((row_hi << (J + R)) | row_lo)) << C) + q;
+    constexpr uint32_t NT = ntt_threads<M, C>();
+    constexpr uint32_t ITER = (GROUPS + NT - 1) / NT;
+    // Thread t takes groups g = t + it * NT; a group is (row_hi, row_lo, q) =
+    // (g >> (C+J), the J bits above C, the C low bits) and starts at tile index
+    // ((row_hi << (J+R) | row_lo) << C) + q. With NT a multiple of 2^(C+J) the
+    // next group of a thread has the same column and low row bits and row_hi
+    // larger by NT >> (C+J); otherwise (NT < 2^(C+J), t < 2^(C+J)) group it has
+    // row_hi = it / P and its low bits t + (it % P) * NT, P = 2^(C+J) / NT.
+    // Either way a thread's groups sit at constant distances from its first in
+    // the tile and in each layer's twiddle table, which the unrolled loop folds
+    // into instruction offsets.
+    constexpr uint32_t CJ = 1u << (C + J);
+    constexpr bool WIDE = NT >= CJ;
+    constexpr uint32_t P = WIDE ? 1u : CJ / NT;
+    const uint32_t g0 = threadIdx.x;
+    const uint32_t q = g0 & ((1u << C) - 1);
+    const uint32_t rest = g0 >> C;
+    const uint32_t row_lo = rest & ((1u << J) - 1);
+    const uint32_t row_hi0 = rest >> J;  // 0 unless WIDE
+    const uint32_t e00 = ((((row_hi0 << (J + R)) | row_lo)) << C) + q;
+#pragma unroll
+    for (uint32_t it = 0; it < ITER; it++) {
+        if (GROUPS % NT != 0 && g0 + it * NT >= GROUPS) break;  // fewer groups than threads
+        const uint32_t e0 = WIDE ? e00 + it * (NT << R) : e00 + (it % P) * NT + ((it / P) << (J + R + C));
+        const uint32_t row_hi = WIDE ? row_hi0 + it * (NT / CJ) : it / P;
         uint32_t v[1 << R];
         uint32_t addr[1 << R];
 #pragma unroll
@@ -160,7 +198,7 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
                 // 4, i.e. [y, -y, -x, x] of the pair at hb / 2; a negated
                 // twiddle swaps the two outputs (forward) or negates the
                 // difference (inverse), so no negation is computed
-                const uint2 xy = *(const uint2*)(T + (hb >> 1));
+                const uint2 xy = ldw2(T, hb >> 2);
 #pragma unroll
                 for (int d = 0; d < 8; d += 2) {
                     const uint32_t w = d < 4 ? xy.y : xy.x;
@@ -235,13 +273,24 @@ __device__ __forceinline__ void inv_rounds(uint32_t* tile, const NttPassArgs& a,
 // the first quarter and, for each quarter q, runs the M layers with q as the
 // row bits above the tile and stores the result to quarter q. Quarter 0 is
 // done last: in place, it overwrites the input the other quarters reload.
+// Tile element e (row e >> C, column e & (2^C - 1)) lies at word base + (row <<
+// s) + column of its column buffer (< 2^31 words, so a 32-bit index; the
+// address itself needs 64 bits).
+template <int M, int C>
+__device__ __forceinline__ uint32_t tile_word(uint32_t base, uint32_t s, uint32_t e) {
+    return base + ((e >> C) << s) + (e & ((1u << C) - 1));
+}
 template <bool FWD, int M, int C>
-__device__ __forceinline__ void tile_load(uint32_t* tile, const uint32_t* src, uint64_t base, uint32_t s,
+__device__ __forceinline__ void tile_load(uint32_t* tile, const uint32_t* src, uint32_t base, uint32_t s,
                                           uint64_t src_n, bool vec) {
     constexpr uint32_t TILE = 1u << (M + C);
+    constexpr uint32_t NT = ntt_threads<M, C>();
     if (vec) {
-        for (uint32_t e = 4 * threadIdx.x; e < TILE; e += 4 * blockDim.x) {
-            const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+#pragma unroll
+        for (uint32_t it = 0; it < (TILE + 4 * NT - 1) / (4 * NT); it++) {
+            const uint32_t e = 4 * threadIdx.x + it * 4 * NT;
+            if (TILE % (4 * NT) != 0 && e >= TILE) break;
+            const uint32_t g = tile_word<M, C>(base, s, e);
             const uint4 x = g < src_n ? *(const uint4*)(src + g) : make_uint4(0, 0, 0, 0);
             const uint32_t p = lpad(e);  // e % 4 == 0: the four words stay within one padded group
             tile[p] = x.x;
@@ -250,27 +299,37 @@ __device__ __forceinline__ void tile_load(uint32_t* tile, const uint32_t* src, u
             tile[p + 3] = x.w;
         }
     } else {
-        for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
-            const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+#pragma unroll
+        for (uint32_t it = 0; it < (TILE + NT - 1) / NT; it++) {
+            const uint32_t e = threadIdx.x + it * NT;
+            if (TILE % NT != 0 && e >= TILE) break;
+            const uint32_t g = tile_word<M, C>(base, s, e);
             tile[lpad(e)] = g < src_n ? src[g] : 0u;
         }
     }
 }
 template <bool FWD, int M, int C>
-__device__ __forceinline__ void tile_store(const uint32_t* tile, uint32_t* dst, uint64_t base, uint32_t s,
+__device__ __forceinline__ void tile_store(const uint32_t* tile, uint32_t* dst, uint32_t base, uint32_t s,
                                            uint32_t scale, bool vec) {
     constexpr uint32_t TILE = 1u << (M + C);
+    constexpr uint32_t NT = ntt_threads<M, C>();
     if (vec) {
-        for (uint32_t e = 4 * threadIdx.x; e < TILE; e += 4 * blockDim.x) {
-            const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+#pragma unroll
+        for (uint32_t it = 0; it < (TILE + 4 * NT - 1) / (4 * NT); it++) {
+            const uint32_t e = 4 * threadIdx.x + it * 4 * NT;
+            if (TILE % (4 * NT) != 0 && e >= TILE) break;
+            const uint32_t g = tile_word<M, C>(base, s, e);
             const uint32_t p = lpad(e);
             uint4 x = make_uint4(tile[p], tile[p + 1], tile[p + 2], tile[p + 3]);
             if (scale) x = make_uint4(mmul(x.x, scale), mmul(x.y, scale), mmul(x.z, scale), mmul(x.w, scale));
             *(uint4*)(dst + g) = x;
         }
     } else {
-        for (uint32_t e = threadIdx.x; e < TILE; e += blockDim.x) {
-            const uint64_t g = base + ((uint64_t)(e >> C) << s) + (e & ((1u << C) - 1));
+#pragma unroll
+        for (uint32_t it = 0; it < (TILE + NT - 1) / NT; it++) {
+            const uint32_t e = threadIdx.x + it * NT;
+            if (TILE % NT != 0 && e >= TILE) break;
+            const uint32_t g = tile_word<M, C>(base, s, e);
             const uint32_t v = tile[lpad(e)];
             dst[g] = scale ? mmul(v, scale) : v;
         }
@@ -287,10 +346,10 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
     const uint32_t s = a.s;
     constexpr uint32_t TILE = 1u << (M + C);
     const uint32_t mid_bits = s - C;
-    const uint64_t blk = blockIdx.x;
-    const uint64_t mid = blk & ((1ull << mid_bits) - 1);
-    const uint64_t hi = blk >> mid_bits;  // REPL: the grid holds no row bits above the tile, hi = 0
-    const uint64_t base = (hi << (s + M)) + (mid << C);
+    const uint32_t blk = blockIdx.x;
+    const uint32_t mid = blk & ((1u << mid_bits) - 1);
+    const uint32_t hi = blk >> mid_bits;  // REPL: the grid holds no row bits above the tile, hi = 0
+    const uint32_t base = (hi << (s + M)) + (mid << C);  // < 2^k <= 2^31 words
     constexpr bool VEC_OK = (C >= 2 || C == 0) && TILE >= 4;
     const bool vec = VEC_OK && a.vec;
 
@@ -303,7 +362,7 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
                 __syncthreads();
             }
             fwd_rounds<FWD, M, C, STWO, M>(tile, a, (uint64_t)q);
-            tile_store<FWD, M, C>(tile, dst, base + ((uint64_t)q << (s + M)), s, 0u, vec);
+            tile_store<FWD, M, C>(tile, dst, base + ((uint32_t)q << (s + M)), s, 0u, vec);
             __syncthreads();  // the next quarter's load overwrites the tile
         }
     } else {
@@ -339,9 +398,7 @@ static std::vector<Pass> lde_plan_passes(uint32_t k) {
 template <bool FWD, int M, int C, bool STWO, bool REPL>
 static void launch_tile(const NttPassArgs& a, uint32_t k, uint32_t n_cols, hipStream_t st) {
     dim3 grid((uint32_t)(1ull << (k - (REPL ? 2 : 0) - M - C)), n_cols);
-    constexpr uint32_t groups = 1u << (M + C > 3 ? M + C - 3 : 0);
-    uint32_t T = groups >= 256 ? 256 : (groups < 64 ? 64 : groups);
-    ntt_pass_kernel<FWD, M, C, STWO, REPL><<<grid, T, 0, st>>>(a);
+    ntt_pass_kernel<FWD, M, C, STWO, REPL><<<grid, ntt_threads<M, C>(), 0, st>>>(a);
 }
 
 // m: rows of the tile (a replicating pass's layers less its two copy layers)
