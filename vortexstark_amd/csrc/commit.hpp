@@ -30,19 +30,22 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
     }
     uint32_t logc = pick_logc(n_leaves, StageLogcMax<Src>::value, batch, stage_uses_plan<Src>());
     uint64_t waves = n_leaves >> (7 + logc);
+    // deep leaf stages hand on MERKLE_STAGE_KEEP nodes a wave (at most n / 128
+    // in all, the scratch bound of the shallowest stage); the others one
+    const uint32_t keep = logc >= MERKLE_KEEP_LOGC ? MERKLE_STAGE_KEEP : 1u;
     merkle_stage_kernel<Src><<<dim3(ceil_div(waves, MERKLE_STAGE_WAVES), batch), 64 * MERKLE_STAGE_WAVES, 0, s>>>(
-        src, 0, n_leaves, logc, sc.a, st, bs);
+        src, 0, n_leaves, logc, sc.a, st, bs, keep);
     check_launch("merkle_stage_kernel");
     mark("stage_leaf");
-    uint64_t n = waves;  // one node per wave
-    uint32_t lvl = 7 + logc;
+    uint64_t n = waves * keep;
+    uint32_t lvl = 7 + logc - (keep == 1 ? 0u : 4u);  // 16 = 64 >> 2 nodes: two of the six levels done
     uint32_t* cur = sc.a;
     uint32_t* nxt = sc.b;
     while (n > PROVER_TAIL_ITEMS) {
         logc = pick_logc(n, MERKLE_LOGC_MAX, batch);
         waves = n >> (7 + logc);
         merkle_stage_kernel<SrcNodes><<<dim3(ceil_div(waves, MERKLE_STAGE_WAVES), batch), 64 * MERKLE_STAGE_WAVES,
-                                        0, s>>>(SrcNodes{cur}, lvl, n, logc, nxt, st, bs);
+                                        0, s>>>(SrcNodes{cur}, lvl, n, logc, nxt, st, bs, 1u);
         check_launch("merkle_stage_kernel<nodes>");
         mark("stage_node");
         n = waves;

@@ -42,6 +42,8 @@ namespace vx {
 
 constexpr uint32_t MERKLE_TAIL_MAX = 2048;  // most items one tail-kernel block reduces
 constexpr uint32_t MERKLE_STAGE_WAVES = 4;  // waves per stage workgroup
+constexpr uint32_t MERKLE_STAGE_KEEP = 16;  // nodes a wave of a deep leaf stage hands on
+constexpr uint32_t MERKLE_KEEP_LOGC = 4;    // chunk depth from which leaf stages do so
 
 // Nodes of levels [from, depth] of a tree with 2^depth leaves, level-major.
 struct LevelStore {
@@ -450,13 +452,14 @@ struct SrcWideFold {
 
 // ---------------------------------------------------------------- narrow levels
 // Reduces n (a power of two <= 64) nodes stored node-major ([node][8 words]) in
-// the LDS buffer `a` to one node with the quads of the calling wave, ping-ponging
-// with `b`. The nodes are at tree level `lvl`, node 0 having index idx0 there.
-// Every lane of the wave must call this; returns the buffer holding the root.
+// the LDS buffer `a` to `stop` nodes (a power of two, default one) with the
+// quads of the calling wave, ping-ponging with `b`. The nodes are at tree level
+// `lvl`, node 0 having index idx0 there. Every lane of the wave must call this;
+// returns the buffer holding the remaining nodes.
 __device__ __forceinline__ uint32_t* narrow_reduce(uint32_t* a, uint32_t* b, uint32_t n, uint32_t lvl, uint64_t idx0,
-                                                   const LevelStore& st) {
+                                                   const LevelStore& st, uint32_t stop = 1) {
     const uint32_t lane = threadIdx.x & 63, quad = lane >> 2, l = lane & 3;
-    while (n > 1) {
+    while (n > stop) {
         const uint32_t half = n / 2;
         for (uint32_t q0 = 0; q0 < half; q0 += 16) {
             const uint32_t q = q0 + quad;
@@ -609,11 +612,14 @@ constexpr uint32_t stage_wave_lds() {
 
 // blockIdx.y selects the proof of a batch: every per-proof buffer (items,
 // out_nodes, level store) lies blockIdx.y * bs words past proof 0's.
+// A wave hands on `keep` (1 or MERKLE_STAGE_KEEP) nodes of its subtree: the
+// last levels of its 64 -> 1 reduction leave most of the wave's quads idle, and
+// the next stage computes those nodes one per lane instead.
 template <class Src>
 __global__ __launch_bounds__(64 * MERKLE_STAGE_WAVES) void merkle_stage_kernel(Src src0, uint32_t item_level,
                                                                              uint64_t n_items, uint32_t logc,
                                                                              uint32_t* out_nodes, LevelStore st0,
-                                                                             uint64_t bs) {
+                                                                             uint64_t bs, uint32_t keep) {
     constexpr uint32_t WAVE_LDS = stage_wave_lds<Src>();
     __shared__ uint32_t lds[MERKLE_STAGE_WAVES * WAVE_LDS];
     const uint64_t boff = (uint64_t)blockIdx.y * bs;
@@ -662,12 +668,12 @@ __global__ __launch_bounds__(64 * MERKLE_STAGE_WAVES) void merkle_stage_kernel(S
             VX_WAVE_SYNC();
         }
     }
-    // the wave's 64 top nodes (level item_level + 1 + logc) -> one node, 6 levels up
+    // the wave's 64 top nodes (level item_level + 1 + logc) -> `keep` nodes
 #pragma unroll
     for (int q = 0; q < 8; q++) wl[lane * 8 + q] = h[q];
     VX_WAVE_SYNC();
-    uint32_t* root = narrow_reduce(wl, wl + 512, 64, item_level + 1 + logc, wave_id * 64, st);
-    if (lane < 8) out_nodes[8 * wave_id + lane] = root[lane];
+    const uint32_t* top = narrow_reduce(wl, wl + 512, 64, item_level + 1 + logc, wave_id * 64, st, keep);
+    for (uint32_t k = lane; k < 8 * keep; k += 64) out_nodes[8 * keep * wave_id + k] = top[k];
 }
 
 // ---------------------------------------------------------------- tail kernel
