@@ -137,7 +137,8 @@ __device__ __forceinline__ uint32_t ntt_twiddle(const uint32_t* T, bool circle0,
 // are zero on entry, so the top two layers only copy: (x, 0) -> (x, x). Those
 // rounds load two values and replicate them instead of running 8 butterflies.
 template <bool FWD, int M, int C, int R, int J, bool STWO>
-__device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi, bool zero_top = false) {
+__device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi, uint32_t tid,
+                                          bool zero_top = false) {
     constexpr uint32_t GROUPS = 1u << (M + C - R);
     constexpr uint32_t STEP = 1u << (J + C);  // tile index distance between rows d and d + 1
     constexpr uint32_t NT = ntt_threads<M, C>();
@@ -154,7 +155,7 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
     constexpr uint32_t CJ = 1u << (C + J);
     constexpr bool WIDE = NT >= CJ;
     constexpr uint32_t P = WIDE ? 1u : CJ / NT;
-    const uint32_t g0 = threadIdx.x;
+    const uint32_t g0 = tid;
     const uint32_t q = g0 & ((1u << C) - 1);
     const uint32_t rest = g0 >> C;
     const uint32_t row_lo = rest & ((1u << J) - 1);
@@ -242,23 +243,25 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
 
 // Radix-8 rounds over the M layers of a tile, top down (forward) or bottom up
 // (inverse), unrolled at compile time.
+// tid is the thread's index in the block (see the replicating pass for why it
+// is a parameter).
 template <bool FWD, int M, int C, bool STWO, int TOP>
-__device__ __forceinline__ void fwd_rounds(uint32_t* tile, const NttPassArgs& a, uint64_t hi) {
+__device__ __forceinline__ void fwd_rounds(uint32_t* tile, const NttPassArgs& a, uint64_t hi, uint32_t tid) {
     if constexpr (TOP > 0) {
         constexpr int R = TOP >= 3 ? 3 : TOP;
         // only the pass's first round (its top three row bits) may skip its top two layers
-        ntt_round<FWD, M, C, R, TOP - R, STWO>(tile, a, hi, TOP == M && a.zero_top2 != 0);
+        ntt_round<FWD, M, C, R, TOP - R, STWO>(tile, a, hi, tid, TOP == M && a.zero_top2 != 0);
         __syncthreads();
-        fwd_rounds<FWD, M, C, STWO, TOP - R>(tile, a, hi);
+        fwd_rounds<FWD, M, C, STWO, TOP - R>(tile, a, hi, tid);
     }
 }
 template <bool FWD, int M, int C, bool STWO, int LO>
-__device__ __forceinline__ void inv_rounds(uint32_t* tile, const NttPassArgs& a, uint64_t hi) {
+__device__ __forceinline__ void inv_rounds(uint32_t* tile, const NttPassArgs& a, uint64_t hi, uint32_t tid) {
     if constexpr (LO < M) {
         constexpr int R = M - LO >= 3 ? 3 : M - LO;
-        ntt_round<FWD, M, C, R, LO, STWO>(tile, a, hi);
+        ntt_round<FWD, M, C, R, LO, STWO>(tile, a, hi, tid);
         __syncthreads();
-        inv_rounds<FWD, M, C, STWO, LO + R>(tile, a, hi);
+        inv_rounds<FWD, M, C, STWO, LO + R>(tile, a, hi, tid);
     }
 }
 
@@ -271,8 +274,9 @@ __device__ __forceinline__ void inv_rounds(uint32_t* tile, const NttPassArgs& a,
 // zero past its first quarter, with two layers above the tile's M. Those two
 // only copy the first quarter into all four, so the block loads its tile from
 // the first quarter and, for each quarter q, runs the M layers with q as the
-// row bits above the tile and stores the result to quarter q. Quarter 0 is
-// done last: in place, it overwrites the input the other quarters reload.
+// row bits above the tile and stores the result to quarter q. The tile is
+// kept in registers between quarters (or reloaded, with word moves); quarter 0
+// is done last, so in place it overwrites its input only after every reload.
 // Tile element e (row e >> C, column e & (2^C - 1)) lies at word base + (row <<
 // s) + column of its column buffer (< 2^31 words, so a 32-bit index; the
 // address itself needs 64 bits).
@@ -282,13 +286,13 @@ __device__ __forceinline__ uint32_t tile_word(uint32_t base, uint32_t s, uint32_
 }
 template <bool FWD, int M, int C>
 __device__ __forceinline__ void tile_load(uint32_t* tile, const uint32_t* src, uint32_t base, uint32_t s,
-                                          uint64_t src_n, bool vec) {
+                                          uint64_t src_n, bool vec, uint32_t tid) {
     constexpr uint32_t TILE = 1u << (M + C);
     constexpr uint32_t NT = ntt_threads<M, C>();
     if (vec) {
 #pragma unroll
         for (uint32_t it = 0; it < (TILE + 4 * NT - 1) / (4 * NT); it++) {
-            const uint32_t e = 4 * threadIdx.x + it * 4 * NT;
+            const uint32_t e = 4 * tid + it * 4 * NT;
             if (TILE % (4 * NT) != 0 && e >= TILE) break;
             const uint32_t g = tile_word<M, C>(base, s, e);
             const uint4 x = g < src_n ? *(const uint4*)(src + g) : make_uint4(0, 0, 0, 0);
@@ -301,7 +305,7 @@ __device__ __forceinline__ void tile_load(uint32_t* tile, const uint32_t* src, u
     } else {
 #pragma unroll
         for (uint32_t it = 0; it < (TILE + NT - 1) / NT; it++) {
-            const uint32_t e = threadIdx.x + it * NT;
+            const uint32_t e = tid + it * NT;
             if (TILE % NT != 0 && e >= TILE) break;
             const uint32_t g = tile_word<M, C>(base, s, e);
             tile[lpad(e)] = g < src_n ? src[g] : 0u;
@@ -310,13 +314,13 @@ __device__ __forceinline__ void tile_load(uint32_t* tile, const uint32_t* src, u
 }
 template <bool FWD, int M, int C>
 __device__ __forceinline__ void tile_store(const uint32_t* tile, uint32_t* dst, uint32_t base, uint32_t s,
-                                           uint32_t scale, bool vec) {
+                                           uint32_t scale, bool vec, uint32_t tid) {
     constexpr uint32_t TILE = 1u << (M + C);
     constexpr uint32_t NT = ntt_threads<M, C>();
     if (vec) {
 #pragma unroll
         for (uint32_t it = 0; it < (TILE + 4 * NT - 1) / (4 * NT); it++) {
-            const uint32_t e = 4 * threadIdx.x + it * 4 * NT;
+            const uint32_t e = 4 * tid + it * 4 * NT;
             if (TILE % (4 * NT) != 0 && e >= TILE) break;
             const uint32_t g = tile_word<M, C>(base, s, e);
             const uint32_t p = lpad(e);
@@ -327,7 +331,7 @@ __device__ __forceinline__ void tile_store(const uint32_t* tile, uint32_t* dst, 
     } else {
 #pragma unroll
         for (uint32_t it = 0; it < (TILE + NT - 1) / NT; it++) {
-            const uint32_t e = threadIdx.x + it * NT;
+            const uint32_t e = tid + it * NT;
             if (TILE % NT != 0 && e >= TILE) break;
             const uint32_t g = tile_word<M, C>(base, s, e);
             const uint32_t v = tile[lpad(e)];
@@ -353,24 +357,66 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
     constexpr bool VEC_OK = (C >= 2 || C == 0) && TILE >= 4;
     const bool vec = VEC_OK && a.vec;
 
-    tile_load<FWD, M, C>(tile, src, base, s, a.src_n, vec);
-    __syncthreads();
     if constexpr (REPL) {
+        // With vector moves each thread keeps its part of the quarter's tile in
+        // registers and puts it back into LDS for the next quarter (reloading it
+        // from memory would miss an L2 that the whole grid's tiles overflow);
+        // word moves reload it.
+        constexpr uint32_t NT = ntt_threads<M, C>();
+        constexpr uint32_t KV = (TILE + 4 * NT - 1) / (4 * NT);
+        uint4 keep[KV];
+        auto put = [&](uint32_t tid) {
+#pragma unroll
+            for (uint32_t it = 0; it < KV; it++) {
+                const uint32_t e = 4 * tid + it * 4 * NT;
+                if (TILE % (4 * NT) != 0 && e >= TILE) break;
+                const uint32_t p = lpad(e);
+                tile[p] = keep[it].x;
+                tile[p + 1] = keep[it].y;
+                tile[p + 2] = keep[it].z;
+                tile[p + 3] = keep[it].w;
+            }
+        };
+        if (vec) {
+#pragma unroll
+            for (uint32_t it = 0; it < KV; it++) {
+                const uint32_t e = 4 * threadIdx.x + it * 4 * NT;
+                if (TILE % (4 * NT) != 0 && e >= TILE) break;
+                const uint32_t g = tile_word<M, C>(base, s, e);
+                keep[it] = g < a.src_n ? *(const uint4*)(src + g) : make_uint4(0, 0, 0, 0);
+            }
+            put(threadIdx.x);
+        } else {
+            tile_load<FWD, M, C>(tile, src, base, s, a.src_n, false, threadIdx.x);
+        }
+        __syncthreads();
+#pragma unroll 1
         for (int q = 3; q >= 0; q--) {
+            // Addresses within the tile do not depend on the quarter: left to
+            // itself the compiler computes all of them once, before the loop,
+            // and holds them in registers (hundreds for the largest tiles); a
+            // thread index it cannot see through keeps them inside the loop.
+            uint32_t tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
             if (q != 3) {
-                tile_load<FWD, M, C>(tile, src, base, s, a.src_n, vec);
+                if (vec)
+                    put(tid);
+                else
+                    tile_load<FWD, M, C>(tile, src, base, s, a.src_n, false, tid);
                 __syncthreads();
             }
-            fwd_rounds<FWD, M, C, STWO, M>(tile, a, (uint64_t)q);
-            tile_store<FWD, M, C>(tile, dst, base + ((uint32_t)q << (s + M)), s, 0u, vec);
-            __syncthreads();  // the next quarter's load overwrites the tile
+            fwd_rounds<FWD, M, C, STWO, M>(tile, a, (uint64_t)q, tid);
+            tile_store<FWD, M, C>(tile, dst, base + ((uint32_t)q << (s + M)), s, 0u, vec, tid);
+            __syncthreads();  // the next quarter overwrites the tile
         }
     } else {
+        tile_load<FWD, M, C>(tile, src, base, s, a.src_n, vec, threadIdx.x);
+        __syncthreads();
         if constexpr (FWD)
-            fwd_rounds<FWD, M, C, STWO, M>(tile, a, hi);
+            fwd_rounds<FWD, M, C, STWO, M>(tile, a, hi, threadIdx.x);
         else
-            inv_rounds<FWD, M, C, STWO, 0>(tile, a, hi);
-        tile_store<FWD, M, C>(tile, dst, base, s, a.scale, vec);
+            inv_rounds<FWD, M, C, STWO, 0>(tile, a, hi, threadIdx.x);
+        tile_store<FWD, M, C>(tile, dst, base, s, a.scale, vec, threadIdx.x);
     }
 }
 
