@@ -289,6 +289,11 @@ void vx_ntt_plan_reset(void);
 int32_t vx_ntt_lde_plan_set(uint32_t log_n, uint32_t n_passes, const uint32_t* m, const uint32_t* c,
                             const uint32_t* r);
 uint32_t vx_ntt_lde_plan_get(uint32_t log_n, uint32_t* m, uint32_t* c, uint32_t* r);
+/* The provers' LDE runs the interpolation's top pass and the evaluation's
+ * replicating top pass as one bridge pass when the two cover the same tile
+ * (the quarter's coefficients then never go to memory); enable = 0 runs them
+ * as two passes (same results). On by default. No reference counterpart. */
+void vx_ntt_set_lde_bridge(int32_t enable);
 
 /* Chunk depth of the Merkle stage kernels (no reference counterpart): a wave
  * hashes 2^c chunks of 128 items before its 64 -> 1 reduction. The depth is

@@ -152,3 +152,66 @@ def test_wide_commit_with_replicating_lde_plans(vx, reset_plans):
         for c, l in enumerate(logs):
             assert np.array_equal(w.lde(c), O.stwo_lde(cols[c], l, blowup)), (plans, c)
         w.close()
+
+
+# LDE bridge: the evaluation plan's replicating top pass over layers [s, k)
+# takes the interpolation's layers [s, k - 2) too; with the general plan of the
+# interpolation, its passes below s are used when they end at s, else the
+# default split of those layers (trace log, general plan, blowup-4 plan)
+BRIDGE_PLANS = [
+    (11, [(11, 0)], [(13, 0, 1)]),                                   # one contiguous bridge pass
+    (11, [(7, 0), (4, 4)], [(7, 0, 0), (6, 4, 1)]),
+    (14, [(9, 0), (5, 5)], [(9, 0, 0), (7, 5, 1)]),
+    (14, [(12, 0), (2, 4)], [(8, 0, 0), (8, 6, 1)]),                 # no general prefix ends at 8
+    (18, [(10, 0), (4, 4), (4, 4)], [(10, 0, 0), (4, 4, 0), (6, 4, 1)]),
+]
+
+
+@pytest.mark.parametrize("bridge", [True, False])
+def test_lde_bridge_is_bit_exact(vx, reset_plans, bridge):
+    vx.lib.vx_ntt_set_lde_bridge(1 if bridge else 0)
+    try:
+        expect = {}
+        for kt, gen, lde in BRIDGE_PLANS:
+            vx.ntt_plan_set(kt, gen)
+            vx.ntt_lde_plan_set(kt + 2, lde)
+            if kt not in expect:
+                expect[kt] = (O.prove(5, 7, kt, 80, 3), O.lde(O.fibonacci_trace(5, 7, kt), kt),
+                              [O.prove(a, b, kt, 80, 3) for a, b in ((11, 13), (17, 19))] if kt <= 14 else None)
+            proof, lde_words, batch = expect[kt]
+            p = vx.Prover(kt, 80, 3)
+            assert p.prove(5, 7) == proof, (kt, gen, lde)
+            assert np.array_equal(p.eval_words(), lde_words), (kt, gen, lde)
+            p.close()
+            if batch is not None:
+                bp = vx.Prover(kt, 80, 3, capacity=4)
+                got = bp.prove_batch([(5, 7), (11, 13), (17, 19)])
+                assert got[0] == proof and got[1:] == batch, (kt, gen, lde)
+                bp.close()
+    finally:
+        vx.lib.vx_ntt_set_lde_bridge(1)
+
+
+@pytest.mark.parametrize("bridge", [True, False])
+def test_wide_commit_with_lde_bridge(vx, reset_plans, bridge):
+    """stwo basis through the bridge: contiguous (2^10 LDE) and strided tiles"""
+    logs, blowup, last = [8, 8, 11, 14], 2, 3
+    cols = inputs.m31_columns(33, logs)
+    root, fri, lastv, beta = O.wide_commit(cols, logs, blowup, last)
+    vx.lib.vx_ntt_set_lde_bridge(1 if bridge else 0)
+    try:
+        for kt, gen, lde in [(8, [(8, 0)], [(10, 0, 1)]), (11, [(6, 0), (5, 5)], [(6, 0, 0), (7, 5, 1)]),
+                             (14, [(8, 0), (6, 6)], [(8, 0, 0), (8, 6, 1)])]:
+            vx.ntt_plan_set(kt, gen)
+            vx.ntt_lde_plan_set(kt + 2, lde)
+        w = vx.WideCommit(logs, blowup, last)
+        bufs = [vx.DeviceBuffer.from_host(c) for c in cols]
+        w.commit_device([b.addr for b in bufs])
+        g_root, g_fri, g_last, g_beta = w.results()
+        assert np.array_equal(g_root, root) and np.array_equal(g_fri, fri)
+        assert np.array_equal(g_last, lastv) and np.array_equal(g_beta, beta)
+        for c, l in enumerate(logs):
+            assert np.array_equal(w.lde(c), O.stwo_lde(cols[c], l, blowup)), c
+        w.close()
+    finally:
+        vx.lib.vx_ntt_set_lde_bridge(1)

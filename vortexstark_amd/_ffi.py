@@ -101,6 +101,7 @@ SIGNATURES = {
     "vx_ntt_plan_reset": (None, []),
     "vx_ntt_lde_plan_set": (i32, [u32, u32, vp, vp, vp]),
     "vx_ntt_lde_plan_get": (u32, [u32, vp, vp, vp]),
+    "vx_ntt_set_lde_bridge": (None, [i32]),
     "vx_get_stream": (vp, []),
     # Part 3: fused prover
     "vx_prover_create": (vp, [u32, u32, u32]),

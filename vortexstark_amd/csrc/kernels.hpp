@@ -1,5 +1,6 @@
 // Internal (C++) interface between the kernel files and the prover.
 #pragma once
+#include <functional>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -38,6 +39,14 @@ void ntt_forward_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables
                        uint64_t src_n = 0, bool aligned = false);
 void ntt_inverse_batch(uint32_t* const* d_cols, uint32_t n_cols, const NttTables& t, hipStream_t s,
                        const uint32_t* const* d_srcs = nullptr, bool aligned = false);
+// Blowup-4 LDE of n_cols columns: interpolation of 2^ti.log_n points (from
+// srcs, or in place) into the first quarter of each column, then evaluation of
+// 2^te.log_n = 4 * 2^ti.log_n points in place (top passes fused when the plans
+// allow).
+// mark("interpolate") / mark("evaluate") follow the two parts when given.
+void lde_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& ti, const NttTables& te, hipStream_t s,
+               const uint32_t* const* srcs, bool src_aligned,
+               const std::function<void(const char*)>& mark = nullptr);
 
 // Builds the twiddles of coset (initial, step) of size 2^log_n directly on the
 // device: line (2^log_n - 1 words), circle (2^(log_n-1) words) and, when the

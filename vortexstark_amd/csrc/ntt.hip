@@ -18,6 +18,7 @@
 // derives its twiddle for pair t from the first line layer as
 // [y, -y, -x, x][t % 4] with (x, y) = (T[2(t/4)], T[2(t/4) + 1]).
 #include <algorithm>
+#include <functional>
 #include <mutex>
 
 #include "common.hpp"
@@ -81,6 +82,8 @@ struct NttPassArgs {
     // layer, which tw[0] then points at). Set on the host, so the kernel reads
     // one pointer per layer instead of looking offsets up.
     const uint32_t* tw[NTT_LOW_MAX];
+    // the LDE bridge's inverse layers (the interpolation's tables), row bit rb
+    const uint32_t* tw2[NTT_LOW_MAX];
     uint64_t src_n;  // words of src that are read; the rest is zero
     uint32_t s;      // lowest global stride bit of this pass
     uint32_t k;      // log2 of the transform size
@@ -136,7 +139,8 @@ __device__ __forceinline__ uint32_t ntt_twiddle(const uint32_t* T, bool circle0,
 // zero past its first quarter), the values of the group's two upper row bits
 // are zero on entry, so the top two layers only copy: (x, 0) -> (x, x). Those
 // rounds load two values and replicate them instead of running 8 butterflies.
-template <bool FWD, int M, int C, int R, int J, bool STWO>
+// ALT: the layers' twiddle tables are a.tw2 (the LDE bridge's inverse layers)
+template <bool FWD, int M, int C, int R, int J, bool STWO, bool ALT = false>
 __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, uint64_t hi, uint32_t tid,
                                           bool zero_top = false) {
     constexpr uint32_t GROUPS = 1u << (M + C - R);
@@ -187,7 +191,7 @@ __device__ __forceinline__ void ntt_round(uint32_t* tile, const NttPassArgs& a, 
             const int t = FWD ? (R - 1 - tt) : tt;  // layer inside the group
             if (FWD && R == 3 && t >= 1 && zero_top) continue;
             const uint32_t rb = J + t;                 // row bit of the layer
-            const uint32_t* T = a.tw[rb];
+            const uint32_t* T = ALT ? a.tw2[rb] : a.tw[rb];
             // only the contiguous pass (stride bits from 0) has a stwo variant,
             // so the circle layer is row bit 0, known at compile time
             const bool circle0 = STWO && rb == 0;
@@ -255,13 +259,13 @@ __device__ __forceinline__ void fwd_rounds(uint32_t* tile, const NttPassArgs& a,
         fwd_rounds<FWD, M, C, STWO, TOP - R>(tile, a, hi, tid);
     }
 }
-template <bool FWD, int M, int C, bool STWO, int LO>
+template <bool FWD, int M, int C, bool STWO, int LO, bool ALT = false>
 __device__ __forceinline__ void inv_rounds(uint32_t* tile, const NttPassArgs& a, uint64_t hi, uint32_t tid) {
     if constexpr (LO < M) {
         constexpr int R = M - LO >= 3 ? 3 : M - LO;
-        ntt_round<FWD, M, C, R, LO, STWO>(tile, a, hi, tid);
+        ntt_round<FWD, M, C, R, LO, STWO, ALT>(tile, a, hi, tid);
         __syncthreads();
-        inv_rounds<FWD, M, C, STWO, LO + R>(tile, a, hi, tid);
+        inv_rounds<FWD, M, C, STWO, LO + R, ALT>(tile, a, hi, tid);
     }
 }
 
@@ -420,6 +424,62 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(NttPassArgs a) {
     }
 }
 
+// The bridge of a blowup-4 LDE. When the interpolation's top pass and the
+// evaluation's replicating top pass cover the same tile (rows s .. s+M-1 of
+// the first quarter, 2^C columns), one pass runs both: the tile of
+// interpolation input is loaded, its M inverse layers run with the
+// interpolation's twiddles (tw2), the coefficients are scaled by the
+// interpolation's 1/n into registers, and each quarter q runs the M forward
+// layers with the evaluation's twiddles (tw, row bits above the tile = q)
+// and is stored, as in the replicating pass. The quarter's coefficients never
+// go to memory. Vector moves only (an LDE's column buffers are 16-byte
+// aligned); the driver uses the two passes otherwise.
+template <int M, int C, bool STWO>
+__global__ __launch_bounds__(256) void ntt_lde_bridge_kernel(NttPassArgs a) {
+    static_assert(C >= 2 || C == 0, "vector moves need rows of at least 4 words");
+    constexpr uint32_t TILE = 1u << (M + C);
+    constexpr uint32_t NT = ntt_threads<M, C>();
+    constexpr uint32_t KV = (TILE + 4 * NT - 1) / (4 * NT);
+    __shared__ uint32_t tile[TILE + TILE / 8];
+    uint32_t* dst = a.cols ? a.cols[blockIdx.y] : a.dst;
+    const uint32_t* src = a.cols ? (a.srcs ? a.srcs[blockIdx.y] : a.cols[blockIdx.y]) : a.src;
+    const uint32_t s = a.s;
+    const uint32_t mid = blockIdx.x;  // the grid holds no row bits above the tile
+    const uint32_t base = mid << C;
+    tile_load<false, M, C>(tile, src, base, s, a.src_n, true, threadIdx.x);
+    __syncthreads();
+    inv_rounds<false, M, C, STWO, 0, true>(tile, a, 0, threadIdx.x);
+    uint4 keep[KV];
+#pragma unroll
+    for (uint32_t it = 0; it < KV; it++) {
+        const uint32_t e = 4 * threadIdx.x + it * 4 * NT;
+        if (TILE % (4 * NT) != 0 && e >= TILE) break;
+        const uint32_t p = lpad(e);
+        uint4 x = make_uint4(tile[p], tile[p + 1], tile[p + 2], tile[p + 3]);
+        if (a.scale) x = make_uint4(mmul(x.x, a.scale), mmul(x.y, a.scale), mmul(x.z, a.scale), mmul(x.w, a.scale));
+        keep[it] = x;
+    }
+#pragma unroll 1
+    for (int q = 3; q >= 0; q--) {
+        uint32_t tid = threadIdx.x;  // hidden from the compiler, as in the replicating pass
+        asm volatile("" : "+v"(tid));
+#pragma unroll
+        for (uint32_t it = 0; it < KV; it++) {
+            const uint32_t e = 4 * tid + it * 4 * NT;
+            if (TILE % (4 * NT) != 0 && e >= TILE) break;
+            const uint32_t p = lpad(e);
+            tile[p] = keep[it].x;
+            tile[p + 1] = keep[it].y;
+            tile[p + 2] = keep[it].z;
+            tile[p + 3] = keep[it].w;
+        }
+        __syncthreads();
+        fwd_rounds<true, M, C, STWO, M>(tile, a, (uint64_t)q, tid);
+        tile_store<true, M, C>(tile, dst, base + ((uint32_t)q << (s + M)), s, 0u, true, tid);
+        __syncthreads();  // the next quarter overwrites the tile
+    }
+}
+
 namespace {
 std::mutex g_plan_mu;
 std::vector<Pass> g_plan[32];      // tuned or set plans; empty: default split
@@ -496,12 +556,24 @@ static void launch_pass(bool fwd, bool stwo, const NttPassArgs& a, const Pass& p
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
+// Twiddle table of layer b (stride 2^b) of a transform.
+static const uint32_t* layer_table(const NttTables& t, uint32_t b) {
+    return b ? t.line + t.offs[b - 1] : (t.stwo ? t.line : t.circle);
+}
+// 1/n of an inverse transform of 2^k points, 0 when it is 1 (no scaling).
+static uint32_t inverse_scale(uint32_t k) {
+    const uint32_t ex = (30u * k) % 31u;
+    return ex == 0 ? 0u : (1u << ex);
+}
+
+// Runs the passes ps (covering some or all layers of a 2^t.log_n transform:
+// all of them, or for the LDE bridge the passes below its top pass, whose
+// inverse then leaves the 1/n scaling to the bridge: last_scale = false).
 static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n, uint32_t* const* cols,
                          const uint32_t* const* srcs, uint32_t n_cols, const NttTables& t, hipStream_t st,
-                         const std::vector<Pass>& ps, bool vec) {
+                         const std::vector<Pass>& ps, bool vec, bool last_scale = true) {
     uint32_t k = t.log_n;
-    uint32_t ex = (30u * k) % 31u;
-    uint32_t inv_n = ex == 0 ? 1u : (1u << ex);
+    const uint32_t inv_n = inverse_scale(k);
     for (size_t i = 0; i < ps.size(); i++) {
         size_t pi = fwd ? ps.size() - 1 - i : i;
         NttPassArgs a = {};
@@ -510,14 +582,11 @@ static void run_ntt_plan(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t 
         a.dst = dst;
         a.src = i == 0 ? src : dst;
         a.src_n = i == 0 ? src_n : (1ull << k);
-        for (uint32_t rb = 0; rb < ps[pi].m && rb < NTT_LOW_MAX; rb++) {
-            const uint32_t b = ps[pi].s + rb;
-            a.tw[rb] = b ? t.line + t.offs[b - 1] : (t.stwo ? t.line : t.circle);
-        }
+        for (uint32_t rb = 0; rb < ps[pi].m && rb < NTT_LOW_MAX; rb++) a.tw[rb] = layer_table(t, ps[pi].s + rb);
         a.s = ps[pi].s;
         a.k = k;
         a.vec = vec ? 1u : 0u;
-        a.scale = (!fwd && i + 1 == ps.size() && inv_n != 1u) ? inv_n : 0u;
+        a.scale = (!fwd && last_scale && i + 1 == ps.size()) ? inv_n : 0u;
         // the top pass of an evaluation of n/4 coefficients: its two top layers copy
         const bool quarter = fwd && i == 0 && ps[pi].s + ps[pi].m == k && k >= 2 && src_n == (1ull << k) / 4;
         if (ps[pi].r && !quarter) {
@@ -536,6 +605,98 @@ static void run_ntt(bool fwd, uint32_t* dst, const uint32_t* src, uint64_t src_n
     const bool lde = fwd && t.log_n >= 2 && src_n == (1ull << t.log_n) / 4;
     run_ntt_plan(fwd, dst, src, src_n, cols, srcs, n_cols, t, st, lde ? lde_plan_passes(t.log_n) : plan_passes(t.log_n),
                  vec);
+}
+
+namespace {
+bool g_lde_bridge = true;
+}
+template <int C, int M, bool STWO>
+static void dispatch_bridge(uint32_t m, const NttPassArgs& a, uint32_t ke, uint32_t n_cols, hipStream_t st) {
+    if constexpr (M >= 1) {
+        if (m == (uint32_t)M) {
+            dim3 grid((uint32_t)(1ull << (ke - 2 - M - C)), n_cols);
+            ntt_lde_bridge_kernel<M, C, STWO><<<grid, ntt_threads<M, C>(), 0, st>>>(a);
+            return;
+        }
+        dispatch_bridge<C, M - 1, STWO>(m, a, ke, n_cols, st);
+    } else {
+        set_error(-2, "ntt: unsupported bridge shape");
+    }
+}
+
+// A blowup-4 LDE of columns: interpolate 2^kt points (from srcs, or in place)
+// into the first quarter of each column, evaluate 2^(kt+2) points in place.
+// When the evaluation plan ends in a replicating pass over layers [s, kt+2),
+// the interpolation's layers [s, kt) run in the same pass (the bridge): the
+// interpolation takes its plan's passes below s if they end at s, else the
+// default split of those layers.
+void lde_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& ti, const NttTables& te, hipStream_t st,
+               const uint32_t* const* srcs, bool src_aligned, const std::function<void(const char*)>& mark) {
+    auto marked = [&](const char* what) {
+        if (mark) mark(what);
+    };
+    const uint32_t kt = ti.log_n, ke = te.log_n;
+    const std::vector<Pass> pe = ke >= 3 ? lde_plan_passes(ke) : std::vector<Pass>{};
+    bool bridge = false;
+    {
+        std::lock_guard<std::mutex> g(g_plan_mu);
+        bridge = g_lde_bridge;
+    }
+    bridge = bridge && n_cols && kt && ke == kt + 2 && ti.stwo == te.stwo && !pe.empty() && pe.back().r == 1 &&
+             (pe.back().c == 0 || pe.back().c >= 2) && (pe.back().s > 0 || src_aligned);
+    if (!bridge) {
+        if (n_cols && kt) {
+            ntt_inverse_batch(cols, n_cols, ti, st, srcs, src_aligned);
+            marked("interpolate");
+            ntt_forward_batch(cols, n_cols, te, st, 1ull << kt, true);
+            marked("evaluate");
+        }
+        return;
+    }
+    const Pass& top = pe.back();
+    std::vector<Pass> pil;
+    {
+        const std::vector<Pass> pi = plan_passes(kt);
+        uint32_t covered = 0;
+        for (const Pass& x : pi) {
+            if (covered + x.m > top.s) break;
+            pil.push_back(x);
+            covered += x.m;
+        }
+        if (covered != top.s) pil = default_plan(top.s);
+    }
+    const std::vector<Pass> pel(pe.begin(), pe.end() - 1);
+    if (!pil.empty()) run_ntt_plan(false, nullptr, nullptr, 1ull << kt, cols, srcs, n_cols, ti, st, pil, src_aligned, false);
+    marked("interpolate");  // with the bridge: the interpolation's layers below it
+    const Pass& pt = top;
+    NttPassArgs a = {};
+    a.cols = cols;
+    a.srcs = pil.empty() ? srcs : nullptr;
+    for (uint32_t rb = 0; rb + 2 < pt.m && rb < NTT_LOW_MAX; rb++) {
+        a.tw[rb] = layer_table(te, pt.s + rb);
+        a.tw2[rb] = layer_table(ti, pt.s + rb);
+    }
+    a.src_n = 1ull << kt;
+    a.s = pt.s;
+    a.k = ke;
+    a.vec = 1u;
+    a.scale = inverse_scale(kt);
+    const uint32_t rows = pt.m - 2;
+    if (pt.c == 0) {
+        if (te.stwo)
+            dispatch_bridge<0, NTT_LOW_MAX, true>(rows, a, ke, n_cols, st);
+        else
+            dispatch_bridge<0, NTT_LOW_MAX, false>(rows, a, ke, n_cols, st);
+    } else if (pt.c == 4) {
+        dispatch_bridge<4, NTT_HIGH_MAX4, false>(rows, a, ke, n_cols, st);
+    } else if (pt.c == 6) {
+        dispatch_bridge<6, NTT_HIGH_MAX6, false>(rows, a, ke, n_cols, st);
+    } else {
+        dispatch_bridge<5, NTT_HIGH_MAX5, false>(rows, a, ke, n_cols, st);
+    }
+    check_launch("ntt_lde_bridge_kernel");
+    if (!pel.empty()) run_ntt_plan(true, nullptr, nullptr, 1ull << ke, cols, nullptr, n_cols, te, st, pel, true);
+    marked("evaluate");  // with the bridge: the bridge and the evaluation's lower passes
 }
 
 void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTables& t, hipStream_t s) {
@@ -858,6 +1019,11 @@ void vx_ntt_plan_reset(void) {
     std::lock_guard<std::mutex> g(g_plan_mu);
     for (auto& p : g_plan) p.clear();
     for (auto& p : g_lde_plan) p.clear();
+}
+
+void vx_ntt_set_lde_bridge(int32_t enable) {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    g_lde_bridge = enable != 0;
 }
 
 int32_t vx_ntt_lde_plan_set(uint32_t log_n, uint32_t n_passes, const uint32_t* m, const uint32_t* c,

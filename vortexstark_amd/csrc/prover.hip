@@ -527,16 +527,14 @@ void plan_arena(vx_prover* p) {
 // trees and the circle fold with its tree.
 void enqueue_heavy(vx_prover* p, uint32_t count) {
     hipStream_t s = p->stream;
-    const uint64_t N = p->N, E = p->E, R = p->R;
+    const uint64_t E = p->E, R = p->R;
     p->mark("start");
     std::string phase;
     auto mk = [&](const char* what) { p->mark(phase + ":" + what); };
     // LDE: interpolate on half_coset(log_n) into the first N words of the LDE
     // buffer, then evaluate on half_coset(log_n + 2) in place (zero pad implicit)
-    ntt_inverse_batch(p->d_eval_ptrs, count, p->trace_tab, s, (const uint32_t* const*)p->d_call, p->traces_aligned);
-    p->mark("lde:interpolate");
-    ntt_forward_batch(p->d_eval_ptrs, count, p->eval_tab, s, N, true);
-    p->mark("lde:evaluate");
+    lde_batch(p->d_eval_ptrs, count, p->trace_tab, p->eval_tab, s, (const uint32_t* const*)p->d_call,
+              p->traces_aligned, [&](const char* what) { p->mark(std::string("lde:") + what); });
     Scratch sc = {p->at(p->o_scr_a), p->at(p->o_scr_b)};
     uint32_t* pr = p->proof();
     uint32_t* ev = p->at(p->o_eval);

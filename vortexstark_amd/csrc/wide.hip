@@ -447,9 +447,13 @@ int32_t vx_wide_commit_device(vx_wide* w, const uint32_t* const* h_cols) {
         const auto& g = w->groups[ng - 1 - gi];  // groups ascend in size: the largest first
         hipStream_t gs = gi == 0 || !ns ? s : w->side[(gi - 1) % ns];
         const NttTables ti = stwo_tables(w->tw_inv + (tc - (1ull << (g.log - 1))), g.log);
-        ntt_inverse_batch(w->d_lde_ptrs + g.first, g.count, ti, gs, w->d_src_ptrs + g.first, w->src_aligned);
         const NttTables te = stwo_tables(w->tw_fwd + (tc - (1ull << (g.log + B - 1))), g.log + B);
-        ntt_forward_batch(w->d_lde_ptrs + g.first, g.count, te, gs, 1ull << g.log, true);
+        if (B == 2) {
+            lde_batch(w->d_lde_ptrs + g.first, g.count, ti, te, gs, w->d_src_ptrs + g.first, w->src_aligned);
+        } else {
+            ntt_inverse_batch(w->d_lde_ptrs + g.first, g.count, ti, gs, w->d_src_ptrs + g.first, w->src_aligned);
+            ntt_forward_batch(w->d_lde_ptrs + g.first, g.count, te, gs, 1ull << g.log, true);
+        }
     }
     for (size_t k = 0; k < ns; k++)
         if (!check(hipEventRecord(w->side_done[k], w->side[k]), "hipEventRecord") ||
