@@ -630,25 +630,28 @@ static void dispatch_bridge(uint32_t m, const NttPassArgs& a, uint32_t ke, uint3
 // the interpolation's layers [s, kt) run in the same pass (the bridge): the
 // interpolation takes its plan's passes below s if they end at s, else the
 // default split of those layers.
-void lde_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& ti, const NttTables& te, hipStream_t st,
-               const uint32_t* const* srcs, bool src_aligned, const std::function<void(const char*)>& mark) {
+static void lde_run(uint32_t* const* cols, uint32_t n_cols, const NttTables& ti, const NttTables& te, hipStream_t st,
+                    const uint32_t* const* srcs, bool src_aligned, const std::vector<Pass>& pe,
+                    const std::function<void(const char*)>& mark) {
     auto marked = [&](const char* what) {
         if (mark) mark(what);
     };
     const uint32_t kt = ti.log_n, ke = te.log_n;
-    const std::vector<Pass> pe = ke >= 3 ? lde_plan_passes(ke) : std::vector<Pass>{};
     bool bridge = false;
     {
         std::lock_guard<std::mutex> g(g_plan_mu);
         bridge = g_lde_bridge;
     }
-    bridge = bridge && n_cols && kt && ke == kt + 2 && ti.stwo == te.stwo && !pe.empty() && pe.back().r == 1 &&
+    bridge = bridge && n_cols && kt && ke == kt + 2 && ke >= 3 && ti.stwo == te.stwo && !pe.empty() && pe.back().r == 1 &&
              (pe.back().c == 0 || pe.back().c >= 2) && (pe.back().s > 0 || src_aligned);
     if (!bridge) {
         if (n_cols && kt) {
             ntt_inverse_batch(cols, n_cols, ti, st, srcs, src_aligned);
             marked("interpolate");
-            ntt_forward_batch(cols, n_cols, te, st, 1ull << kt, true);
+            if (ke == kt + 2 && !pe.empty())
+                run_ntt_plan(true, nullptr, nullptr, 1ull << kt, cols, nullptr, n_cols, te, st, pe, true);
+            else
+                ntt_forward_batch(cols, n_cols, te, st, 1ull << kt, true);
             marked("evaluate");
         }
         return;
@@ -697,6 +700,11 @@ void lde_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& ti, cons
     check_launch("ntt_lde_bridge_kernel");
     if (!pel.empty()) run_ntt_plan(true, nullptr, nullptr, 1ull << ke, cols, nullptr, n_cols, te, st, pel, true);
     marked("evaluate");  // with the bridge: the bridge and the evaluation's lower passes
+}
+void lde_batch(uint32_t* const* cols, uint32_t n_cols, const NttTables& ti, const NttTables& te, hipStream_t st,
+               const uint32_t* const* srcs, bool src_aligned, const std::function<void(const char*)>& mark) {
+    lde_run(cols, n_cols, ti, te, st, srcs, src_aligned, te.log_n >= 3 ? lde_plan_passes(te.log_n) : std::vector<Pass>{},
+            mark);
 }
 
 void ntt_forward(uint32_t* dst, const uint32_t* src, uint64_t src_n, const NttTables& t, hipStream_t s) {
@@ -1069,12 +1077,14 @@ int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms) {
     constexpr uint32_t LDE_LOG_WORDS = 26, LDE_MAX_COLS = 1024;
     const uint64_t words = std::max<uint64_t>(N, 1ull << LDE_LOG_WORDS);
     hipStream_t st = nullptr;
-    uint32_t *data = nullptr, *line = nullptr, *circle = nullptr;
+    uint32_t *data = nullptr, *line = nullptr, *circle = nullptr, *line2 = nullptr, *circle2 = nullptr;
     uint32_t** d_cols = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool ok = check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate") &&
               check(hipMalloc((void**)&data, 4 * words), "hipMalloc") &&
               check(hipMalloc((void**)&d_cols, sizeof(uint32_t*) * LDE_MAX_COLS), "hipMalloc") &&
+              check(hipMalloc((void**)&line2, 4 * N / 4), "hipMalloc") &&
+              check(hipMalloc((void**)&circle2, 4 * N / 8 + 4), "hipMalloc") &&
               check(hipMalloc((void**)&line, 4 * N), "hipMalloc") &&
               check(hipMalloc((void**)&circle, 4 * N / 2), "hipMalloc") && check(hipEventCreate(&e0), "hipEvent") &&
               check(hipEventCreate(&e1), "hipEvent");
@@ -1092,17 +1102,29 @@ int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms) {
         for (uint32_t c = 0; c < n_cols; c++) h_cols[c] = data + ((uint64_t)c << k);
         ok = ok && check(hipMemcpyAsync(d_cols, h_cols.data(), sizeof(uint32_t*) * n_cols, hipMemcpyHostToDevice, st),
                          "hipMemcpyAsync") && check(hipStreamSynchronize(st), "sync");
+        // tables of a 2^(k-2) transform for the LDE's interpolation (any
+        // canonical words time the same; these are the forward ones of that size)
+        NttTables ti;
+        if (k >= 3) {
+            HostPt hi2, hs2;
+            host_coset(1, k - 2, hi2, hs2);
+            ti.line = line2;
+            ti.circle = circle2;
+            ti.log_n = k - 2;
+            build_ntt_tables(CPt{hi2.x, hi2.y}, CPt{hs2.x, hs2.y}, k - 2, line2, circle2, nullptr, nullptr, ti.offs, st);
+        }
         // the general table on one full input, then the blowup-4 table on a batch
-        // of inputs that fill their first quarter (the LDE evaluations)
+        // of LDEs (interpolation with the tuned plan of 2^(k-2), evaluation with
+        // the candidate, bridged when the candidate allows)
         for (int lde = 0; lde < 2 && ok; lde++) {
             if (lde && k < 3) break;
-            const uint64_t src_n = lde ? (1ull << k) / 4 : 1ull << k;
+            const uint64_t src_n = 1ull << k;
             const auto cands = lde ? lde_plan_candidates(k) : plan_candidates(k);
             float best = 1e30f;
             size_t bi = 0;
             auto run = [&](const std::vector<Pass>& ps) {
                 if (lde)
-                    run_ntt_plan(true, nullptr, nullptr, src_n, d_cols, nullptr, n_cols, t, st, ps, true);
+                    lde_run(d_cols, n_cols, ti, t, st, nullptr, true, ps, nullptr);
                 else
                     run_ntt_plan(true, data, data, src_n, nullptr, nullptr, 1, t, st, ps, true);
             };
@@ -1131,6 +1153,8 @@ int32_t vx_ntt_autotune(uint32_t log_min, uint32_t log_max, float* best_ms) {
     if (st) (void)hipStreamSynchronize(st);
     if (data) (void)hipFree(data);
     if (d_cols) (void)hipFree(d_cols);
+    if (line2) (void)hipFree(line2);
+    if (circle2) (void)hipFree(circle2);
     if (line) (void)hipFree(line);
     if (circle) (void)hipFree(circle);
     if (st) (void)hipStreamDestroy(st);
