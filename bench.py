@@ -277,24 +277,36 @@ def bench_plain(vx, args) -> dict:
     dom_ms = phases.get("quotient_commit:stage_leaf")
     roofline = valu_roofline("merkle_stage_kernel<SrcQuotient>", stage_compressions(E, 1), dom_ms) if dom_ms else None
     ntt_ms = phases.get("lde:evaluate")
+    # the LDE's evaluation plan; when it ends in a replicating pass the library
+    # runs the interpolation's layers above the pass's start in that pass too
+    # (the LDE bridge), so the evaluate phase also holds those layers
+    lde_plan = vx.ntt_lde_plan(log_n + 2)
+    bridged = bool(lde_plan) and lde_plan[-1][2] == 1
+    bridged_layers = log_n - sum(m for m, _, _ in lde_plan[:-1]) if bridged else 0
+    what = "evaluate 2^%d, all passes" % (log_n + 2)
+    if bridged:
+        what += " + the interpolation's top %d layers (LDE bridge)" % bridged_layers
+    # counted traffic belongs to the plan it was measured with
+    lde_traffic = TRAFFIC.get("lde:evaluate")
+    if lde_traffic and [list(x) for x in lde_traffic.get("plan", [])] != [list(x) for x in lde_plan]:
+        lde_traffic = None
     roofline_hbm = None
     if ntt_ms:
-        byts = 4 * (1 << log_n) + 4 * E  # read coefficients once, write the LDE once
+        byts = 4 * (1 << log_n) + 4 * E  # read the quarter once, write the LDE once
         achieved = byts / (ntt_ms * 1e-3) / 1e9
-        roofline_hbm = {"bound": "hbm", "kernel": "ntt_pass_kernel<true> (evaluate 2^%d, all passes)" % (log_n + 2),
+        roofline_hbm = {"bound": "hbm", "kernel": "ntt_pass_kernel<true> (%s)" % what,
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get("lde:evaluate"),
-                        "phase_ms": round(ntt_ms, 4),
-                        "passes": (MEASURED.get("ntt_plans", {}).get(str(log_n + 2)) or {}).get("lde_passes")}
-    # the same evaluate priced against the VALU: its butterflies are the work
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": lde_traffic,
+                        "phase_ms": round(ntt_ms, 4), "passes": lde_plan}
+    # the same phase priced against the VALU: its butterflies are the work
     # (the top two layers of a zero-padded input only copy and are not counted)
     roofline_ntt = None
     if ntt_ms:
         k = log_n + 2
-        bfly = (k - 2) * (E // 2)
+        bfly = (k - 2) * (E // 2) + bridged_layers * ((1 << log_n) // 2)
         ops = bfly * OPS_PER_BUTTERFLY
         achieved = ops / (ntt_ms * 1e-3) / 1e12
-        roofline_ntt = {"bound": "valu", "kernel": "ntt_pass_kernel<true> (evaluate 2^%d, all passes)" % k,
+        roofline_ntt = {"bound": "valu", "kernel": "ntt_pass_kernel<true> (%s)" % what,
                         "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS, "unit": "Tops/s (int32 lane-ops)",
                         "frac": round(achieved / VALU_PEAK_TOPS, 4), "butterflies": bfly,
                         "ops_per_butterfly": OPS_PER_BUTTERFLY, "phase_ms": round(ntt_ms, 4)}
@@ -946,6 +958,9 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=4.0, help="offered load, jobs/s")
     ap.add_argument("--no-tune", action="store_true", help="keep the default NTT pass split")
+    ap.add_argument("--lde-plan", action="append", default=[], metavar="K:m,c,r/m,c,r...",
+                    help="pin the blowup-4 evaluation plan of 2^K points after tuning (repeatable), e.g. "
+                         "26:12,0,0/6,6,0/8,6,1 (counter runs that must see one plan)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -976,6 +991,11 @@ def main() -> int:
                                           for k, (ms, c) in vx.merkle_autotune(12, 27).items()}
         log(f"merkle stage autotune 2^12..2^27: {time.time() - t:.1f}s "
             f"{ {k: v['logc'] for k, v in MEASURED['merkle_stage_plans'].items()} }")
+    for spec in args.lde_plan:
+        k, passes = spec.split(":")
+        plan = [tuple(int(x) for x in p.split(",")) for p in passes.split("/")]
+        vx.ntt_lde_plan_set(int(k), plan)
+        log(f"blowup-4 plan of 2^{k} pinned: {vx.ntt_lde_plan(int(k))}")
     extra = {}
     if "plain" in skip:  # profiling runs of the other workloads only
         calibration_copy(vx, args)
