@@ -287,9 +287,8 @@ def bench_plain(vx, args) -> dict:
     if bridged:
         what += " + the interpolation's top %d layers (LDE bridge)" % bridged_layers
     # counted traffic belongs to the plan it was measured with
-    lde_traffic = TRAFFIC.get("lde:evaluate")
-    if lde_traffic and [list(x) for x in lde_traffic.get("plan", [])] != [list(x) for x in lde_plan]:
-        lde_traffic = None
+    lde_traffic = next((t for t in TRAFFIC.get("lde:evaluate", [])
+                        if [list(x) for x in t.get("plan", [])] == [list(x) for x in lde_plan]), None)
     roofline_hbm = None
     if ntt_ms:
         byts = 4 * (1 << log_n) + 4 * E  # read the quarter once, write the LDE once
