@@ -71,6 +71,28 @@ def test_graph_replays_match_eager(vx):
     bp.close()
 
 
+@pytest.mark.parametrize("lde_plan", [[(11, 0, 1)], [(6, 0, 0), (5, 5, 1)]])
+def test_graph_replays_with_lde_bridge(vx, lde_plan):
+    """the bridged LDE (one contiguous bridge pass, or a strided one above a
+    contiguous pass) captured in a graph and replayed with eager work between
+    replays; every replay gives the oracle's proofs"""
+    log_n, cap = 9, 4
+    vx.ntt_lde_plan_set(log_n + 2, lde_plan)
+    try:
+        bp = vx.Prover(log_n, 80, 3, capacity=cap)
+        bp.set_graph(True)
+        scratch = vx.DeviceBuffer(1 << 12)
+        ab = pairs(7, cap)
+        want = [O.prove(a, b, log_n, 80, 3) for a, b in ab]
+        for rep in range(3):
+            assert bp.prove_batch(ab) == want, rep
+            vx.lib.vx_memset(scratch.ptr, rep, 4 << 12)
+            vx.synchronize()
+        bp.close()
+    finally:
+        vx.lib.vx_ntt_plan_reset()
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("log_n,nq,last,cap", [(10, 80, 3, 4), (14, 64, 4, 2), (18, 100, 4, 1), (20, 80, 3, 3)])
 def test_device_bytes_equal_estimate(vx, log_n, nq, last, cap):
