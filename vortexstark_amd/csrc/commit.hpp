@@ -19,10 +19,11 @@ struct Scratch {
 
 // Commits n_leaves (power of two >= 2) items of a leaf source, for `batch`
 // proofs at once whose buffers are bs words apart (bs = 0 for one proof).
+// Trees of at most direct_max leaves go straight to the tail kernel.
 template <class Src, class Mark>
 void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, Scratch sc, hipStream_t s,
-                 Mark&& mark, uint32_t batch = 1, uint64_t bs = 0) {
-    if (n_leaves <= PROVER_TAIL_ITEMS) {
+                 Mark&& mark, uint32_t batch = 1, uint64_t bs = 0, uint64_t direct_max = PROVER_TAIL_ITEMS) {
+    if (n_leaves <= direct_max) {
         merkle_tail_kernel<Src><<<dim3(1, batch), TAIL_THREADS, 0, s>>>(src, 0, (uint32_t)n_leaves, st, out, bs);
         check_launch("merkle_tail_kernel");
         mark("tail");

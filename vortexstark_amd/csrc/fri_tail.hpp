@@ -1,16 +1,17 @@
 // The small FRI layers of a proof in one launch.
 //
-// Up to 2048 values a layer's fold, leaf hashes, tree and transcript step fit
+// Up to 1024 values a layer's fold, leaf hashes, tree and transcript step fit
 // one workgroup, and launching kernels per layer costs more than the layer
 // itself. fri_tail_kernel runs every remaining line fold of a proof in one
 // workgroup of 1024 threads: fold (cuda/fri.cu:10-39) into the next layer,
 // hash it as an SoA4 tree (level-store nodes written as by the stage/tail
-// kernels; a layer of up to 1024 values with one leaf per thread, the largest
-// with two leaves and their parent per thread; levels wider than the block's
-// 256 quads with one compression per thread, the rest quad-cooperatively
-// across the whole block), reduce to the
-// root, mix the root into the transcript and draw the next challenge, which
-// stays in LDS for the next layer (and is also written to device memory).
+// kernels; one leaf per thread; levels wider than the block's 256 quads with
+// one compression per thread, the rest quad-cooperatively across the whole
+// block), reduce to the root, mix the root into the transcript and draw the
+// next challenge, which stays in LDS for the next layer (and is also written
+// to device memory). A 2048-value layer is left to a stage launch and a tree
+// tail: its 4095 compressions in one workgroup are bound by one CU's issue
+// rate (about 34 us, against about 20 us for a stage over 16 CUs and a tail).
 // blockIdx.y selects the proof of a batch (buffers bs words apart).
 #pragma once
 #include "merkle_engine.hpp"
@@ -18,8 +19,9 @@
 namespace vx {
 
 constexpr uint32_t FRI_TAIL_MAX_LAYERS = 12;
-constexpr uint32_t FRI_TAIL_MAX_ITEMS = 2048;  // values of the largest layer handled here
+constexpr uint32_t FRI_TAIL_MAX_ITEMS = 1024;  // values of the largest layer handled here
 constexpr uint32_t FRI_TAIL_THREADS = 1024;
+static_assert(FRI_TAIL_MAX_ITEMS <= FRI_TAIL_THREADS, "one leaf per thread");
 
 struct FriTailLayer {
     const uint32_t* in[4];   // previous layer, 2n values per column
@@ -38,10 +40,9 @@ struct FriTailArgs {
 };
 
 static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTailArgs args, uint64_t bs) {
-    constexpr uint32_t W = FRI_TAIL_MAX_ITEMS / 2;  // nodes of the widest level
     constexpr uint32_t QUADS = FRI_TAIL_THREADS / 4;
-    __shared__ uint4 bufa[2 * W];      // node-major: node j in bufa[2j], bufa[2j + 1]
-    __shared__ uint4 bufb[W];          // half of that
+    __shared__ uint4 bufa[2 * FRI_TAIL_MAX_ITEMS];  // node-major: node j in bufa[2j], bufa[2j + 1]
+    __shared__ uint4 bufb[FRI_TAIL_MAX_ITEMS];      // half of that
     __shared__ uint32_t msg[16];
     __shared__ uint32_t al[4];
     const uint32_t tid = threadIdx.x;
@@ -53,57 +54,31 @@ static __global__ __launch_bounds__(FRI_TAIL_THREADS) void fri_tail_kernel(FriTa
         const FriTailLayer& L = args.L[li];
         const LevelStore st = L.st.shifted(boff);
         const QM31 alpha = {{al[0], al[1], al[2], al[3]}};
-        const uint32_t half = L.n / 2;
         const uint32_t* i0 = L.in[0] + boff;
         const uint32_t* i1 = L.in[1] + boff;
         const uint32_t* i2 = L.in[2] + boff;
         const uint32_t* i3 = L.in[3] + boff;
         uint32_t* cur = (uint32_t*)bufa;
         uint32_t* nxt = (uint32_t*)bufb;
-        uint32_t m = half, lvl = 1;
-        if (L.n <= FRI_TAIL_THREADS) {
-            // one leaf per thread, each a fold of two values; their parents are
-            // the first of the parallel levels below (a thread hashing two
-            // leaves and their parent in turn would leave most lanes idle)
-            const uint32_t k = tid;
-            if (k < L.n) {
-                const uint32_t x = 2 * k;
-                const uint2 a = *(const uint2*)(i0 + x), b = *(const uint2*)(i1 + x);
-                const uint2 c = *(const uint2*)(i2 + x), d = *(const uint2*)(i3 + x);
-                const QM31 f = fold_pair({{a.x, b.x, c.x, d.x}}, {{a.y, b.y, c.y, d.y}}, alpha, L.tw[k]);
-                L.out[0][boff + k] = f.v[0];
-                L.out[1][boff + k] = f.v[1];
-                L.out[2][boff + k] = f.v[2];
-                L.out[3][boff + k] = f.v[3];
-                uint32_t h[8];
-                b2s_leaf4(f.v[0], f.v[1], f.v[2], f.v[3], h);
-                bufa[2 * k] = make_uint4(h[0], h[1], h[2], h[3]);
-                bufa[2 * k + 1] = make_uint4(h[4], h[5], h[6], h[7]);
-            }
-            m = L.n;
-            lvl = 0;
-        }
-        // level 1 of the largest layers: parents of leaf pairs (2j, 2j+1)
-        for (uint32_t j = tid; L.n > FRI_TAIL_THREADS && j < half; j += FRI_TAIL_THREADS) {
-            const uint32_t k = 2 * j;
+        // one leaf per thread, each a fold of two values; their parents are the
+        // first of the parallel levels below (a thread hashing two leaves and
+        // their parent in turn would leave most lanes idle)
+        const uint32_t k = tid;
+        if (k < L.n) {
             const uint32_t x = 2 * k;
-            const uint4 a = *(const uint4*)(i0 + x), b = *(const uint4*)(i1 + x);
-            const uint4 c = *(const uint4*)(i2 + x), d = *(const uint4*)(i3 + x);
-            const uint2 w = *(const uint2*)(L.tw + k);
-            const QM31 f0 = fold_pair({{a.x, b.x, c.x, d.x}}, {{a.y, b.y, c.y, d.y}}, alpha, w.x);
-            const QM31 f1 = fold_pair({{a.z, b.z, c.z, d.z}}, {{a.w, b.w, c.w, d.w}}, alpha, w.y);
-            *(uint2*)(L.out[0] + boff + k) = make_uint2(f0.v[0], f1.v[0]);
-            *(uint2*)(L.out[1] + boff + k) = make_uint2(f0.v[1], f1.v[1]);
-            *(uint2*)(L.out[2] + boff + k) = make_uint2(f0.v[2], f1.v[2]);
-            *(uint2*)(L.out[3] + boff + k) = make_uint2(f0.v[3], f1.v[3]);
-            uint32_t l[8], r[8], h[8];
-            b2s_leaf4(f0.v[0], f0.v[1], f0.v[2], f0.v[3], l);
-            b2s_leaf4(f1.v[0], f1.v[1], f1.v[2], f1.v[3], r);
-            b2s_node(l, r, h);
-            if (st.want(1)) store8(st.at(1, j), h);
-            bufa[2 * j] = make_uint4(h[0], h[1], h[2], h[3]);
-            bufa[2 * j + 1] = make_uint4(h[4], h[5], h[6], h[7]);
+            const uint2 a = *(const uint2*)(i0 + x), b = *(const uint2*)(i1 + x);
+            const uint2 c = *(const uint2*)(i2 + x), d = *(const uint2*)(i3 + x);
+            const QM31 f = fold_pair({{a.x, b.x, c.x, d.x}}, {{a.y, b.y, c.y, d.y}}, alpha, L.tw[k]);
+            L.out[0][boff + k] = f.v[0];
+            L.out[1][boff + k] = f.v[1];
+            L.out[2][boff + k] = f.v[2];
+            L.out[3][boff + k] = f.v[3];
+            uint32_t h[8];
+            b2s_leaf4(f.v[0], f.v[1], f.v[2], f.v[3], h);
+            bufa[2 * k] = make_uint4(h[0], h[1], h[2], h[3]);
+            bufa[2 * k + 1] = make_uint4(h[4], h[5], h[6], h[7]);
         }
+        uint32_t m = L.n, lvl = 0;
         __syncthreads();
         // wide levels (more compressions than quads): one per thread, all lanes busy
         while (m > 2 * QUADS) {
@@ -147,7 +122,13 @@ void launch_fri_tail(uint32_t j0, uint32_t J, LayerFn&& layer, AlphaFn&& alpha, 
     for (uint32_t j = j0; j < J; j += FRI_TAIL_MAX_LAYERS) {
         FriTailArgs a = {};
         a.n_layers = J - j < FRI_TAIL_MAX_LAYERS ? J - j : FRI_TAIL_MAX_LAYERS;
-        for (uint32_t i = 0; i < a.n_layers; i++) a.L[i] = layer(j + i);
+        for (uint32_t i = 0; i < a.n_layers; i++) {
+            a.L[i] = layer(j + i);
+            if (a.L[i].n < 2 || a.L[i].n > FRI_TAIL_MAX_ITEMS || (a.L[i].n & (a.L[i].n - 1))) {
+                set_error(-2, "fri_tail: layer size out of range");
+                return;
+            }
+        }
         a.alpha0 = alpha(j);
         a.channel = ch;
         fri_tail_kernel<<<dim3(1, batch), FRI_TAIL_THREADS, 0, s>>>(a, bs);

@@ -561,7 +561,9 @@ void enqueue_light(vx_prover* p, uint32_t count) {
     auto mk = [&](const char* what) { p->mark(phase + ":" + what); };
     Scratch sc = {p->at(p->o_scr_a), p->at(p->o_scr_b)};
     uint32_t* pr = p->proof();
-    // layers of at most FRI_TAIL_MAX_ITEMS values all go to one fri_tail_kernel launch
+    // layers of at most FRI_TAIL_MAX_ITEMS values all go to one fri_tail_kernel
+    // launch; the larger ones take a stage launch even where the tree would fit
+    // one tail block (direct_max), so that their leaves spread over many CUs
     uint32_t js = 1;
     while (js < p->J && (E >> (js + 1)) > FRI_TAIL_MAX_ITEMS) js++;
     for (uint32_t j = 1; j < js; j++) {
@@ -570,7 +572,7 @@ void enqueue_light(vx_prover* p, uint32_t count) {
                            p->fri_tw[j],     p->col(j, 0),     p->col(j, 1),     p->col(j, 2),     p->col(j, 3)};
         uint32_t* nxt = j + 1 < p->J ? p->alpha_f(j + 1) : nullptr;
         commit_tree(src, E >> (j + 1), p->stores[2 + j], TailOut{pr + p->off_fri_roots + 8ull * j, p->ch(), nxt, 0},
-                    sc, s, mk, count, R);
+                    sc, s, mk, count, R, FRI_TAIL_MAX_ITEMS);
     }
     if (js < p->J) {
         auto layer = [&](uint32_t j) {

@@ -476,14 +476,15 @@ int32_t vx_wide_commit_device(vx_wide* w, const uint32_t* const* h_cols) {
         uint32_t* nxt = w->J > 1 ? w->chal + 8 : nullptr;
         commit_tree(src, w->layer_size(0), none, TailOut{w->roots + 8, w->ch, nxt, 0}, w->scratch, s, mk);
     }
-    uint32_t js = 1;
+    uint32_t js = 1;  // as in the plain prover: larger layers get a stage launch each
     while (js < w->J && w->layer_size(js) > FRI_TAIL_MAX_ITEMS) js++;
     for (uint32_t j = 1; j < js; j++) {
         phase = "fri" + std::to_string(j) + "_line_fold_commit";
         SrcLineFold src = {w->col(j - 1, 0), w->col(j - 1, 1), w->col(j - 1, 2), w->col(j - 1, 3), w->chal + 4 * (j + 1),
                            w->tw_line[j],    w->col(j, 0),     w->col(j, 1),     w->col(j, 2),     w->col(j, 3)};
         uint32_t* nxt = j + 1 < w->J ? w->chal + 4 * (j + 2) : nullptr;
-        commit_tree(src, w->layer_size(j), none, TailOut{w->roots + 8 * (j + 1), w->ch, nxt, 0}, w->scratch, s, mk);
+        commit_tree(src, w->layer_size(j), none, TailOut{w->roots + 8 * (j + 1), w->ch, nxt, 0}, w->scratch, s, mk,
+                    1, 0, FRI_TAIL_MAX_ITEMS);
     }
     if (js < w->J) {
         auto layer = [&](uint32_t j) {
