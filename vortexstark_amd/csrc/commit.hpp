@@ -60,6 +60,45 @@ void commit_tree(const Src& src, uint64_t n_leaves, LevelStore st, TailOut out, 
     mark("tail");
 }
 
+// Line-fold layers of at most LINE_QUAD_MAX values (and more than the FRI tail
+// kernel takes) of a single proof commit through line_fold_quad_stage_kernel:
+// 16 leaves a wave up to 2^13 values, 32 at 2^14, so the tail starts from at
+// most 512 nodes. Such a layer's chain is latency-bound for one proof; a batch
+// of proofs fills the device with its waves, where the lane-per-pair stage
+// does the same hashing in fewer instructions (about 980 a compression
+// against 4 x 410), so batches keep it.
+constexpr uint64_t LINE_QUAD_MAX = 1ull << 14;
+inline uint32_t line_quad_leaves(uint64_t n) { return n <= (1ull << 13) ? 16u : 32u; }
+// Stage output (nodes per proof) of the quad line-fold stages of the layers
+// E/4, E/8, ... that take it: 0 if none does.
+inline uint64_t max_line_quad_nodes(uint64_t e) {
+    uint64_t mx = 0;
+    for (uint64_t n = e / 4; n >= 2; n /= 2)
+        if (n <= LINE_QUAD_MAX && n >= 2048) mx = std::max<uint64_t>(mx, n / line_quad_leaves(n));
+    return mx;
+}
+
+// Commits a line-fold layer of n values (2048 <= n <= LINE_QUAD_MAX, a power
+// of two) with the quad leaf stage and the tail kernel.
+template <class Mark>
+void commit_line_fold_quad(const SrcLineFold& src, uint64_t n, LevelStore st, TailOut out, Scratch sc, hipStream_t s,
+                           Mark&& mark, uint32_t batch = 1, uint64_t bs = 0) {
+    const uint32_t lpw = line_quad_leaves(n);
+    const uint64_t waves = n / lpw;
+    const dim3 grid((uint32_t)ceil_div(waves, LINE_QUAD_WAVES), batch);
+    if (lpw == 16)
+        line_fold_quad_stage_kernel<1><<<grid, 64 * LINE_QUAD_WAVES, 0, s>>>(src, n, sc.a, st, bs);
+    else
+        line_fold_quad_stage_kernel<2><<<grid, 64 * LINE_QUAD_WAVES, 0, s>>>(src, n, sc.a, st, bs);
+    check_launch("line_fold_quad_stage_kernel");
+    mark("stage_leaf");
+    const uint32_t lvl = lpw == 16 ? 4u : 5u;
+    merkle_tail_kernel<SrcNodes><<<dim3(1, batch), TAIL_THREADS, 0, s>>>(SrcNodes{sc.a}, lvl, (uint32_t)waves, st, out,
+                                                                         bs);
+    check_launch("merkle_tail_kernel<nodes>");
+    mark("tail");
+}
+
 // Largest stage output of a tree of n_leaves items (per proof), for sizing
 // the stage scratch buffers.
 inline uint64_t max_stage_nodes(uint64_t n_leaves) { return max_stage_nodes(n_leaves, PROVER_TAIL_ITEMS); }

@@ -676,6 +676,58 @@ __global__ __launch_bounds__(64 * MERKLE_STAGE_WAVES) void merkle_stage_kernel(S
     for (uint32_t k = lane; k < 8 * keep; k += 64) out_nodes[8 * keep * wave_id + k] = top[k];
 }
 
+// ---------------------------------------------------------------- small line folds
+// Leaf stage of a small line-fold layer (fri.cu:10-39 and the SoA4 leaf rule):
+// one leaf per quad, hashed quad-cooperatively, 16 * R leaves a wave reduced to
+// one node. On the few waves such a layer has (one per SIMD), a wave's chain is
+// then one quad compression for the leaves and log2(16 R) node levels, where
+// the lane-per-pair form (128 leaves a wave) takes three lane compressions and
+// seven quad rounds. Value i of the layer: every lane of its quad folds it,
+// lane l stores and hashes coordinate l.
+constexpr uint32_t LINE_QUAD_WAVES = 4;  // waves per workgroup
+template <uint32_t R>
+__global__ __launch_bounds__(64 * LINE_QUAD_WAVES) void line_fold_quad_stage_kernel(SrcLineFold src0, uint64_t n_items,
+                                                                                  uint32_t* out_nodes, LevelStore st0,
+                                                                                  uint64_t bs) {
+    constexpr uint32_t LPW = 16 * R;  // leaves a wave
+    __shared__ uint32_t lds[LINE_QUAD_WAVES][2 * 8 * LPW + 16 * 16];
+    const uint64_t boff = (uint64_t)blockIdx.y * bs;
+    const SrcLineFold src = src0.at(boff);
+    const LevelStore st = st0.shifted(boff);
+    out_nodes += boff;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6, quad = lane >> 2, l = lane & 3;
+    const uint64_t wave_id = (uint64_t)blockIdx.x * LINE_QUAD_WAVES + w;
+    const uint64_t base = wave_id * LPW;
+    if (base >= n_items) return;  // wave-uniform
+    uint32_t* na = lds[w];                         // LPW nodes, node-major
+    uint32_t* nb = na + 8 * LPW;                   // the next level
+    uint32_t* msg = nb + 8 * LPW + 16 * quad;      // the quad's leaf message
+    const QM31 al = load_alpha(src.alpha);
+    uint32_t* out = l == 0 ? src.o0 : l == 1 ? src.o1 : l == 2 ? src.o2 : src.o3;
+    for (uint32_t r = 0; r < R; r++) {
+        const uint64_t i = base + 16 * r + quad;
+        const uint2 a = *(const uint2*)(src.i0 + 2 * i), b = *(const uint2*)(src.i1 + 2 * i);
+        const uint2 c = *(const uint2*)(src.i2 + 2 * i), d = *(const uint2*)(src.i3 + 2 * i);
+        const QM31 f = fold_pair({{a.x, b.x, c.x, d.x}}, {{a.y, b.y, c.y, d.y}}, al, src.tw[i]);
+        const uint32_t fl = l == 0 ? f.v[0] : l == 1 ? f.v[1] : l == 2 ? f.v[2] : f.v[3];
+        out[i] = fl;
+        msg[l] = fl;
+        msg[4 + l] = 0;
+        msg[8 + l] = 0;
+        msg[12 + l] = 0;
+        VX_WAVE_SYNC();
+        uint32_t hl, hh;
+        b2s_quad_init(hl, hh);
+        b2s_compress_quad(msg, 16u, 0xFFFFFFFFu, hl, hh);
+        const uint32_t k = 16 * r + quad;
+        na[8 * k + l] = hl;
+        na[8 * k + 4 + l] = hh;
+        VX_WAVE_SYNC();  // the message is rewritten next round; the leaves are read below
+    }
+    const uint32_t* top = narrow_reduce(na, nb, LPW, 0, base, st);
+    if (lane < 8) out_nodes[8 * wave_id + lane] = top[lane];
+}
+
 // ---------------------------------------------------------------- tail kernel
 struct TailOut {
     uint32_t* root_dst;      // 8 words (e.g. inside the proof buffer); may be null

@@ -512,7 +512,7 @@ void plan_arena(vx_prover* p) {
         const uint32_t d = depth_of(t);
         p->o_store.push_back(take(level_store_words(d, store_from(d))));
     }
-    const uint64_t mx = max_stage_nodes(p->E);
+    const uint64_t mx = std::max(max_stage_nodes(p->E), max_line_quad_nodes(p->E));
     p->o_scr_a = take(8 * (mx ? mx : 1));
     p->o_scr_b = take(8 * (mx ? mx : 1));
     p->o_ch = take(sizeof(DevChannel) / 4);
@@ -571,8 +571,11 @@ void enqueue_light(vx_prover* p, uint32_t count) {
         SrcLineFold src = {p->col(j - 1, 0), p->col(j - 1, 1), p->col(j - 1, 2), p->col(j - 1, 3), p->alpha_f(j),
                            p->fri_tw[j],     p->col(j, 0),     p->col(j, 1),     p->col(j, 2),     p->col(j, 3)};
         uint32_t* nxt = j + 1 < p->J ? p->alpha_f(j + 1) : nullptr;
-        commit_tree(src, E >> (j + 1), p->stores[2 + j], TailOut{pr + p->off_fri_roots + 8ull * j, p->ch(), nxt, 0},
-                    sc, s, mk, count, R, FRI_TAIL_MAX_ITEMS);
+        const TailOut to{pr + p->off_fri_roots + 8ull * j, p->ch(), nxt, 0};
+        if (count == 1 && (E >> (j + 1)) <= LINE_QUAD_MAX)
+            commit_line_fold_quad(src, E >> (j + 1), p->stores[2 + j], to, sc, s, mk, count, R);
+        else
+            commit_tree(src, E >> (j + 1), p->stores[2 + j], to, sc, s, mk, count, R, FRI_TAIL_MAX_ITEMS);
     }
     if (js < p->J) {
         auto layer = [&](uint32_t j) {

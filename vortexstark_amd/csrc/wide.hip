@@ -376,7 +376,7 @@ vx_wide* vx_wide_create(uint32_t n_cols, const uint32_t* log_sizes, uint32_t log
     ok = ok && check(hipHostMalloc((void**)&w->h_src_ptrs, 8ull * n_cols), "hipHostMalloc");
     w->layer.assign(w->J, nullptr);
     for (uint32_t j = 0; ok && j < w->J; j++) ok = w->alloc((void**)&w->layer[j], 16 * w->layer_size(j));
-    const uint64_t mxn = max_stage_nodes(1ull << S);
+    const uint64_t mxn = std::max(max_stage_nodes(1ull << S), max_line_quad_nodes(1ull << (S + 1)));
     ok = ok && w->alloc((void**)&w->scratch.a, 32 * (mxn ? mxn : 1)) &&
          w->alloc((void**)&w->scratch.b, 32 * (mxn ? mxn : 1));
     ok = ok && w->alloc((void**)&w->ch, sizeof(DevChannel)) && w->alloc((void**)&w->chal, 16ull * (w->J + 1)) &&
@@ -483,8 +483,11 @@ int32_t vx_wide_commit_device(vx_wide* w, const uint32_t* const* h_cols) {
         SrcLineFold src = {w->col(j - 1, 0), w->col(j - 1, 1), w->col(j - 1, 2), w->col(j - 1, 3), w->chal + 4 * (j + 1),
                            w->tw_line[j],    w->col(j, 0),     w->col(j, 1),     w->col(j, 2),     w->col(j, 3)};
         uint32_t* nxt = j + 1 < w->J ? w->chal + 4 * (j + 2) : nullptr;
-        commit_tree(src, w->layer_size(j), none, TailOut{w->roots + 8 * (j + 1), w->ch, nxt, 0}, w->scratch, s, mk,
-                    1, 0, FRI_TAIL_MAX_ITEMS);
+        const TailOut to{w->roots + 8 * (j + 1), w->ch, nxt, 0};
+        if (w->layer_size(j) <= LINE_QUAD_MAX)
+            commit_line_fold_quad(src, w->layer_size(j), none, to, w->scratch, s, mk);
+        else
+            commit_tree(src, w->layer_size(j), none, to, w->scratch, s, mk, 1, 0, FRI_TAIL_MAX_ITEMS);
     }
     if (js < w->J) {
         auto layer = [&](uint32_t j) {
