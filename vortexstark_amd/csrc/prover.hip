@@ -81,6 +81,7 @@ __global__ __launch_bounds__(256) void queries_kernel(DevChannel* ch0, const uin
                                                      const uint32_t* ab, uint64_t bs) {
     __shared__ uint32_t words[MAX_LAST_WORDS];
     __shared__ uint32_t st[8];
+    __shared__ uint32_t msg[16];
     const uint32_t tid = threadIdx.x;
     const uint64_t off = (uint64_t)blockIdx.x * bs;
     DevChannel* ch = (DevChannel*)((uint32_t*)ch0 + off);
@@ -94,12 +95,36 @@ __global__ __launch_bounds__(256) void queries_kernel(DevChannel* ch0, const uin
         }
     }
     __syncthreads();
-    if (tid == 0) {
-        ch_mix_felts(ch, words, 4 * n_last);
-        for (int c = 0; c < 8; c++) st[c] = ch->state[c];
-        ch->n_draws = n_q;
-        proof[3] = ab[2 * blockIdx.x];
-        proof[4] = ab[2 * blockIdx.x + 1];
+    if (tid < 64) {
+        // mix_felts (channel.rs:48-61: Blake2s over state || felts in 64-byte
+        // blocks) by wave 0 quad-cooperatively: every quad hashes the same
+        // blocks, quad 0's chaining value is kept
+        const uint32_t l = tid & 3;
+        const uint32_t total_words = 8 + 4 * n_last, total_bytes = 4 * total_words;
+        const uint32_t nb = (total_bytes + 63) / 64;
+        uint32_t hl, hh;
+        b2s_quad_init(hl, hh);
+        for (uint32_t b = 0; b < nb; b++) {
+            if (tid < 16) {
+                const uint32_t gi = 16 * b + tid;
+                msg[tid] = gi < 8 ? ch->state[gi] : (gi < total_words ? words[gi - 8] : 0u);
+            }
+            VX_WAVE_SYNC();
+            const bool last = b + 1 == nb;
+            b2s_compress_quad(msg, last ? total_bytes : 64u * (b + 1), last ? 0xFFFFFFFFu : 0u, hl, hh);
+            VX_WAVE_SYNC();  // every lane has read the block before the next is written
+        }
+        if (tid < 4) {
+            st[l] = hl;
+            st[4 + l] = hh;
+        }
+        VX_WAVE_SYNC();
+        if (tid < 8) ch->state[tid] = st[tid];
+        if (tid == 0) {
+            ch->n_draws = n_q;
+            proof[3] = ab[2 * blockIdx.x];
+            proof[4] = ab[2 * blockIdx.x + 1];
+        }
     }
     __syncthreads();
     uint32_t s[8];
